@@ -1,0 +1,222 @@
+/* C ABI of the gfx950 CORE-SG library (libcoresg_hip.so).
+ *
+ * Plain C types only: POD structs, raw pointers and sizes. Functions that can
+ * fail return 1 on success and 0 on failure; the message of the most recent
+ * failure on the calling thread is coresg_hip_last_error(). A message that
+ * contains "out of memory" marks a device allocation failure (the host
+ * side treats it as the reference does: record it, optionally fall back).
+ *
+ * Two groups of entry points:
+ *
+ *  1. Drop-in replacements for the reference's device ABI
+ *     (include/coresg/cuda_api.hpp), same arguments and meaning, host
+ *     pointers in and out. A `Backend` implementation calls these exactly
+ *     where the reference's CUDA backend calls the cuda:: functions.
+ *
+ *  2. A device-resident CORE-SG (`coresg_hip_graph`): points, k-NN lists,
+ *     bridges and the deduplicated edge list stay in HBM across calls, so a
+ *     build and a whole mpts sweep run without host round trips. These extend
+ *     the reference boundary, which only covers the k-NN descent and the MRD
+ *     block; each names the reference host function whose result it
+ *     reproduces.
+ */
+#ifndef CORESG_HIP_H
+#define CORESG_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CORESG_HIP_ABI_VERSION 1
+
+/* Metric codes; replaces coresg::cuda::MetricCode (cuda_api.hpp:32-35). */
+enum { CORESG_HIP_METRIC_L2 = 0, CORESG_HIP_METRIC_COSINE = 1 };
+
+/* Replaces coresg::cuda::DeviceProps (cuda_api.hpp:17-25). `major`/`minor`
+ * carry the gfx major/minor (gfx950 -> 9, 5). */
+typedef struct coresg_hip_device_props {
+    int id;
+    char name[256];
+    char arch[64];
+    size_t total_mem;
+    size_t free_mem;
+    int cu_count;
+    int major;
+    int minor;
+} coresg_hip_device_props;
+
+/* Replaces coresg::cuda::NndConfig (cuda_api.hpp:40-51). */
+typedef struct coresg_hip_nnd_config {
+    uint32_t n;
+    uint32_t dim;
+    uint32_t stride;
+    uint32_t k;
+    uint32_t sample;
+    uint32_t iters;
+    uint64_t seed;
+    int metric;
+    float delta;
+    int random_fill;
+} coresg_hip_nnd_config;
+
+int coresg_hip_abi_version(void);
+
+/* ---- 1. reference device ABI --------------------------------------------- */
+
+/* cuda_api.hpp:53 (device_count). */
+int coresg_hip_device_count(void);
+/* cuda_api.hpp:54 (get_device_props). */
+int coresg_hip_get_device_props(int device, coresg_hip_device_props* out);
+/* cuda_api.hpp:57 (last_error). */
+const char* coresg_hip_last_error(void);
+/* cuda_api.hpp:62-64 (nndescent_gpu): Dynamic NN-Descent on `device`, host
+ * graph arrays refined in place (metric units), updates counted. */
+int coresg_hip_nndescent(int device, const coresg_hip_nnd_config* cfg, const float* host_vectors,
+                         uint32_t* host_ids, float* host_dists, uint8_t* host_flags,
+                         uint32_t* host_counts, uint64_t* out_updates);
+/* cuda_api.hpp:68-70 (mrd_block_gpu): rows [r0,r1) of the m x m
+ * mutual-reachability matrix into host_out ((r1-r0)*m floats). */
+int coresg_hip_mrd_block(int device, const float* host_pts, size_t m, uint32_t dim,
+                         uint32_t stride, int metric, const float* host_core, size_t r0,
+                         size_t r1, float* host_out);
+/* cuda_api.hpp:74-78 (mrd_row_min_gpu): per row in [r0,r1) the minimum MRD to
+ * a column in another component (host_component may be NULL: any j != i),
+ * first index on ties; +inf / 0xFFFFFFFF when none. */
+int coresg_hip_mrd_row_min(int device, const float* host_pts, size_t m, uint32_t dim,
+                           uint32_t stride, int metric, const float* host_core,
+                           const uint32_t* host_component, size_t r0, size_t r1,
+                           float* host_min, uint32_t* host_argmin);
+/* cuda_api.hpp:82 (enable_peer_access). One GPU per process here: a no-op
+ * that reports success for a single listed device. */
+int coresg_hip_enable_peer_access(const int* devices, size_t count);
+/* cuda_api.hpp:84 (device_memory). */
+int coresg_hip_device_memory(int device, size_t* free_bytes, size_t* total_bytes);
+/* cuda_api.hpp:86 (release_thread_cache): drop this thread's cached MRD
+ * operands. */
+void coresg_hip_release_thread_cache(void);
+
+/* ---- 2. device-resident CORE-SG ------------------------------------------ */
+
+typedef struct coresg_hip_graph coresg_hip_graph;
+
+/* Where a buffer argument lives. */
+enum { CORESG_HIP_HOST = 0, CORESG_HIP_DEVICE = 1 };
+
+/* Build parameters: the struct defaults of CoreSGParams / HubnessParams /
+ * NNDescentParams (coresg.hpp:63-106, hubness.hpp:72-119, nndescent.hpp:45-71)
+ * with the CLI's sample/iters (coresg_cli.cpp:154-158). */
+typedef struct coresg_hip_build_params {
+    uint32_t sample;
+    uint32_t iters;
+    double delta;
+    uint64_t seed;
+    uint32_t brute_force_limit;
+    double antihub_multiplier;
+    double max_pair_ops;
+    size_t min_selected;
+    size_t max_selected;
+    size_t min_per_component;
+    int ensure_connected;
+    size_t reps_per_component;
+    size_t max_reps;
+    /* local-join candidate routing (0 = library default) */
+    uint32_t join_keep;
+    uint32_t join_bucket;
+} coresg_hip_build_params;
+
+typedef struct coresg_hip_build_stats {
+    double knn_seconds;
+    double hubness_seconds;
+    double bridge_seconds;
+    double total_seconds;
+    uint32_t rounds;
+    uint64_t updates;
+    uint64_t knn_edges;
+    uint64_t antihubs_selected;
+    uint64_t bridge_edges;
+    uint64_t repair_edges;
+    uint32_t components_before_repair;
+    uint32_t components_after_repair;
+} coresg_hip_build_stats;
+
+/* One extracted MST (MstResult, mst.hpp:77-85). */
+typedef struct coresg_hip_mst_info {
+    uint32_t mpts;
+    uint32_t num_edges;
+    uint32_t num_components;
+    uint32_t rounds;
+    double weight;
+} coresg_hip_mst_info;
+
+void coresg_hip_default_build_params(coresg_hip_build_params* p);
+
+coresg_hip_graph* coresg_hip_graph_create(int device, uint32_t n, uint32_t dim, uint32_t k_max);
+void coresg_hip_graph_destroy(coresg_hip_graph* g);
+/* Rows: n x dim floats (padded=0) or n x stride floats (padded=1). */
+int coresg_hip_graph_set_points(coresg_hip_graph* g, const float* rows, int where, int padded);
+/* k-NN lists in the reference SoA layout (knn_graph.hpp:188-195). */
+int coresg_hip_graph_set_knn(coresg_hip_graph* g, const uint32_t* ids, const float* dist,
+                             const uint8_t* flags, const uint32_t* count, int where);
+int coresg_hip_graph_get_knn(coresg_hip_graph* g, uint32_t* ids, float* dist, uint8_t* flags,
+                             uint32_t* count);
+/* Bridge edges (CoreSG::missing_edges, coresg.hpp:153), base distances. */
+int coresg_hip_graph_set_bridges(coresg_hip_graph* g, const uint32_t* u, const uint32_t* v,
+                                 const float* w, size_t m, size_t repair_count);
+size_t coresg_hip_graph_get_bridges(coresg_hip_graph* g, uint32_t* u, uint32_t* v, float* w,
+                                    size_t cap, size_t* repair_count);
+/* Device pointers of the resident arrays (for callers that keep data in HBM). */
+int coresg_hip_graph_device_arrays(coresg_hip_graph* g, float** pts, uint32_t** ids,
+                                   float** dist, uint8_t** flags, uint32_t** count);
+size_t coresg_hip_graph_device_bytes(const coresg_hip_graph* g);
+
+/* k-NN half: init_random + Dynamic NN-Descent on device (replaces
+ * IncrementalKnn::build's single-instance path, incremental_knn.cpp:1012-1050,
+ * and CpuBackend::run_knn_job, cpu_backend.cpp:47-59), or exact brute force
+ * when n <= brute_force_limit (incremental_knn.cpp:997-1007). */
+int coresg_hip_build_knn(coresg_hip_graph* g, const coresg_hip_build_params* p,
+                         coresg_hip_build_stats* st);
+/* Hubness, anti-hub selection, bridge MST and repair (CoreSG::build after
+ * the k-NN half, coresg.cpp:278-305). */
+int coresg_hip_build_bridges(coresg_hip_graph* g, const coresg_hip_build_params* p,
+                             coresg_hip_build_stats* st);
+/* Both halves (CoreSG::build, coresg.cpp:260-323). */
+int coresg_hip_build(coresg_hip_graph* g, const coresg_hip_build_params* p,
+                     coresg_hip_build_stats* st);
+
+size_t coresg_hip_graph_get_selected(coresg_hip_graph* g, uint32_t* out, size_t cap);
+int coresg_hip_graph_get_hubness(coresg_hip_graph* g, int32_t* rdeg, double* score,
+                                 uint32_t* conn_labels, uint32_t* num_components);
+
+/* Deduplicated CORE-SG edge list (knn_graph.cpp:127-200 + bridges), built on
+ * device once and kept resident, sorted by (u, v). */
+int coresg_hip_prepare_edges(coresg_hip_graph* g, uint64_t* num_edges, uint64_t* knn_edges);
+/* CoreSG::edges(mpts) (coresg.cpp:391-406) in (u, v) order. */
+int coresg_hip_edges(coresg_hip_graph* g, uint32_t mpts, uint32_t* u, uint32_t* v, float* w);
+/* CoreSG::mst for each mpts (coresg.cpp:408-417; the MST is the unique one
+ * under edge_less, types.hpp:38-42, i.e. the reference's Kruskal tree).
+ * Outputs per mpts i: info[i]; when non-NULL, out_u/v/w + i*(n-1) receive
+ * the edges in (u, v) order and out_labels + i*n the dense component labels
+ * (first appearance, mst.cpp:48-59). */
+int coresg_hip_extract(coresg_hip_graph* g, const uint32_t* mpts, uint32_t count,
+                       coresg_hip_mst_info* info, uint32_t* out_u, uint32_t* out_v,
+                       float* out_w, uint32_t* out_labels);
+
+/* Sampled k-NN recall on device (eval.cpp:24-97; queries chosen on the
+ * host by the same mt19937_64 draw). */
+int coresg_hip_sampled_recall(coresg_hip_graph* g, size_t queries, uint64_t seed,
+                              double* recall, double* distance_ratio);
+
+/* Per-kernel device timing with HIP events recorded on the graph's own
+ * stream around each launch of the named kernels (off by default). */
+int coresg_hip_graph_set_profiling(coresg_hip_graph* g, int on);
+int coresg_hip_graph_kernel_times(coresg_hip_graph* g, const char* kernel, double* avg_ms,
+                                  uint64_t* launches);
+int coresg_hip_graph_reset_timers(coresg_hip_graph* g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,27 @@
+// Build-side entry points (k-NN descent, hubness, bridges) and the
+// reference-shaped host-pointer device calls.
+#pragma once
+
+#include "../../include/coresg_hip.h"
+#include "graph.hpp"
+
+namespace coresg_hip {
+
+void build_knn(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_stats* st);
+void build_bridges(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_stats* st);
+void sampled_recall(Graph& g, std::size_t queries, std::uint64_t seed, double* recall,
+                    double* ratio);
+
+void nndescent_host(int device, const coresg_hip_nnd_config& cfg, const float* vectors,
+                    std::uint32_t* ids, float* dists, std::uint8_t* flags, std::uint32_t* counts,
+                    std::uint64_t* updates);
+void mrd_block_host(int device, const float* pts, std::size_t m, std::uint32_t dim,
+                    std::uint32_t stride, int metric, const float* core, std::size_t r0,
+                    std::size_t r1, float* out);
+void mrd_row_min_host(int device, const float* pts, std::size_t m, std::uint32_t dim,
+                      std::uint32_t stride, int metric, const float* core,
+                      const std::uint32_t* component, std::size_t r0, std::size_t r1,
+                      float* out_min, std::uint32_t* out_argmin);
+void release_thread_cache();
+
+}  // namespace coresg_hip

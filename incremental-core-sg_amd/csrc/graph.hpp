@@ -1,0 +1,128 @@
+// Device-resident CORE-SG state shared by the build and extraction paths.
+#pragma once
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace coresg_hip {
+
+// Owning device allocation (hipMalloc / hipFree) with its element count.
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    std::size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    // Ensure capacity for `count` elements (contents are not preserved).
+    void reserve(std::size_t count) {
+        if (count <= n && p) return;
+        release();
+        const std::size_t bytes = (count ? count : 1) * sizeof(T);
+        CSG_CHECK(hipMalloc(reinterpret_cast<void**>(&p), bytes));
+        n = count;
+    }
+    std::size_t bytes() const { return n * sizeof(T); }
+};
+
+// One live edge of a Boruvka round: endpoints plus the packed
+// (weight bits << 32 | rank) key that totally orders edges as edge_less does.
+struct alignas(16) LiveEdge {
+    std::uint32_t u;
+    std::uint32_t v;
+    std::uint64_t key;
+};
+
+// Accumulated device time of one kernel (HIP events on the graph's stream).
+struct KernelTimer {
+    std::vector<hipEvent_t> pending;  // start/stop pairs not yet resolved
+    double total_ms = 0.0;
+    std::uint64_t launches = 0;
+};
+
+struct Graph {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::uint32_t n = 0, dim = 0, stride = 0, k = 0;
+
+    // Points and k-NN lists (reference SoA layout, metric units).
+    DBuf<float> pts;
+    DBuf<std::uint32_t> ids;
+    DBuf<float> dist;
+    DBuf<std::uint8_t> flags;
+    DBuf<std::uint32_t> count;
+    bool points_set = false, knn_set = false;
+
+    // Bridges (host copy is authoritative; device copy for the edge build).
+    std::vector<std::uint32_t> bu, bv;
+    std::vector<float> bw;
+    std::size_t repair_count = 0;
+
+    // Deduplicated, (u,v)-sorted CORE-SG edge list.
+    bool edges_ready = false;
+    std::uint64_t num_edges = 0, knn_edges = 0;
+    DBuf<std::uint32_t> eu, ev, eoff;
+    DBuf<float> ew;
+
+    // Extraction scratch.
+    DBuf<float> core;
+    DBuf<std::uint32_t> comp, parent, newroot, minv, labels_tmp;
+    DBuf<std::uint64_t> best, mst_keys;
+    DBuf<LiveEdge> live_a, live_b;
+    DBuf<std::uint8_t> mst_flag;
+    DBuf<std::uint32_t> counters;     // small device counters
+    DBuf<std::uint64_t> counters64;
+    std::vector<std::uint32_t> host_counters;
+    DBuf<std::uint32_t> scan_tmp;
+    DBuf<std::uint32_t> out_pos;
+
+    // Hubness / selection state (filled by the build).
+    DBuf<std::int32_t> rdeg;
+    DBuf<double> score;
+    DBuf<std::uint32_t> conn;
+    std::uint32_t num_components = 0;
+    std::vector<std::uint32_t> selected;
+    bool hub_ready = false;
+
+    // Profiling.
+    bool profiling = false;
+    std::map<std::string, KernelTimer> timers;
+
+    ~Graph();
+    void ensure_scan_tmp(std::size_t n);
+    void timer_start(const char* name, hipEvent_t* start);
+    void timer_stop(const char* name, hipEvent_t start);
+    void resolve_timers();
+};
+
+// RAII bracket for a profiled launch.
+struct ScopedKernelTimer {
+    Graph& g;
+    const char* name;
+    hipEvent_t start = nullptr;
+    ScopedKernelTimer(Graph& graph, const char* nm) : g(graph), name(nm) {
+        if (g.profiling) g.timer_start(name, &start);
+    }
+    ~ScopedKernelTimer() {
+        if (g.profiling && start) g.timer_stop(name, start);
+    }
+};
+
+// extraction (extract.hip)
+void prepare_edges(Graph& g);
+void extract_msts(Graph& g, const std::uint32_t* mpts, std::uint32_t count, void* info,
+                  std::uint32_t* out_u, std::uint32_t* out_v, float* out_w,
+                  std::uint32_t* out_labels);
+void reweighted_edges(Graph& g, std::uint32_t mpts, std::uint32_t* u, std::uint32_t* v,
+                      float* w);
+
+}  // namespace coresg_hip
