@@ -1,0 +1,232 @@
+"""CORE-SG benchmark on one MI355X.
+
+A step is one pass of the hot path over one synthetic batch resident in HBM:
+build the CORE-SG (k-NN descent + hubness + anti-hub bridges) and extract the
+full mpts sweep (one MRD reweight + Boruvka MST per mpts). Default workload:
+n = 1,000,000 points, d = 24, a 32-component Gaussian mixture (std 0.2-3.0,
+2% uniform noise, seed 1), k_max = 32, NN-Descent sample 16 / 10 iterations,
+mpts 1..32.
+
+Prints one JSON line. `value` is the build rate in points/s; the sweep rate
+(MSTs/s) and the per-MST extraction latency percentiles sit beside it, with
+the dominant kernel's roofline fraction (device time from HIP events on the
+library's stream) and the reference CPU path timed on this host.
+
+    python bench.py [--steps K] [--warmup W] [--n N] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(REPO, "incremental-core-sg_amd")
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def _load(name):
+    full = f"coresg_amd_{name}"
+    if full in sys.modules:
+        return sys.modules[full]
+    spec = importlib.util.spec_from_file_location(full, os.path.join(PKG, f"{name}.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[full] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def percentile(xs, p):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    i = min(len(xs) - 1, max(0, int(round(p / 100.0 * (len(xs) - 1)))))
+    return xs[i]
+
+
+def cpu_baseline(x_full, k, sample_n, mpts, threads):
+    """The reference CPU path (oracle/_ref/coresg_cli, built from the
+    reference sources) on a bounded sample of the same workload."""
+    cli = os.path.join(REPO, "oracle", "_ref", "coresg_cli")
+    if not os.path.exists(cli):
+        return None
+    x = x_full[:sample_n]
+    d = x.shape[1]
+    with tempfile.TemporaryDirectory() as tmp:
+        data = os.path.join(tmp, "sample.bin")
+        x.tofile(data)
+        sg = os.path.join(tmp, "sample.coresg")
+        t0 = time.perf_counter()
+        out = subprocess.run([cli, "build", "--data", data, "--dim", str(d), "--kmax", str(k),
+                              "--threads", str(threads), "--instances", "1", "--gpu", "0",
+                              "--sample", "16", "--iters", "10", "--verbose", "0", "--out", sg],
+                             check=True, capture_output=True, text=True).stdout
+        build_wall = time.perf_counter() - t0
+        m = re.search(r"total ([0-9.]+)", out)
+        build_s = float(m.group(1)) if m else build_wall
+        t0 = time.perf_counter()
+        out2 = subprocess.run([cli, "extract", "--in", sg, "--threads", str(threads), "--mpts",
+                               ",".join(str(v) for v in mpts), "--verbose", "0"],
+                              check=True, capture_output=True, text=True).stdout
+        extract_wall = time.perf_counter() - t0
+        mst_s = sum(float(v) for v in re.findall(r"\(([0-9.]+)s\)", out2))
+    return dict(build_s=build_s, build_pts_per_s=sample_n / build_s,
+                extract_wall_s=extract_wall, mst_only_s=mst_s,
+                msts_per_s=len(mpts) / extract_wall)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=None, help="override the point count")
+    ap.add_argument("--mpts-max", type=int, default=None)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--recall-queries", type=int, default=300)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(one process per GPU box, no collectives)", file=sys.stderr)
+        sys.exit(2)
+
+    chip = _load("coresg_hip")
+    datagen = _load("datagen")
+    workload = "mixture-1m-d24"
+    x, k = datagen.workload(workload, n=args.n)
+    n, d = x.shape
+    mpts = list(range(1, (args.mpts_max or k) + 1))
+    print(f"[bench] workload {workload} n={n} d={d} k_max={k} mpts=1..{mpts[-1]}",
+          file=sys.stderr, flush=True)
+
+    g = chip.Graph(n, d, k)
+    g.set_points(x)  # resident in HBM before any timed work
+    params = chip.build_params(sample=16, iters=10)
+
+    def step(record):
+        t0 = time.perf_counter()
+        st = g.build(params)
+        t1 = time.perf_counter()
+        g.prepare_edges()
+        lat = []
+        for m in mpts:
+            ta = time.perf_counter()
+            g.extract([m])
+            lat.append((time.perf_counter() - ta) * 1e3)
+        t2 = time.perf_counter()
+        if record is not None:
+            record.append(dict(build_s=t1 - t0, sweep_s=t2 - t1, lat_ms=lat, stats=st))
+
+    for w in range(args.warmup):
+        step(None)
+        print(f"[bench] warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    g.reset_timers()
+    g.set_profiling(True)
+    rec = []
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        step(rec)
+        print(f"[bench] step {s + 1}/{args.steps}: build {rec[-1]['build_s']*1e3:.1f} ms, "
+              f"sweep {rec[-1]['sweep_s']*1e3:.1f} ms", file=sys.stderr, flush=True)
+    total = time.perf_counter() - t_start
+    g.set_profiling(False)
+
+    build_s = float(np.mean([r["build_s"] for r in rec]))
+    sweep_s = float(np.mean([r["sweep_s"] for r in rec]))
+    lat = [v for r in rec for v in r["lat_ms"]]
+    st = rec[-1]["stats"]
+
+    kernels = {}
+    for name in ("nnd_join", "nnd_sample", "nnd_merge", "boruvka_round1", "boruvka_round2",
+                 "boruvka_round_live", "hub_slot_pass", "hub_score", "bridge_mrd_matrix",
+                 "bridge_prim"):
+        ms, cnt = g.kernel_time(name)
+        if cnt:
+            kernels[name] = dict(avg_ms=ms, launches=cnt, total_ms=ms * cnt / args.steps)
+    dominant = max(kernels, key=lambda kk: kernels[kk]["total_ms"]) if kernels else None
+
+    # Algorithmic bytes per launch of the Boruvka first round: every edge
+    # (u, v, w: 12 B) read once, two core-distance gathers (8 B), and two
+    # 8-byte atomic-min updates (16 B). E = resident CORE-SG edges.
+    ne, ke = g.prepare_edges()
+    roofline = None
+    if dominant:
+        avg_ms = kernels[dominant]["avg_ms"]
+        if dominant == "boruvka_round1":
+            algo = ne * (12 + 8 + 16)
+            roofline = dict(kernel=dominant, bound="hbm", achieved=algo / (avg_ms * 1e-3) / 1e9,
+                            peak=HBM_PEAK_GBS, unit="GB/s", traffic=None)
+        elif dominant == "nnd_join":
+            # per node: candidate rows (<= 6*smp of them) read once; the
+            # bound figure uses the tallied pairs
+            algo = n * 6 * 16 * ((d + 7) // 8 * 8) * 4
+            roofline = dict(kernel=dominant, bound="hbm", achieved=algo / (avg_ms * 1e-3) / 1e9,
+                            peak=HBM_PEAK_GBS, unit="GB/s", traffic=None)
+        else:
+            roofline = dict(kernel=dominant, bound="hbm", achieved=None, peak=HBM_PEAK_GBS,
+                            unit="GB/s", traffic=None)
+        if roofline.get("achieved") is not None:
+            roofline["frac"] = roofline["achieved"] / roofline["peak"]
+
+    recall, ratio = g.sampled_recall(args.recall_queries, 42)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        sample_n = min(args.cpu_sample, n)
+        cb = cpu_baseline(x, k, sample_n, [1, 2, 4, 8, 16, 32][: len(mpts)], threads)
+        if cb:
+            cpu = dict(value=cb["build_pts_per_s"], unit="pts/s", cores=threads,
+                       kind="reference",
+                       sample=f"first {sample_n} points of {workload}; reference coresg_cli "
+                              f"build --threads {threads} --instances 1 --sample 16 --iters 10",
+                       build_s=cb["build_s"], msts_per_s=cb["msts_per_s"],
+                       extract_mpts="1,2,4,8,16,32")
+
+    out = {
+        "metric": "CORE-SG build pts/s + mpts-sweep MSTs/s at n=1M; p50/p99 extract latency",
+        "value": n / build_s,
+        "unit": "pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": workload, "n": n, "d": d, "k_max": k, "sample": 16, "iters": 10,
+                   "mpts": f"1..{mpts[-1]}"},
+        "build_ms": build_s * 1e3,
+        "sweep_ms": sweep_s * 1e3,
+        "sweep_msts_per_s": len(mpts) / sweep_s,
+        "extract_p50_ms": percentile(lat, 50),
+        "extract_p99_ms": percentile(lat, 99),
+        "knn_rounds": st["rounds"],
+        "edges": ne,
+        "knn_edges": ke,
+        "antihubs_selected": st["antihubs_selected"],
+        "bridge_edges": st["bridge_edges"],
+        "recall_sampled": recall,
+        "recall_queries": args.recall_queries,
+        "kernels_ms_per_step": {kk: round(v["total_ms"], 3) for kk, v in kernels.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

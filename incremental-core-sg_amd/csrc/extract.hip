@@ -43,8 +43,10 @@ constexpr unsigned kThreads = 256;
 __global__ void __launch_bounds__(kThreads)
     emit_count_kernel(const std::uint32_t* __restrict__ ids, const float* __restrict__ dist,
                       const std::uint32_t* __restrict__ count, std::uint32_t n, std::uint32_t k,
-                      std::uint8_t* __restrict__ emit, std::uint32_t* __restrict__ deg) {
+                      std::uint8_t* __restrict__ emit, std::uint32_t* __restrict__ deg,
+                      unsigned long long* __restrict__ total_out) {
     const std::size_t total = std::size_t(n) * k;
+    unsigned long long local = 0;
     for (std::size_t gid = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; gid < total;
          gid += std::size_t(gridDim.x) * blockDim.x) {
         const std::uint32_t p = static_cast<std::uint32_t>(gid / k);
@@ -68,11 +70,16 @@ __global__ void __launch_bounds__(kThreads)
                 }
                 if (!mutual) {
                     e = 1;
-                    atomicAdd(&deg[p < q ? p : q], 1u);
+                    ++local;
+                    if (deg) atomicAdd(&deg[p < q ? p : q], 1u);
                 }
             }
         }
-        emit[gid] = e;
+        if (emit) emit[gid] = e;
+    }
+    if (total_out) {
+        local = wave_sum(local);
+        if (__lane_id() == 0 && local) atomicAdd(total_out, local);
     }
 }
 
@@ -299,7 +306,7 @@ __global__ void __launch_bounds__(kThreads)
         if (best[other] == b && c < other) continue;
         parent[c] = other;
         const std::uint32_t slot = atomicAdd(&counters[0], 1u);
-        mst_keys[slot] = b;
+        if (slot < n) mst_keys[slot] = b;
         if (mst_flag) mst_flag[e] = 1;
     }
 }
@@ -407,7 +414,8 @@ __global__ void __launch_bounds__(kThreads)
     // wave reduce (values < 2^53 each, so a 64-lane sum cannot wrap u64)
     for (int off = 32; off > 0; off >>= 1) {
         local += __shfl_xor(local, off, kWave);
-        over = over || __shfl_xor(static_cast<int>(over), off, kWave);
+        const int other = __shfl_xor(static_cast<int>(over), off, kWave);
+        over = over || other != 0;
     }
     if (__lane_id() == 0) {
         if (local) {
@@ -503,7 +511,7 @@ void prepare_edges(Graph& g) {
     g.eoff.reserve(std::size_t(n) + 1);
     fill_u32(g.eoff.p, std::size_t(n) + 1, 0u, s);
     emit_count_kernel<<<capped_blocks(slots, kThreads), kThreads, 0, s>>>(
-        g.ids.p, g.dist.p, g.count.p, n, k, emit.p, g.eoff.p);
+        g.ids.p, g.dist.p, g.count.p, n, k, emit.p, g.eoff.p, nullptr);
     CSG_LAUNCH_CHECK();
 
     DBuf<std::uint32_t> dbu, dbv;
@@ -557,6 +565,23 @@ void prepare_edges(Graph& g) {
     for (std::uint32_t i = 0; i < nb; ++i) nbv += (g.bu[i] < n) ? 1 : 0;
     g.knn_edges = m - nbv;
     g.edges_ready = true;
+}
+
+// Undirected k-NN edge count (KnnGraph::undirected_edge_count,
+// knn_graph.cpp:202-217) without materialising the list.
+std::uint64_t count_knn_edges(Graph& g) {
+    const std::size_t slots = std::size_t(g.n) * g.k;
+    g.counters64.reserve(4);
+    std::uint64_t* d_total = g.counters64.p + 3;
+    fill_u64(d_total, 1, 0ull, g.stream);
+    emit_count_kernel<<<capped_blocks(slots, kThreads), kThreads, 0, g.stream>>>(
+        g.ids.p, g.dist.p, g.count.p, g.n, g.k, nullptr, nullptr,
+        reinterpret_cast<unsigned long long*>(d_total));
+    CSG_LAUNCH_CHECK();
+    std::uint64_t total = 0;
+    CSG_CHECK(hipMemcpyAsync(&total, d_total, 8, hipMemcpyDeviceToHost, g.stream));
+    CSG_CHECK(hipStreamSynchronize(g.stream));
+    return total;
 }
 
 void reweighted_edges(Graph& g, std::uint32_t mpts, std::uint32_t* u, std::uint32_t* v,

@@ -1,0 +1,434 @@
+// Hubness, components and anti-hub selection on device.
+//
+// Replaces the host code of
+//   * ConnectivityTracker::rebuild (src/connectivity.cpp:99-139, labels :64-74):
+//     lock-free union-find linking the larger root under the smaller, so every
+//     root is the smallest id of its component and the dense labels (by first
+//     appearance) are the rank of the root among roots;
+//   * HubnessIndex::build's reverse degrees (src/hubness.cpp:86-100);
+//   * compute_score / apply_scores (src/hubness.cpp:307-352, 404-422): int64
+//     neighbour sums and the double division, bit for bit;
+//   * select (src/hubness.cpp:424-478): the `budget` smallest nodes under the
+//     total order (score, reverse degree, id) by an MSB radix select over the
+//     128-bit key, plus the per-component quota of the `per` smallest.
+#include <algorithm>
+#include <vector>
+
+#include "hubness.hpp"
+#include "scan.hpp"
+
+namespace coresg_hip {
+namespace {
+
+constexpr unsigned kThreads = 256;
+
+__device__ __forceinline__ std::uint32_t uf_find(std::uint32_t* parent, std::uint32_t x) {
+    // Path halving; parents only ever move to ancestors, so a plain store is
+    // safe under concurrent unions.
+    while (true) {
+        const std::uint32_t p = __atomic_load_n(&parent[x], __ATOMIC_RELAXED);
+        if (p == x) return x;
+        const std::uint32_t gp = __atomic_load_n(&parent[p], __ATOMIC_RELAXED);
+        if (gp != p) __atomic_store_n(&parent[x], gp, __ATOMIC_RELAXED);
+        x = gp;
+    }
+}
+
+__device__ __forceinline__ void uf_unite(std::uint32_t* parent, std::uint32_t a,
+                                         std::uint32_t b) {
+    while (true) {
+        a = uf_find(parent, a);
+        b = uf_find(parent, b);
+        if (a == b) return;
+        if (a > b) {
+            const std::uint32_t t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS(&parent[b], b, a) == b) return;
+    }
+}
+
+__global__ void iota_kernel(std::uint32_t* __restrict__ p, std::uint32_t n) {
+    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += gridDim.x * blockDim.x)
+        p[i] = i;
+}
+
+// One pass over every stored slot: reverse-degree count and union.
+__global__ void __launch_bounds__(kThreads)
+    slot_pass_kernel(const std::uint32_t* __restrict__ ids,
+                     const std::uint32_t* __restrict__ count, std::uint32_t n, std::uint32_t k,
+                     std::int32_t* __restrict__ rdeg, std::uint32_t* __restrict__ parent) {
+    const std::size_t total = std::size_t(n) * k;
+    for (std::size_t gid = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; gid < total;
+         gid += std::size_t(gridDim.x) * blockDim.x) {
+        const std::uint32_t p = static_cast<std::uint32_t>(gid / k);
+        const std::uint32_t j = static_cast<std::uint32_t>(gid - std::size_t(p) * k);
+        if (j >= count[p]) continue;
+        const std::uint32_t q = ids[gid];
+        if (q >= n) continue;
+        atomicAdd(&rdeg[q], 1);
+        if (q != p) uf_unite(parent, p, q);
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    root_flag_kernel(std::uint32_t n, std::uint32_t* __restrict__ parent,
+                     std::uint32_t* __restrict__ flag) {
+    for (std::uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += gridDim.x * blockDim.x) {
+        const std::uint32_t r = uf_find(parent, v);
+        flag[v] = (r == v) ? 1u : 0u;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    label_from_root_kernel(std::uint32_t n, std::uint32_t* __restrict__ parent,
+                           const std::uint32_t* __restrict__ root_scan,
+                           std::uint32_t* __restrict__ labels) {
+    for (std::uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += gridDim.x * blockDim.x)
+        labels[v] = root_scan[uf_find(parent, v)];
+}
+
+// Scores: `lanes` lanes per node (power of two <= 64) sum the neighbours'
+// reverse degrees, then one lane applies the degenerate rules and divides.
+__global__ void __launch_bounds__(kThreads)
+    score_kernel(const std::uint32_t* __restrict__ ids, const std::uint32_t* __restrict__ count,
+                 std::uint32_t n, std::uint32_t k, std::uint32_t k_for_score,
+                 const std::int32_t* __restrict__ rdeg, std::uint32_t lanes,
+                 double* __restrict__ score, std::uint32_t* __restrict__ tagged) {
+    const std::uint32_t lane = threadIdx.x & 63u;
+    const std::uint32_t sub = lane & (lanes - 1u);
+    const std::uint32_t per_wave = 64u / lanes;
+    const std::uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const std::uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+    std::uint32_t local_tagged = 0;
+    for (std::uint32_t base = wave * per_wave; base < n; base += waves * per_wave) {
+        const std::uint32_t v = base + lane / lanes;
+        std::int64_t sum = 0;
+        std::uint32_t deg = 0;
+        if (v < n) {
+            deg = count[v];
+            for (std::uint32_t j = sub; j < deg; j += lanes) {
+                const std::uint32_t q = ids[std::size_t(v) * k + j];
+                if (q < n) sum += rdeg[q];
+            }
+        }
+        for (std::uint32_t off = 1; off < lanes; off <<= 1) sum += __shfl_xor(sum, off, 64);
+        if (v < n && sub == 0) {
+            const std::int32_t rd = rdeg[v];
+            double s;
+            if (rd > 0 && sum <= 0) {
+                s = double(k_for_score);
+            } else if (rd <= 0 && sum > 0) {
+                s = 0.0;
+            } else if (rd <= 0 && sum <= 0) {
+                s = 1.0;
+            } else if (deg == 0) {
+                s = 1.0;
+            } else {
+                const double denom = double(sum) / double(deg);
+                s = (denom > 0.0) ? double(rd) / denom : double(k_for_score);
+            }
+            score[v] = s;
+            local_tagged += (s < 1.0) ? 1u : 0u;
+        }
+    }
+    local_tagged = wave_sum(local_tagged);
+    if (lane == 0 && local_tagged) atomicAdd(tagged, local_tagged);
+}
+
+// ---- radix select over the 128-bit key (score bits, rdeg, id) -------------
+
+struct SelectState {
+    unsigned long long pref_hi, mask_hi, pref_lo, mask_lo;
+    unsigned int rank;  // 0-based rank still to find inside the matching set
+    unsigned int pad;
+};
+
+__device__ __forceinline__ void node_key(const double* score, const std::int32_t* rdeg,
+                                         std::uint32_t v, unsigned long long* hi,
+                                         unsigned long long* lo) {
+    *hi = static_cast<unsigned long long>(__double_as_longlong(score[v]));
+    *lo = (static_cast<unsigned long long>(static_cast<std::uint32_t>(rdeg[v])) << 32) | v;
+}
+
+__global__ void __launch_bounds__(kThreads)
+    radix_hist_kernel(std::uint32_t n, const double* __restrict__ score,
+                      const std::int32_t* __restrict__ rdeg, const SelectState* __restrict__ st,
+                      int digit, std::uint32_t* __restrict__ hist) {
+    __shared__ std::uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const SelectState s = *st;
+    const int shift = (digit < 8) ? (56 - 8 * digit) : (56 - 8 * (digit - 8));
+    for (std::uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += gridDim.x * blockDim.x) {
+        unsigned long long hi, lo;
+        node_key(score, rdeg, v, &hi, &lo);
+        if ((hi & s.mask_hi) != s.pref_hi || (lo & s.mask_lo) != s.pref_lo) continue;
+        const unsigned b = static_cast<unsigned>(((digit < 8) ? hi : lo) >> shift) & 0xFFu;
+        atomicAdd(&h[b], 1u);
+    }
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void __launch_bounds__(256)
+    radix_pick_kernel(SelectState* __restrict__ st, int digit, std::uint32_t* __restrict__ hist) {
+    __shared__ std::uint32_t c[256];
+    c[threadIdx.x] = hist[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        SelectState s = *st;
+        std::uint32_t cum = 0, b = 0;
+        for (b = 0; b < 256; ++b) {
+            if (s.rank < cum + c[b]) break;
+            cum += c[b];
+        }
+        if (b == 256) b = 255;  // unreachable when rank < |matching set|
+        s.rank -= cum;
+        const int shift = (digit < 8) ? (56 - 8 * digit) : (56 - 8 * (digit - 8));
+        if (digit < 8) {
+            s.pref_hi |= static_cast<unsigned long long>(b) << shift;
+            s.mask_hi |= 0xFFull << shift;
+        } else {
+            s.pref_lo |= static_cast<unsigned long long>(b) << shift;
+            s.mask_lo |= 0xFFull << shift;
+        }
+        *st = s;
+    }
+    __syncthreads();
+    hist[threadIdx.x] = 0;
+}
+
+// Append every node whose key is <= the threshold key (exactly `budget` of them).
+__global__ void __launch_bounds__(kThreads)
+    collect_le_kernel(std::uint32_t n, const double* __restrict__ score,
+                      const std::int32_t* __restrict__ rdeg, const SelectState* __restrict__ st,
+                      std::uint32_t cap, std::uint32_t* __restrict__ out,
+                      std::uint32_t* __restrict__ out_count) {
+    const SelectState s = *st;
+    for (std::uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += gridDim.x * blockDim.x) {
+        unsigned long long hi, lo;
+        node_key(score, rdeg, v, &hi, &lo);
+        if (hi < s.pref_hi || (hi == s.pref_hi && lo <= s.pref_lo)) {
+            const std::uint32_t pos = atomicAdd(out_count, 1u);
+            if (pos < cap) out[pos] = v;
+        }
+    }
+}
+
+// ---- per-component quota: `per` rounds of a three-stage lexicographic min ---
+
+__global__ void __launch_bounds__(kThreads)
+    quota_stage_kernel(std::uint32_t n, const double* __restrict__ score,
+                       const std::int32_t* __restrict__ rdeg,
+                       const std::uint32_t* __restrict__ comp, std::uint32_t ncomp,
+                       const std::uint8_t* __restrict__ excluded, int stage,
+                       unsigned long long* __restrict__ qs, std::uint32_t* __restrict__ qr,
+                       std::uint32_t* __restrict__ qi) {
+    for (std::uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += gridDim.x * blockDim.x) {
+        if (excluded[v]) continue;
+        const std::uint32_t c = comp[v];
+        if (c >= ncomp) continue;
+        const unsigned long long sb =
+            static_cast<unsigned long long>(__double_as_longlong(score[v]));
+        if (stage == 0) {
+            atomicMin(&qs[c], sb);
+        } else if (stage == 1) {
+            if (sb == qs[c]) atomicMin(&qr[c], static_cast<std::uint32_t>(rdeg[v]));
+        } else {
+            if (sb == qs[c] && static_cast<std::uint32_t>(rdeg[v]) == qr[c]) atomicMin(&qi[c], v);
+        }
+    }
+}
+
+__global__ void quota_commit_kernel(std::uint32_t ncomp, std::uint32_t round,
+                                    unsigned long long* __restrict__ qs,
+                                    std::uint32_t* __restrict__ qr,
+                                    std::uint32_t* __restrict__ qi,
+                                    std::uint8_t* __restrict__ excluded,
+                                    std::uint32_t* __restrict__ winners) {
+    for (std::uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncomp;
+         c += gridDim.x * blockDim.x) {
+        const std::uint32_t w = qi[c];
+        winners[std::size_t(round) * ncomp + c] = w;
+        if (w != kNone) excluded[w] = 1;
+        qs[c] = ~0ull;
+        qr[c] = kNone;
+        qi[c] = kNone;
+    }
+}
+
+template <typename T>
+__global__ void gather_kernel(const T* __restrict__ src, const std::uint32_t* __restrict__ idx,
+                              std::uint32_t m, T* __restrict__ dst) {
+    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) dst[i] = src[idx[i]];
+}
+
+}  // namespace
+
+void hubness_compute(Graph& g, std::uint32_t* tagged_out) {
+    const std::uint32_t n = g.n, k = g.k;
+    hipStream_t s = g.stream;
+    g.rdeg.reserve(n);
+    g.score.reserve(n);
+    g.conn.reserve(n);
+    DBuf<std::uint32_t> parent, flag;
+    parent.reserve(n);
+    flag.reserve(std::size_t(n) + 1);
+    g.counters.reserve(16);
+    fill_u32(reinterpret_cast<std::uint32_t*>(g.rdeg.p), n, 0u, s);
+    iota_kernel<<<capped_blocks(n, kThreads), kThreads, 0, s>>>(parent.p, n);
+    CSG_LAUNCH_CHECK();
+    const std::size_t slots = std::size_t(n) * k;
+    {
+        ScopedKernelTimer t(g, "hub_slot_pass");
+        slot_pass_kernel<<<capped_blocks(slots, kThreads), kThreads, 0, s>>>(
+            g.ids.p, g.count.p, n, k, g.rdeg.p, parent.p);
+        CSG_LAUNCH_CHECK();
+    }
+    root_flag_kernel<<<capped_blocks(n, kThreads), kThreads, 0, s>>>(n, parent.p, flag.p);
+    CSG_LAUNCH_CHECK();
+    g.ensure_scan_tmp(std::size_t(n) + 1);
+    std::uint32_t* d_ncomp = g.counters.p + 12;
+    exclusive_scan_u32(flag.p, n, d_ncomp, g.scan_tmp.p, s);
+    label_from_root_kernel<<<capped_blocks(n, kThreads), kThreads, 0, s>>>(n, parent.p, flag.p,
+                                                                          g.conn.p);
+    CSG_LAUNCH_CHECK();
+
+    std::uint32_t lanes = 1;
+    while (lanes < k && lanes < 64) lanes <<= 1;
+    std::uint32_t* d_tagged = g.counters.p + 13;
+    fill_u32(d_tagged, 1, 0u, s);
+    {
+        ScopedKernelTimer t(g, "hub_score");
+        score_kernel<<<capped_blocks(std::size_t(n) * lanes, kThreads), kThreads, 0, s>>>(
+            g.ids.p, g.count.p, n, k, k, g.rdeg.p, lanes, g.score.p, d_tagged);
+        CSG_LAUNCH_CHECK();
+    }
+    std::uint32_t h[2] = {0, 0};
+    CSG_CHECK(hipMemcpyAsync(h, d_ncomp, 8, hipMemcpyDeviceToHost, s));
+    CSG_CHECK(hipStreamSynchronize(s));
+    g.num_components = h[0];
+    if (tagged_out) *tagged_out = h[1];
+    g.hub_ready = true;
+}
+
+std::vector<std::uint32_t> select_antihubs(Graph& g, std::size_t budget, std::size_t per) {
+    const std::uint32_t n = g.n;
+    hipStream_t s = g.stream;
+    std::vector<std::uint32_t> sel;
+    if (budget > n) budget = n;
+    DBuf<std::uint32_t> picked;
+    picked.reserve(budget ? budget : 1);
+    g.counters.reserve(16);
+    std::uint32_t* d_cnt = g.counters.p + 14;
+    fill_u32(d_cnt, 1, 0u, s);
+    if (budget > 0 && budget < n) {
+        DBuf<SelectState> st;
+        DBuf<std::uint32_t> hist;
+        st.reserve(1);
+        hist.reserve(256);
+        SelectState init{0, 0, 0, 0, static_cast<unsigned int>(budget - 1), 0};
+        CSG_CHECK(hipMemcpyAsync(st.p, &init, sizeof(init), hipMemcpyHostToDevice, s));
+        fill_u32(hist.p, 256, 0u, s);
+        const unsigned nb = capped_blocks(n, kThreads, 2048);
+        for (int d = 0; d < 16; ++d) {
+            radix_hist_kernel<<<nb, kThreads, 0, s>>>(n, g.score.p, g.rdeg.p, st.p, d, hist.p);
+            CSG_LAUNCH_CHECK();
+            radix_pick_kernel<<<1, 256, 0, s>>>(st.p, d, hist.p);
+            CSG_LAUNCH_CHECK();
+        }
+        collect_le_kernel<<<capped_blocks(n, kThreads), kThreads, 0, s>>>(
+            n, g.score.p, g.rdeg.p, st.p, static_cast<std::uint32_t>(budget), picked.p, d_cnt);
+        CSG_LAUNCH_CHECK();
+        std::uint32_t got = 0;
+        CSG_CHECK(hipMemcpyAsync(&got, d_cnt, 4, hipMemcpyDeviceToHost, s));
+        CSG_CHECK(hipStreamSynchronize(s));
+        if (got != budget) throw std::runtime_error("select: radix select miscounted");
+        sel.resize(budget);
+        CSG_CHECK(hipMemcpyAsync(sel.data(), picked.p, budget * 4, hipMemcpyDeviceToHost, s));
+    } else if (budget == n) {
+        sel.resize(n);
+        for (std::uint32_t i = 0; i < n; ++i) sel[i] = i;
+    }
+
+    std::vector<std::uint32_t> quota;
+    const std::uint32_t ncomp = g.num_components;
+    if (per > 0 && ncomp > 0) {
+        DBuf<unsigned long long> qs;
+        DBuf<std::uint32_t> qr, qi, winners;
+        DBuf<std::uint8_t> excl;
+        qs.reserve(ncomp);
+        qr.reserve(ncomp);
+        qi.reserve(ncomp);
+        winners.reserve(std::size_t(ncomp) * per);
+        excl.reserve(n);
+        fill_u64(reinterpret_cast<std::uint64_t*>(qs.p), ncomp, ~0ull, s);
+        fill_u32(qr.p, ncomp, kNone, s);
+        fill_u32(qi.p, ncomp, kNone, s);
+        fill_u8(excl.p, n, 0, s);
+        const unsigned nb = capped_blocks(n, kThreads);
+        for (std::size_t r = 0; r < per; ++r) {
+            for (int stage = 0; stage < 3; ++stage) {
+                quota_stage_kernel<<<nb, kThreads, 0, s>>>(n, g.score.p, g.rdeg.p, g.conn.p,
+                                                          ncomp, excl.p, stage, qs.p, qr.p, qi.p);
+                CSG_LAUNCH_CHECK();
+            }
+            quota_commit_kernel<<<capped_blocks(ncomp, kThreads), kThreads, 0, s>>>(
+                ncomp, static_cast<std::uint32_t>(r), qs.p, qr.p, qi.p, excl.p, winners.p);
+            CSG_LAUNCH_CHECK();
+        }
+        quota.resize(std::size_t(ncomp) * per);
+        CSG_CHECK(hipMemcpyAsync(quota.data(), winners.p, quota.size() * 4,
+                                 hipMemcpyDeviceToHost, s));
+    }
+    CSG_CHECK(hipStreamSynchronize(s));
+
+    // Union of the budget set and the quota winners (distinct by
+    // construction: one component per node, earlier winners excluded), then
+    // the reference's final order (score, rdeg, id).
+    std::vector<std::uint32_t> all = sel;
+    std::sort(sel.begin(), sel.end());
+    for (std::uint32_t w : quota) {
+        if (w == kNone) continue;
+        if (std::binary_search(sel.begin(), sel.end(), w)) continue;
+        all.push_back(w);
+    }
+    if (all.empty()) return all;
+    DBuf<std::uint32_t> d_ids;
+    DBuf<double> d_sc;
+    DBuf<std::int32_t> d_rd;
+    const std::uint32_t m = static_cast<std::uint32_t>(all.size());
+    d_ids.reserve(m);
+    d_sc.reserve(m);
+    d_rd.reserve(m);
+    CSG_CHECK(hipMemcpyAsync(d_ids.p, all.data(), std::size_t(m) * 4, hipMemcpyHostToDevice, s));
+    gather_kernel<double><<<blocks_for(m, 256), 256, 0, s>>>(g.score.p, d_ids.p, m, d_sc.p);
+    gather_kernel<std::int32_t><<<blocks_for(m, 256), 256, 0, s>>>(g.rdeg.p, d_ids.p, m, d_rd.p);
+    CSG_LAUNCH_CHECK();
+    std::vector<double> sc(m);
+    std::vector<std::int32_t> rd(m);
+    CSG_CHECK(hipMemcpyAsync(sc.data(), d_sc.p, std::size_t(m) * 8, hipMemcpyDeviceToHost, s));
+    CSG_CHECK(hipMemcpyAsync(rd.data(), d_rd.p, std::size_t(m) * 4, hipMemcpyDeviceToHost, s));
+    CSG_CHECK(hipStreamSynchronize(s));
+    std::vector<std::uint32_t> order(m);
+    for (std::uint32_t i = 0; i < m; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](std::uint32_t a, std::uint32_t b) {
+        if (sc[a] != sc[b]) return sc[a] < sc[b];
+        if (rd[a] != rd[b]) return rd[a] < rd[b];
+        return all[a] < all[b];
+    });
+    std::vector<std::uint32_t> out(m);
+    for (std::uint32_t i = 0; i < m; ++i) out[i] = all[order[i]];
+    return out;
+}
+
+}  // namespace coresg_hip

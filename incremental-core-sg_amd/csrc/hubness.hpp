@@ -1,0 +1,26 @@
+// Device hubness, components and anti-hub selection.
+#pragma once
+
+#include <vector>
+
+#include "graph.hpp"
+
+namespace coresg_hip {
+
+// Reverse degrees, component labels (dense, first appearance) and scores of
+// every node into g.rdeg / g.conn / g.score; g.num_components set. `tagged`
+// receives the number of nodes scoring below 1.
+void hubness_compute(Graph& g, std::uint32_t* tagged);
+
+// The `budget` lowest nodes by (score, rdeg, id) plus the `per` lowest of every
+// component, in that order (HubnessIndex::select).
+std::vector<std::uint32_t> select_antihubs(Graph& g, std::size_t budget, std::size_t per);
+
+// MST over the subset `ids` in mutual-reachability space at k_max, edges in
+// Prim acceptance order, stored with the base distance in global ids
+// (CoreSG::mst_over_subset). Appends to u/v/w.
+void mst_over_subset(Graph& g, const std::vector<std::uint32_t>& ids,
+                     std::vector<std::uint32_t>* u, std::vector<std::uint32_t>* v,
+                     std::vector<float>* w);
+
+}  // namespace coresg_hip

@@ -1,0 +1,409 @@
+// Dense mutual-reachability blocks and the bridge MST.
+//
+//   * mrd_block (reference ABI cuda_api.hpp:68-70; CPU semantics
+//     src/cpu_backend.cpp:61-90): rows of the m x m MRD matrix with the exact
+//     host distance (reference summation order, IEEE sqrt), 0 on the diagonal,
+//     max(d, max(c_i, c_j)) elsewhere.
+//   * mst_over_subset (src/coresg.cpp:97-141): gather the subset, core at
+//     k_max, dense MRD matrix, Prim from node 0 with strict improvement and the
+//     first index on ties (src/mst.cpp:172-220), edges re-stored with the base
+//     distance in global ids.
+//   * mrd_row_min (cuda_api.hpp:74-78): per-row minimum across components.
+#include <cstring>
+#include <vector>
+
+#include "build.hpp"
+#include "hubness.hpp"
+#include "scan.hpp"
+
+namespace coresg_hip {
+namespace {
+
+constexpr int kTile = 16;
+
+// One 16x16 tile of the matrix per 256-thread block; both row blocks staged
+// in LDS (stride <= kMaxLdsStride floats each), exact per-pair distance.
+constexpr std::uint32_t kMaxLdsStride = 1024;
+
+__global__ void __launch_bounds__(256)
+    mrd_tile_kernel(const float* __restrict__ pts, std::uint32_t m, std::uint32_t stride,
+                    const float* __restrict__ core, std::uint32_t r0, std::uint32_t rows,
+                    float* __restrict__ out) {
+    extern __shared__ float lds[];
+    float* A = lds;                    // kTile rows
+    float* B = lds + kTile * stride;   // kTile rows
+    const int tx = threadIdx.x & (kTile - 1), ty = threadIdx.x / kTile;
+    const std::uint32_t row_base = blockIdx.y * kTile, col_base = blockIdx.x * kTile;
+    for (std::uint32_t e = threadIdx.x; e < kTile * stride; e += 256) {
+        const std::uint32_t rr = e / stride, dd = e - rr * stride;
+        const std::uint32_t gi = r0 + row_base + rr;
+        A[e] = (row_base + rr < rows && gi < m) ? pts[std::size_t(gi) * stride + dd] : 0.0f;
+        const std::uint32_t gj = col_base + rr;
+        B[e] = (gj < m) ? pts[std::size_t(gj) * stride + dd] : 0.0f;
+    }
+    __syncthreads();
+    const std::uint32_t lr = row_base + ty, gc = col_base + tx;
+    if (lr >= rows || gc >= m) return;
+    const std::uint32_t gr = r0 + lr;
+    float value = 0.0f;
+    if (gr != gc) {
+        const float d = sqrt_exact(l2sqr_exact(A + ty * stride, B + tx * stride, stride));
+        value = fmax_ref(d, fmax_ref(core[gr], core[gc]));
+    }
+    out[std::size_t(lr) * m + gc] = value;
+}
+
+__global__ void __launch_bounds__(256)
+    mrd_plain_kernel(const float* __restrict__ pts, std::uint32_t m, std::uint32_t stride,
+                     const float* __restrict__ core, std::uint32_t r0, std::uint32_t rows,
+                     float* __restrict__ out) {
+    const std::size_t total = std::size_t(rows) * m;
+    for (std::size_t t = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
+         t += std::size_t(gridDim.x) * blockDim.x) {
+        const std::uint32_t lr = static_cast<std::uint32_t>(t / m);
+        const std::uint32_t gc = static_cast<std::uint32_t>(t - std::size_t(lr) * m);
+        const std::uint32_t gr = r0 + lr;
+        float value = 0.0f;
+        if (gr != gc) {
+            const float d = sqrt_exact(l2sqr_exact(pts + std::size_t(gr) * stride,
+                                                   pts + std::size_t(gc) * stride, stride));
+            value = fmax_ref(d, fmax_ref(core[gr], core[gc]));
+        }
+        out[t] = value;
+    }
+}
+
+void launch_mrd(const float* pts, std::uint32_t m, std::uint32_t stride, const float* core,
+                std::uint32_t r0, std::uint32_t rows, float* out, hipStream_t s) {
+    if (rows == 0 || m == 0) return;
+    if (stride <= kMaxLdsStride) {
+        dim3 grid((m + kTile - 1) / kTile, (rows + kTile - 1) / kTile);
+        const std::size_t lds = std::size_t(2) * kTile * stride * sizeof(float);
+        mrd_tile_kernel<<<grid, 256, lds, s>>>(pts, m, stride, core, r0, rows, out);
+    } else {
+        mrd_plain_kernel<<<capped_blocks(std::size_t(rows) * m, 256), 256, 0, s>>>(
+            pts, m, stride, core, r0, rows, out);
+    }
+    CSG_LAUNCH_CHECK();
+}
+
+// Argmin of (value, index) across a 256-thread block; ties -> lower index.
+__device__ __forceinline__ void block_argmin(float* v, std::uint32_t* i, float* sv,
+                                             std::uint32_t* si) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const float ov = __shfl_xor(*v, off, 64);
+        const std::uint32_t oi = __shfl_xor(*i, off, 64);
+        if (ov < *v || (ov == *v && oi < *i)) {
+            *v = ov;
+            *i = oi;
+        }
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sv[w] = *v;
+        si[w] = *i;
+    }
+    __syncthreads();
+    float bv = sv[0];
+    std::uint32_t bi = si[0];
+    for (int t = 1; t < 4; ++t) {
+        if (sv[t] < bv || (sv[t] == bv && si[t] < bi)) {
+            bv = sv[t];
+            bi = si[t];
+        }
+    }
+    __syncthreads();
+    *v = bv;
+    *i = bi;
+}
+
+// Prim over the dense matrix, one 256-thread block. best/from/in_tree live in
+// global scratch of m entries each. Writes accepted edges (local ids,
+// canonical u < v, MRD weight) in acceptance order; out_count[0] = count.
+__global__ void __launch_bounds__(256)
+    prim_kernel(const float* __restrict__ mat, std::uint32_t m, float* __restrict__ best,
+                std::uint32_t* __restrict__ from, std::uint8_t* __restrict__ in_tree,
+                std::uint32_t* __restrict__ ou, std::uint32_t* __restrict__ ov,
+                float* __restrict__ ow, std::uint32_t* __restrict__ out_count) {
+    __shared__ float sv[4];
+    __shared__ std::uint32_t si[4];
+    for (std::uint32_t j = threadIdx.x; j < m; j += 256) {
+        best[j] = __builtin_inff();
+        from[j] = kNone;
+        in_tree[j] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        in_tree[0] = 1;
+        best[0] = 0.0f;
+    }
+    __syncthreads();
+    std::uint32_t current = 0, t = 0;
+    for (std::uint32_t step = 1; step < m; ++step) {
+        const float* row = mat + std::size_t(current) * m;
+        float lv = __builtin_inff();
+        std::uint32_t li = kNone;
+        for (std::uint32_t j = threadIdx.x; j < m; j += 256) {
+            if (in_tree[j]) continue;
+            const float w = row[j];
+            float b = best[j];
+            if (w < b) {
+                b = w;
+                best[j] = w;
+                from[j] = current;
+            }
+            if (b < lv || (b == lv && j < li)) {
+                lv = b;
+                li = j;
+            }
+        }
+        block_argmin(&lv, &li, sv, si);
+        if (li == kNone) break;
+        if (threadIdx.x == 0) {
+            in_tree[li] = 1;
+            const std::uint32_t f = from[li];
+            if (f != kNone && lv < __builtin_inff()) {
+                ou[t] = f < li ? f : li;
+                ov[t] = f < li ? li : f;
+                ow[t] = lv;
+            }
+        }
+        if (from[li] != kNone && lv < __builtin_inff()) ++t;
+        __syncthreads();
+        current = li;
+    }
+    if (threadIdx.x == 0) *out_count = t;
+}
+
+__global__ void subset_gather_kernel(const float* __restrict__ pts, std::uint32_t stride,
+                                     const std::uint32_t* __restrict__ ids, std::uint32_t m,
+                                     const float* __restrict__ dist,
+                                     const std::uint32_t* __restrict__ count, std::uint32_t k,
+                                     float* __restrict__ sub, float* __restrict__ core) {
+    for (std::size_t t = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+         t < std::size_t(m) * stride; t += std::size_t(gridDim.x) * blockDim.x) {
+        const std::uint32_t i = static_cast<std::uint32_t>(t / stride);
+        const std::uint32_t d = static_cast<std::uint32_t>(t - std::size_t(i) * stride);
+        sub[t] = pts[std::size_t(ids[i]) * stride + d];
+        if (d == 0) {
+            // core_dist(ids[i], k_max) (knn_graph.hpp:157-163)
+            const std::uint32_t c = count[ids[i]];
+            core[i] = c ? dist[std::size_t(ids[i]) * k + (k - 1 < c - 1 ? k - 1 : c - 1)]
+                        : __builtin_inff();
+        }
+    }
+}
+
+__global__ void base_weight_kernel(const float* __restrict__ sub, std::uint32_t stride,
+                                   const std::uint32_t* __restrict__ ids,
+                                   const std::uint32_t* __restrict__ count,
+                                   std::uint32_t* __restrict__ ou, std::uint32_t* __restrict__ ov,
+                                   float* __restrict__ ow) {
+    const std::uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *count) return;
+    const std::uint32_t a = ou[t], b = ov[t];
+    ow[t] = sqrt_exact(l2sqr_exact(sub + std::size_t(a) * stride, sub + std::size_t(b) * stride,
+                                   stride));
+    const std::uint32_t ga = ids[a], gb = ids[b];
+    ou[t] = ga < gb ? ga : gb;
+    ov[t] = ga < gb ? gb : ga;
+}
+
+__global__ void __launch_bounds__(256)
+    row_min_kernel(const float* __restrict__ pts, std::uint32_t m, std::uint32_t stride,
+                   const float* __restrict__ core, const std::uint32_t* __restrict__ comp,
+                   std::uint32_t r0, std::uint32_t rows, float* __restrict__ out_min,
+                   std::uint32_t* __restrict__ out_arg) {
+    __shared__ float sv[4];
+    __shared__ std::uint32_t si[4];
+    for (std::uint32_t lr = blockIdx.x; lr < rows; lr += gridDim.x) {
+        const std::uint32_t gr = r0 + lr;
+        const float* a = pts + std::size_t(gr) * stride;
+        const std::uint32_t my = comp ? comp[gr] : 0u;
+        float lv = __builtin_inff();
+        std::uint32_t li = kNone;
+        for (std::uint32_t j = threadIdx.x; j < m; j += 256) {
+            if (j == gr || (comp && comp[j] == my)) continue;
+            const float d = sqrt_exact(l2sqr_exact(a, pts + std::size_t(j) * stride, stride));
+            const float w = fmax_ref(d, fmax_ref(core[gr], core[j]));
+            if (w < lv || (w == lv && j < li)) {
+                lv = w;
+                li = j;
+            }
+        }
+        block_argmin(&lv, &li, sv, si);
+        if (threadIdx.x == 0) {
+            out_min[lr] = lv;
+            out_arg[lr] = li;
+        }
+        __syncthreads();
+    }
+}
+
+// Per-thread residency of the MRD operands: Prim over a virtual graph asks for
+// one row at a time over the same points, so they stay on the device until the
+// caller's pointers change or release_thread_cache() is called.
+struct MrdCache {
+    int device = -1;
+    const float* pts_key = nullptr;
+    const float* core_key = nullptr;
+    std::size_t m = 0;
+    std::uint32_t stride = 0;
+    DBuf<float> pts, core, out;
+    hipStream_t stream = nullptr;
+    void release() {
+        pts.release();
+        core.release();
+        out.release();
+        pts_key = core_key = nullptr;
+        m = 0;
+        stride = 0;
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+        device = -1;
+    }
+    ~MrdCache() { release(); }
+};
+
+thread_local MrdCache t_cache;
+
+void ensure_resident(int device, const float* host_pts, std::size_t m, std::uint32_t stride,
+                     const float* host_core) {
+    if (t_cache.device == device && t_cache.pts_key == host_pts &&
+        t_cache.core_key == host_core && t_cache.m == m && t_cache.stride == stride)
+        return;
+    t_cache.release();
+    t_cache.device = device;
+    CSG_CHECK(hipStreamCreateWithFlags(&t_cache.stream, hipStreamNonBlocking));
+    t_cache.pts.reserve(m * stride);
+    t_cache.core.reserve(m);
+    CSG_CHECK(hipMemcpyAsync(t_cache.pts.p, host_pts, m * stride * 4, hipMemcpyHostToDevice,
+                             t_cache.stream));
+    CSG_CHECK(hipMemcpyAsync(t_cache.core.p, host_core, m * 4, hipMemcpyHostToDevice,
+                             t_cache.stream));
+    t_cache.pts_key = host_pts;
+    t_cache.core_key = host_core;
+    t_cache.m = m;
+    t_cache.stride = stride;
+}
+
+void check_metric(int metric) {
+    if (metric != CORESG_HIP_METRIC_L2)
+        throw std::runtime_error("unsupported metric for the device path (L2 only)");
+}
+
+}  // namespace
+
+void mst_over_subset(Graph& g, const std::vector<std::uint32_t>& ids,
+                     std::vector<std::uint32_t>* u, std::vector<std::uint32_t>* v,
+                     std::vector<float>* w) {
+    const std::uint32_t m = static_cast<std::uint32_t>(ids.size());
+    if (m < 2) return;
+    hipStream_t s = g.stream;
+    const std::uint32_t stride = g.stride;
+    DBuf<std::uint32_t> d_ids, from, ou, ov, cnt;
+    DBuf<float> sub, core, mat, best, ow;
+    DBuf<std::uint8_t> in_tree;
+    d_ids.reserve(m);
+    sub.reserve(std::size_t(m) * stride);
+    core.reserve(m);
+    mat.reserve(std::size_t(m) * m);
+    best.reserve(m);
+    from.reserve(m);
+    in_tree.reserve(m);
+    ou.reserve(m);
+    ov.reserve(m);
+    ow.reserve(m);
+    cnt.reserve(1);
+    CSG_CHECK(hipMemcpyAsync(d_ids.p, ids.data(), std::size_t(m) * 4, hipMemcpyHostToDevice, s));
+    subset_gather_kernel<<<capped_blocks(std::size_t(m) * stride, 256), 256, 0, s>>>(
+        g.pts.p, stride, d_ids.p, m, g.dist.p, g.count.p, g.k, sub.p, core.p);
+    CSG_LAUNCH_CHECK();
+    {
+        ScopedKernelTimer t(g, "bridge_mrd_matrix");
+        launch_mrd(sub.p, m, stride, core.p, 0, m, mat.p, s);
+    }
+    {
+        ScopedKernelTimer t(g, "bridge_prim");
+        prim_kernel<<<1, 256, 0, s>>>(mat.p, m, best.p, from.p, in_tree.p, ou.p, ov.p, ow.p,
+                                      cnt.p);
+        CSG_LAUNCH_CHECK();
+    }
+    base_weight_kernel<<<blocks_for(m, 256), 256, 0, s>>>(sub.p, stride, d_ids.p, cnt.p, ou.p,
+                                                          ov.p, ow.p);
+    CSG_LAUNCH_CHECK();
+    std::uint32_t t = 0;
+    CSG_CHECK(hipMemcpyAsync(&t, cnt.p, 4, hipMemcpyDeviceToHost, s));
+    CSG_CHECK(hipStreamSynchronize(s));
+    const std::size_t base = u->size();
+    u->resize(base + t);
+    v->resize(base + t);
+    w->resize(base + t);
+    if (t) {
+        CSG_CHECK(hipMemcpyAsync(u->data() + base, ou.p, std::size_t(t) * 4,
+                                 hipMemcpyDeviceToHost, s));
+        CSG_CHECK(hipMemcpyAsync(v->data() + base, ov.p, std::size_t(t) * 4,
+                                 hipMemcpyDeviceToHost, s));
+        CSG_CHECK(hipMemcpyAsync(w->data() + base, ow.p, std::size_t(t) * 4,
+                                 hipMemcpyDeviceToHost, s));
+        CSG_CHECK(hipStreamSynchronize(s));
+    }
+}
+
+void mrd_block_host(int device, const float* pts, std::size_t m, std::uint32_t dim,
+                    std::uint32_t stride, int metric, const float* core, std::size_t r0,
+                    std::size_t r1, float* out) {
+    (void)dim;
+    if (m == 0 || r1 <= r0) return;
+    check_metric(metric);
+    if (stride % 8 != 0) throw std::runtime_error("mrd_block: stride must be a multiple of 8");
+    if (r1 > m) throw std::runtime_error("mrd_block: row range exceeds m");
+    ensure_resident(device, pts, m, stride, core);
+    const std::size_t rows = r1 - r0;
+    t_cache.out.reserve(rows * m);
+    launch_mrd(t_cache.pts.p, static_cast<std::uint32_t>(m), stride, t_cache.core.p,
+               static_cast<std::uint32_t>(r0), static_cast<std::uint32_t>(rows), t_cache.out.p,
+               t_cache.stream);
+    CSG_CHECK(hipMemcpyAsync(out, t_cache.out.p, rows * m * 4, hipMemcpyDeviceToHost,
+                             t_cache.stream));
+    CSG_CHECK(hipStreamSynchronize(t_cache.stream));
+}
+
+void mrd_row_min_host(int device, const float* pts, std::size_t m, std::uint32_t dim,
+                      std::uint32_t stride, int metric, const float* core,
+                      const std::uint32_t* component, std::size_t r0, std::size_t r1,
+                      float* out_min, std::uint32_t* out_argmin) {
+    (void)dim;
+    (void)device;
+    if (m == 0 || r1 <= r0) return;
+    check_metric(metric);
+    if (stride % 8 != 0) throw std::runtime_error("mrd_row_min: stride must be a multiple of 8");
+    if (r1 > m) throw std::runtime_error("mrd_row_min: row range exceeds m");
+    hipStream_t s;
+    CSG_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    const std::size_t rows = r1 - r0;
+    DBuf<float> d_pts, d_core, d_min;
+    DBuf<std::uint32_t> d_comp, d_arg;
+    d_pts.reserve(m * stride);
+    d_core.reserve(m);
+    d_min.reserve(rows);
+    d_arg.reserve(rows);
+    CSG_CHECK(hipMemcpyAsync(d_pts.p, pts, m * stride * 4, hipMemcpyHostToDevice, s));
+    CSG_CHECK(hipMemcpyAsync(d_core.p, core, m * 4, hipMemcpyHostToDevice, s));
+    if (component) {
+        d_comp.reserve(m);
+        CSG_CHECK(hipMemcpyAsync(d_comp.p, component, m * 4, hipMemcpyHostToDevice, s));
+    }
+    row_min_kernel<<<static_cast<unsigned>(rows < 65536 ? rows : 65536), 256, 0, s>>>(
+        d_pts.p, static_cast<std::uint32_t>(m), stride, d_core.p, component ? d_comp.p : nullptr,
+        static_cast<std::uint32_t>(r0), static_cast<std::uint32_t>(rows), d_min.p, d_arg.p);
+    CSG_LAUNCH_CHECK();
+    CSG_CHECK(hipMemcpyAsync(out_min, d_min.p, rows * 4, hipMemcpyDeviceToHost, s));
+    CSG_CHECK(hipMemcpyAsync(out_argmin, d_arg.p, rows * 4, hipMemcpyDeviceToHost, s));
+    CSG_CHECK(hipStreamSynchronize(s));
+    CSG_CHECK(hipStreamDestroy(s));
+}
+
+void release_thread_cache() { t_cache.release(); }
+
+}  // namespace coresg_hip

@@ -1,0 +1,148 @@
+// HipBackend: the reference's Backend plugin (include/coresg/backend.hpp:72-92)
+// served by libcoresg_hip.so through its C ABI (include/coresg_hip.h).
+//
+// A drop-in beside cpu_backend.cpp: compiled against the reference's own
+// headers, it implements the same three virtuals with the same semantics as
+// the reference's CUDA backend (src/cuda_backend.cpp:15-133):
+//   * run_knn_job: Dynamic NN-Descent on the slot's GPU, graph refined in place
+//     in metric units; metrics the device does not cover and instances below
+//     512 points run on the host backend; a device out-of-memory is recorded
+//     in the job and, unless oom_fallback is off, finished on the host; any
+//     other device failure throws.
+//   * mrd_block: rows [r0, r1) of the m x m mutual-reachability matrix,
+//     bit-identical to the host backend's (same distance summation order).
+//   * trim: releases the calling thread's resident MRD operands.
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "coresg/backend.hpp"
+#include "coresg_hip.h"
+
+namespace coresg {
+namespace {
+
+bool metric_to_hip(MetricKind m, int* code) {
+    // The device kernels compute the true L2 metric; cosine, L1 and inner
+    // product stay on the host (the reference gates L1/IP the same way).
+    if (m == MetricKind::L2) {
+        *code = CORESG_HIP_METRIC_L2;
+        return true;
+    }
+    return false;
+}
+
+class HipBackend : public Backend {
+public:
+    HipBackend(const std::vector<int>& device_ids, bool oom_fallback)
+        : ids_(device_ids), oom_fallback_(oom_fallback) {
+        if (ids_.empty()) throw std::runtime_error("hip backend: no device ids");
+        for (int id : ids_) {
+            coresg_hip_device_props p;
+            if (!coresg_hip_get_device_props(id, &p)) {
+                throw std::runtime_error(std::string("hip device ") + std::to_string(id) +
+                                         " unavailable: " + coresg_hip_last_error());
+            }
+            DeviceInfo d;
+            // The reference's DeviceKind names its only GPU kind `Cuda`; GPU
+            // jobs report it so the reference's job accounting
+            // (incremental_knn.cpp:1039-1040) counts them as device jobs.
+            d.kind = DeviceKind::Cuda;
+            d.id = p.id;
+            d.name = std::string(p.name) + " (" + p.arch + ")";
+            d.total_memory = p.total_mem;
+            d.free_memory = p.free_mem;
+            d.multiprocessors = p.cu_count;
+            d.cc_major = p.major;
+            d.cc_minor = p.minor;
+            devices_.push_back(d);
+        }
+        fallback_ = make_cpu_backend_slots(0, static_cast<unsigned>(ids_.size()), nullptr);
+    }
+
+    std::string name() const override { return "hip"; }
+    const std::vector<DeviceInfo>& devices() const override { return devices_; }
+    unsigned parallelism() const override { return static_cast<unsigned>(ids_.size()); }
+
+    void run_knn_job(unsigned slot, KnnJob& job) override {
+        int code = 0;
+        if (!metric_to_hip(job.metric, &code) || job.local_n < 512) {
+            fallback_->run_knn_job(slot, job);
+            return;
+        }
+        coresg_hip_nnd_config cfg;
+        cfg.n = static_cast<std::uint32_t>(job.local_n);
+        cfg.dim = job.dim;
+        cfg.stride = job.stride;
+        cfg.k = job.graph.k();
+        cfg.sample = job.params.sample;
+        cfg.iters = job.params.iters;
+        // The host backend seeds init_random with params.seed
+        // (cpu_backend.cpp:56); using the same seed keeps the device's
+        // initial graph bit-identical to the host's.
+        cfg.seed = job.params.seed;
+        cfg.metric = code;
+        cfg.delta = static_cast<float>(job.params.delta);
+        cfg.random_fill = 1;
+
+        job.graph.grow(static_cast<idx_t>(job.local_n));
+        std::uint64_t updates = 0;
+        const int device = device_for(slot);
+        const int ok = coresg_hip_nndescent(
+            device, &cfg, job.vectors.data(), job.graph.ids_storage().data(),
+            job.graph.dist_storage().data(), job.graph.flag_storage().data(),
+            job.graph.count_storage().data(), &updates);
+        if (!ok) {
+            const std::string err = coresg_hip_last_error();
+            if (err.find("out of memory") != std::string::npos ||
+                err.find("hipErrorOutOfMemory") != std::string::npos) {
+                job.out_of_memory = true;
+                job.error = err;
+                if (!oom_fallback_) return;
+                fallback_->run_knn_job(slot, job);
+                return;
+            }
+            throw std::runtime_error(std::string("hip nndescent failed on device ") +
+                                     std::to_string(device) + ": " + err);
+        }
+        job.ran_on = DeviceKind::Cuda;
+        job.ran_on_device = device;
+        job.stats.rounds = job.params.iters;
+        job.stats.updates = static_cast<std::size_t>(updates);
+    }
+
+    void mrd_block(unsigned slot, const float* pts, std::size_t m, std::uint32_t dim,
+                   std::uint32_t stride, MetricKind metric, const float* core, std::size_t r0,
+                   std::size_t r1, float* out) override {
+        int code = 0;
+        if (!metric_to_hip(metric, &code)) {
+            fallback_->mrd_block(slot, pts, m, dim, stride, metric, core, r0, r1, out);
+            return;
+        }
+        if (!coresg_hip_mrd_block(device_for(slot), pts, m, dim, stride, code, core, r0, r1,
+                                  out)) {
+            throw std::runtime_error(std::string("hip mrd_block failed: ") +
+                                     coresg_hip_last_error());
+        }
+    }
+
+    void trim() override { coresg_hip_release_thread_cache(); }
+
+private:
+    int device_for(unsigned slot) const { return ids_[slot < ids_.size() ? slot : 0]; }
+
+    std::vector<int> ids_;
+    bool oom_fallback_ = true;
+    std::vector<DeviceInfo> devices_;
+    std::unique_ptr<Backend> fallback_;
+};
+
+}  // namespace
+
+std::unique_ptr<Backend> make_hip_backend(const std::vector<int>& device_ids,
+                                          bool oom_fallback) {
+    return std::unique_ptr<Backend>(new HipBackend(device_ids, oom_fallback));
+}
+
+}  // namespace coresg

@@ -105,6 +105,7 @@ def lib():
         L.orc_sg_save.restype = C.c_int
         L.orc_sg_save.argtypes = [C.c_void_p, C.c_char_p]
         L.orc_sg_free.argtypes = [C.c_void_p]
+        L.orc_sg_rebuild_bridges.argtypes = [C.c_void_p]
         for nm in ("orc_sg_n", "orc_sg_k", "orc_sg_dim", "orc_sg_stride"):
             getattr(L, nm).restype = C.c_uint32
             getattr(L, nm).argtypes = [C.c_void_p]
@@ -322,6 +323,9 @@ class CoreSG:
         if getattr(self, "h", None) and _lib is not None:
             _lib.orc_sg_free(self.h)
             self.h = None
+
+    def rebuild_bridges(self) -> None:
+        lib().orc_sg_rebuild_bridges(self.h)
 
     def points(self) -> np.ndarray:
         p = lib().orc_sg_points(self.h)

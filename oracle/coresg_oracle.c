@@ -1129,6 +1129,11 @@ orc_sg* orc_sg_load(const char* path, const orc_params* p) {
     return sg;
 }
 
+void orc_sg_rebuild_bridges(orc_sg* sg) {
+    derive_hubness(sg);
+    recompute_bridges(sg);
+}
+
 uint32_t orc_sg_n(const orc_sg* sg) { return sg->n; }
 uint32_t orc_sg_k(const orc_sg* sg) { return sg->k; }
 uint32_t orc_sg_dim(const orc_sg* sg) { return sg->dim; }

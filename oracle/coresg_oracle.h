@@ -117,6 +117,10 @@ size_t orc_sg_knn_edges(const orc_sg* sg);
 void orc_sg_stats(const orc_sg* sg, double* out /* knn_s, hub_s, bridge_s, total_s, rounds,
                                                     updates, evals */);
 
+/* Recompute selection-derived bridges (and repair) from the current graph,
+ * as CoreSG::build does after the k-NN half (coresg.cpp:278-305). */
+void orc_sg_rebuild_bridges(orc_sg* sg);
+
 /* Full CORE-SG edge list reweighted for mpts (CoreSG::edges). Returns |E|;
  * pass NULL outputs to count only. */
 size_t orc_sg_edges(const orc_sg* sg, uint32_t mpts, uint32_t* u, uint32_t* v, float* w);

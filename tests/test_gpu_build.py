@@ -1,0 +1,189 @@
+"""The build half on the GPU: init_random, brute force, hubness, anti-hub
+selection, bridges and repair against the reference's own outputs (golden
+fixtures) and the oracle; NN-Descent against the reference's recall."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import golden, golden_cases, load_pkg_module
+
+chip = load_pkg_module("coresg_hip")
+coresg_io = load_pkg_module("coresg_io")
+datagen = load_pkg_module("datagen")
+pytestmark = pytest.mark.gpu
+
+BUILD_FLAGS = {"brute_d8": dict(brute_force_limit=1000),
+               "islands_q0": dict(min_per_component=0),
+               "islands_repair": dict(min_per_component=0)}
+
+
+def exact_recall(ids, count, x_padded, k):
+    """Mean fraction of the exact k nearest (by (d, id)) found in each list."""
+    bids, _, _, _ = oracle.brute_force(x_padded, k)
+    n = count.size
+    a = ids.reshape(n, k)
+    b = bids.reshape(n, k)
+    return float(np.mean([len(set(a[i][:count[i]]) & set(b[i])) / k for i in range(n)]))
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_init_random_bitexact(name):
+    print(f"[gpu] init_random: {name}", flush=True)
+    g = golden(name)
+    x = oracle.pad(g["data"])
+    (ids, dist, flags, count), _ = chip.nndescent(x, g["data"].shape[1], int(g["k"]), iters=0,
+                                                  seed=42)
+    assert np.array_equal(ids, g["init_ids"])
+    assert np.array_equal(dist.view(np.uint32), g["init_dist"].view(np.uint32))
+    assert np.array_equal(flags, g["init_flags"])
+    assert np.array_equal(count, g["init_count"])
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_hubness_selection_bridges_from_reference_graph(name):
+    """Given the reference's k-NN lists, the device reproduces its reverse
+    degrees, scores, components, selection, bridges and repair edges."""
+    print(f"[gpu] hubness/bridges: {name}", flush=True)
+    gd = golden(name)
+    f = coresg_io.loads(gd["coresg_bytes"].tobytes())
+    g = chip.Graph(f.n, f.dim, f.k_max)
+    g.set_points(f.points)
+    g.set_knn(f.ids, f.dist, f.flags, f.count)
+    # default parameters: the state the reference derives on load
+    g.build_bridges(chip.build_params())
+    rd, sc, lab, nc = g.hubness()
+    assert np.array_equal(rd, gd["state_rdeg"])
+    assert np.array_equal(sc.view(np.uint64), gd["state_score"].view(np.uint64))
+    assert np.array_equal(lab, gd["state_conn_labels"])
+    assert nc == int(gd["state_conn_components"][0])
+    assert np.array_equal(g.selected(), gd["state_selected"])
+    # the build's own parameters: bridges and repair exactly as in the file
+    st = g.build_bridges(chip.build_params(**{k: v for k, v in BUILD_FLAGS.get(name, {}).items()
+                                              if k != "brute_force_limit"}))
+    u, v, w, rc = g.get_bridges()
+    ru, rv, rw = f.bridge_uvw()
+    assert np.array_equal(u, ru) and np.array_equal(v, rv)
+    assert np.array_equal(w.view(np.uint32), rw.view(np.uint32))
+    assert rc == f.repair_count and st["repair_edges"] == f.repair_count
+    g.close()
+
+
+def test_brute_force_pipeline_bitexact_end_to_end():
+    """With the exact k-NN half the whole build is deterministic: the device
+    build reproduces the reference's .coresg and MSTs bit for bit."""
+    print("[gpu] brute-force end to end", flush=True)
+    gd = golden("brute_d8")
+    x = gd["data"]
+    f = coresg_io.loads(gd["coresg_bytes"].tobytes())
+    g = chip.Graph(x.shape[0], x.shape[1], int(gd["k"]))
+    g.set_points(x)
+    g.build(chip.build_params(brute_force_limit=1000))
+    ids, dist, flags, count = g.get_knn()
+    assert np.array_equal(ids, f.ids) and np.array_equal(count, f.count)
+    assert np.array_equal(dist.view(np.uint32), f.dist.view(np.uint32))
+    assert np.array_equal(flags, f.flags)
+    u, v, w, rc = g.get_bridges()
+    ru, rv, rw = f.bridge_uvw()
+    assert np.array_equal(u, ru) and np.array_equal(v, rv) and rc == f.repair_count
+    for m in gd["mpts"]:
+        r = g.extract([int(m)], edges=True)[0]
+        o = np.lexsort((r["v"], r["u"], r["w"]))
+        assert np.array_equal(r["u"][o], gd[f"mst{m}_mst_u"])
+        assert np.array_equal(r["v"][o], gd[f"mst{m}_mst_v"])
+        assert np.float64(r["weight"]).view(np.uint64) == gd[f"mst{m}_weight"].view(np.uint64)[0]
+    g.close()
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_mrd_block_abi_bitexact(name):
+    gd = golden(name)
+    pts = oracle.pad(gd["data"])[gd["mrd_ids"]]
+    mat = chip.mrd_block(pts, gd["data"].shape[1], gd["mrd_core"])
+    assert np.array_equal(mat.ravel().view(np.uint32), gd["mrd_matrix"].view(np.uint32))
+    # a row band, as JobScheduler::mrd_matrix splits it
+    band = chip.mrd_block(pts, gd["data"].shape[1], gd["mrd_core"], 5, 17)
+    m = pts.shape[0]
+    assert np.array_equal(band.ravel().view(np.uint32),
+                          gd["mrd_matrix"].reshape(m, m)[5:17].ravel().view(np.uint32))
+
+
+@pytest.mark.parametrize("name", ["mix_d16", "mix_d24", "ties", "islands"])
+def test_nndescent_recall_at_least_reference(name):
+    """The descent is schedule dependent; the gate is recall >= the
+    reference's serial descent on the same input (exact recall, all nodes)."""
+    print(f"[gpu] nndescent recall: {name}", flush=True)
+    gd = golden(name)
+    x = gd["data"]
+    k = int(gd["k"])
+    f = coresg_io.loads(gd["coresg_bytes"].tobytes())
+    g = chip.Graph(x.shape[0], x.shape[1], k)
+    g.set_points(x)
+    st = g.build_knn(chip.build_params())
+    ids, dist, flags, count = g.get_knn()
+    assert oracle.lib().orc_validate(ids, dist, count, x.shape[0], k) == 1
+    xp = oracle.pad(x)
+    # stored distances are the exact host distances of the stored pairs
+    nn = np.repeat(np.arange(x.shape[0]), k)
+    sel = ids != 0xFFFFFFFF
+    r, d = oracle.l2sqr_pairs(xp, np.stack([nn[sel], ids[sel]], 1).astype(np.uint32))
+    assert np.array_equal(d.view(np.uint32), dist[sel].view(np.uint32))
+    ours = exact_recall(ids, count, xp, k)
+    ref = exact_recall(f.ids, f.count, xp, k)
+    print(f"  recall gpu={ours:.4f} reference={ref:.4f} rounds={st['rounds']}", flush=True)
+    assert ours >= ref - 1e-9
+    g.close()
+
+
+def test_sampled_recall_matches_oracle():
+    gd = golden("mix_d16")
+    p = __import__("tempfile").mktemp(suffix=".coresg")
+    open(p, "wb").write(gd["coresg_bytes"].tobytes())
+    sg = oracle.CoreSG.load(p)
+    ids, dist, flags, count = sg.graph()
+    g = chip.Graph(sg.n, sg.dim, sg.k)
+    g.set_points(sg.points())
+    g.set_knn(ids, dist, flags, count)
+    r, ratio = g.sampled_recall(200, 42)
+    orr, oratio = sg.sampled_recall(200, 42)
+    assert r == orr and ratio == oratio
+    g.close()
+
+
+def test_full_build_mixture_20k():
+    """20k points, d=16, k_max=16: device build + sweep; recall >= the
+    reference's serial descent, and the sweep is bit-exact against the oracle
+    run on the device-built graph."""
+    print("[gpu] full build 20k", flush=True)
+    x, k = datagen.workload("mixture-20k-d16")
+    g = chip.Graph(x.shape[0], x.shape[1], k)
+    g.set_points(x)
+    st = g.build(chip.build_params())
+    r_gpu, _ = g.sampled_recall(500, 42)
+    ref = oracle.CoreSG.build(x, k_max=k)
+    r_ref, _ = ref.sampled_recall(500, 42)
+    print(f"  sampled recall gpu={r_gpu:.4f} reference-serial={r_ref:.4f} "
+          f"rounds={st['rounds']} selected={st['antihubs_selected']}", flush=True)
+    assert r_gpu >= r_ref - 1e-9
+    # bridges/hubness of the device graph == oracle on the same graph
+    ids, dist, flags, count = g.get_knn()
+    path = __import__("tempfile").mktemp(suffix=".coresg")
+    u, v, w, rc = g.get_bridges()
+    fobj = coresg_io.CoreSGFile(k, x.shape[1], 16, 0, 0, oracle.pad(x), ids, dist, flags, count,
+                                coresg_io.bridges_from_uvw(u, v, w), rc)
+    coresg_io.save(path, fobj)
+    sg = oracle.CoreSG.load(path)
+    assert np.array_equal(g.selected(), sg.selected())
+    sg.rebuild_bridges()
+    ou, ov, ow = sg.bridges()
+    assert np.array_equal(u, ou) and np.array_equal(v, ov)
+    assert np.array_equal(w.view(np.uint32), ow.view(np.uint32)) and rc == sg.repair_count()
+    for m in (1, 4, 16):
+        rr = g.extract([m], edges=True, labels=True)[0]
+        o = sg.mst(m)
+        oo = np.lexsort((rr["v"], rr["u"], rr["w"]))
+        assert np.array_equal(rr["u"][oo], o["u"]) and np.array_equal(rr["v"][oo], o["v"])
+        assert np.array_equal(rr["labels"], o["labels"])
+        assert np.float64(rr["weight"]).view(np.uint64) == np.float64(o["weight"]).view(np.uint64)
+    g.close()
