@@ -148,7 +148,7 @@ def main():
     st = rec[-1]["stats"]
 
     kernels = {}
-    for name in ("nnd_join", "nnd_sample", "nnd_merge", "boruvka_round1", "boruvka_round2",
+    for name in ("nnd_join", "nnd_gather", "nnd_sample", "nnd_merge", "boruvka_round1", "boruvka_round2",
                  "boruvka_round_live", "hub_slot_pass", "hub_score", "bridge_mrd_matrix",
                  "bridge_prim"):
         ms, cnt = g.kernel_time(name)

@@ -122,11 +122,9 @@ typedef struct coresg_hip_build_params {
     int ensure_connected;
     size_t reps_per_component;
     size_t max_reps;
-    /* local join: candidates proposed per member and node (1..32, rounded up to
-     * 8, 16 or 32; default k within [8, 32]) and the
-     * per-node candidate buffer capacity (16..256); 0 = library default */
-    uint32_t join_keep;
-    uint32_t join_bucket;
+    /* local join: per-node proposal buffer capacity per round (16..512;
+     * 0 = library default, 256) */
+    uint32_t join_capacity;
 } coresg_hip_build_params;
 
 typedef struct coresg_hip_build_stats {

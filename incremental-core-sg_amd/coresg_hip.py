@@ -41,7 +41,7 @@ class BuildParams(C.Structure):
                 ("min_selected", C.c_size_t), ("max_selected", C.c_size_t),
                 ("min_per_component", C.c_size_t), ("ensure_connected", C.c_int),
                 ("reps_per_component", C.c_size_t), ("max_reps", C.c_size_t),
-                ("join_keep", C.c_uint32), ("join_bucket", C.c_uint32)]
+                ("join_capacity", C.c_uint32)]
 
 
 class BuildStats(C.Structure):

@@ -155,8 +155,7 @@ void coresg_hip_default_build_params(coresg_hip_build_params* p) {
     p->ensure_connected = 1;
     p->reps_per_component = 4;
     p->max_reps = 4096;
-    p->join_keep = 0;
-    p->join_bucket = 0;
+    p->join_capacity = 0;
 }
 
 coresg_hip_graph* coresg_hip_graph_create(int device, uint32_t n, uint32_t dim, uint32_t k_max) {

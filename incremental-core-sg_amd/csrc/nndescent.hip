@@ -17,13 +17,13 @@
 // lock-free phases:
 //   1. sample: one wavefront per node, ballots give the new/old ranks;
 //      reverse lists are bounded atomic appends.
-//   2. join: one 128-thread group per node stages its candidate rows in LDS,
-//      computes every new x (new u old) distance in the reference's exact
-//      summation order, keeps for each member the best T candidates that beat
-//      its current k-th distance, and appends each as a packed
-//      (distance bits << 32 | id) key to the target's bounded candidate buffer
-//      (atomic cursor; proposals beyond the capacity are dropped, which only
-//      happens in the first rounds of a cold start).
+//   2. join: a gather pass lists each node's distinct candidates (new u
+//      reverse-new, then old u reverse-old) with their thresholds; one
+//      128-thread group per node stages the candidate rows in LDS, computes
+//      every new x (new u old) distance in the reference's exact summation
+//      order, and appends every candidate that beats a member's current k-th
+//      distance to that member's bounded proposal buffer as a packed
+//      (distance bits << 32 | id) key.
 //   3. merge: one 64-thread group per node drops duplicates, self and ids
 //      already listed, ranks the buffer by counting, and merges the best into
 //      the sorted list (new entries after equal distances, flagged new).
@@ -209,132 +209,210 @@ __global__ void __launch_bounds__(256)
 // local join
 // ---------------------------------------------------------------------------
 
+// Gather: one wavefront per node builds its deduplicated candidate list --
+// A = sampled new u reverse new (first), then old u reverse old -- keeping the
+// first copy of every id, plus each member's threshold D[k-1]^2 (to_rank of
+// its current k-th distance, nndescent.cpp:41-42). cnt = na | ns << 16.
+__global__ void __launch_bounds__(256)
+    gather_kernel(std::uint32_t n, std::uint32_t k, std::uint32_t smp,
+                  const float* __restrict__ dist, const std::uint32_t* __restrict__ new_list,
+                  const std::uint32_t* __restrict__ new_cnt,
+                  const std::uint32_t* __restrict__ old_list,
+                  const std::uint32_t* __restrict__ old_cnt,
+                  const std::uint32_t* __restrict__ rev_new,
+                  const std::uint32_t* __restrict__ rev_new_cnt,
+                  const std::uint32_t* __restrict__ rev_old,
+                  const std::uint32_t* __restrict__ rev_old_cnt,
+                  std::uint32_t* __restrict__ out_ids, float* __restrict__ out_thr,
+                  std::uint32_t* __restrict__ out_cnt) {
+    __shared__ std::uint32_t s_raw[4][6 * kMaxSample];
+    const std::uint32_t lane = threadIdx.x & 63u, wib = threadIdx.x >> 6;
+    const std::uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const std::uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+    const std::uint32_t S = 6u * smp;
+    std::uint32_t* raw = s_raw[wib];
+    for (std::uint32_t v = wave; v < n; v += waves) {
+        const std::uint32_t nn = new_cnt[v];
+        const std::uint32_t rn = min(rev_new_cnt[v], smp);
+        const std::uint32_t no = old_cnt[v];
+        const std::uint32_t ro = min(rev_old_cnt[v], smp);
+        const std::uint32_t na0 = nn + rn, ns0 = na0 + no + ro;
+        for (std::uint32_t i = lane; i < ns0; i += 64) {
+            std::uint32_t id;
+            if (i < nn) id = new_list[std::size_t(v) * smp + i];
+            else if (i < na0) id = rev_new[std::size_t(v) * smp + (i - nn)];
+            else if (i < na0 + no) id = old_list[std::size_t(v) * 3 * smp + (i - na0)];
+            else id = rev_old[std::size_t(v) * smp + (i - na0 - no)];
+            raw[i] = id < n ? id : kNone;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        std::uint32_t na = 0, ns = 0;
+        for (std::uint32_t i0 = 0; i0 < ns0; i0 += 64) {
+            const std::uint32_t i = i0 + lane;
+            std::uint32_t id = kNone;
+            bool keep = false;
+            if (i < ns0) {
+                id = raw[i];
+                keep = id != kNone;
+                for (std::uint32_t j = 0; j < i && keep; ++j) keep = raw[j] != id;
+            }
+            const std::uint64_t m = __ballot(keep);
+            const std::uint32_t pos = ns + __popcll(m & ((1ull << lane) - 1ull));
+            if (keep) {
+                out_ids[std::size_t(v) * S + pos] = id;
+                const float t = dist[std::size_t(id) * k + k - 1];
+                out_thr[std::size_t(v) * S + pos] = t * t;
+            }
+            // members taken from the A part (positions < na0)
+            na += __popcll(__ballot(keep && i < na0));
+            ns += __popcll(m);
+        }
+        if (lane == 0) out_cnt[v] = na | (ns << 16);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
 struct JoinArgs {
     const float* pts;
-    std::uint32_t n, stride, k, smp;
-    const float* dist;
-    const std::uint32_t *new_list, *new_cnt, *old_list, *old_cnt;
-    const std::uint32_t *rev_new, *rev_new_cnt, *rev_old, *rev_old_cnt;
+    std::uint32_t n, stride, smp;
+    const std::uint32_t* ids;  // gathered members, n x S
+    const float* thr;          // their thresholds, n x S
+    const std::uint32_t* cnt;  // na | ns << 16
     unsigned long long* cand;  // n x cap proposal buffer
     std::uint32_t* fill;       // proposals offered per node this round
     std::uint32_t cap;
-    std::uint32_t keep;        // best candidates proposed per member and node
 };
 
-// LDS layout (floats/u32): ids[S], thr[S], rows[S*stride], D[A*S].
-// KEEP = candidates proposed per member (the best that beat its k-th).
-template <std::uint32_t KEEP>
+// Exact squared distance with `a` held in registers (compile-time stride),
+// same summation order as l2sqr_exact.
+template <std::uint32_t STRIDE>
+__device__ __forceinline__ float l2sqr_reg(const float (&a)[STRIDE],
+                                           const float* __restrict__ b) {
+    float acc0[8], acc1[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        acc0[j] = 0.0f;
+        acc1[j] = 0.0f;
+    }
+#pragma unroll
+    for (std::uint32_t i = 0; i + 16 <= STRIDE; i += 16) {
+        const float4* pb = reinterpret_cast<const float4*>(b + i);
+        const float4 b0 = pb[0], b1 = pb[1], b2 = pb[2], b3 = pb[3];
+        const float bb[16] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w,
+                              b2.x, b2.y, b2.z, b2.w, b3.x, b3.y, b3.z, b3.w};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float d0 = a[i + j] - bb[j];
+            const float d1 = a[i + 8 + j] - bb[8 + j];
+            acc0[j] = __builtin_fmaf(d0, d0, acc0[j]);
+            acc1[j] = __builtin_fmaf(d1, d1, acc1[j]);
+        }
+    }
+    if constexpr (STRIDE % 16 == 8) {
+        constexpr std::uint32_t i = STRIDE - 8;
+        const float4* pb = reinterpret_cast<const float4*>(b + i);
+        const float4 b0 = pb[0], b1 = pb[1];
+        const float bb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float d0 = a[i + j] - bb[j];
+            acc0[j] = __builtin_fmaf(d0, d0, acc0[j]);
+        }
+    }
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = acc0[j] + acc1[j];
+    const float l0 = v[0] + v[4], l1 = v[1] + v[5], l2 = v[2] + v[6], l3 = v[3] + v[7];
+    return (l0 + l1) + (l2 + l3);
+}
+
+// One 128-thread group per node. LDS: ids[S], thr[S], rows[S*stride] and the
+// distance tile D[A][S+1] (odd row pitch: the A rows are written column-wise
+// and read row-wise without bank conflicts). Thread t owns A-row
+// rb + (t & 31) in registers and computes its distances to every 4th member
+// (broadcast LDS reads). Then each member appends the candidates that beat
+// its threshold to its own proposal buffer.
+// STRIDE = 0 selects the runtime-stride path (rows read from LDS).
+template <std::uint32_t STRIDE>
 __global__ void __launch_bounds__(kJoinThreads) join_kernel(JoinArgs a) {
     extern __shared__ float lds[];
     const std::uint32_t smp = a.smp;
-    const std::uint32_t S = 6u * smp;       // A + 3*smp old + smp reverse old
+    const std::uint32_t S = 6u * smp, A = 2u * smp, P = S + 1;
+    const std::uint32_t stride = STRIDE ? STRIDE : a.stride;
     std::uint32_t* s_id = reinterpret_cast<std::uint32_t*>(lds);
     float* s_thr = lds + S;
     float* s_rows = lds + 2 * S;
-    float* s_D = s_rows + std::size_t(S) * a.stride;
-    __shared__ std::uint32_t s_na, s_ns;
+    float* s_D = s_rows + std::size_t(S) * stride;
+    (void)A;
     const std::uint32_t tid = threadIdx.x;
-    const std::uint32_t stride = a.stride;
-
     for (std::uint32_t v = blockIdx.x; v < a.n; v += gridDim.x) {
-        // candidate ids: A = new u reverse-new, then old u reverse-old
-        if (tid == 0) {
-            const std::uint32_t nn = a.new_cnt[v];
-            std::uint32_t rn = a.rev_new_cnt[v];
-            rn = rn < smp ? rn : smp;
-            const std::uint32_t no = a.old_cnt[v];
-            std::uint32_t ro = a.rev_old_cnt[v];
-            ro = ro < smp ? ro : smp;
-            s_na = nn + rn;
-            s_ns = nn + rn + no + ro;
-        }
-        __syncthreads();
-        const std::uint32_t na = s_na, ns = s_ns;
-        if (na == 0) {
-            __syncthreads();
-            continue;
-        }
+        const std::uint32_t cnt = a.cnt[v];
+        const std::uint32_t na = cnt & 0xFFFFu, ns = cnt >> 16;
+        if (na == 0) continue;  // block-uniform; LDS untouched this iteration
         for (std::uint32_t i = tid; i < ns; i += kJoinThreads) {
-            const std::uint32_t nn = a.new_cnt[v];
-            std::uint32_t rn = a.rev_new_cnt[v];
-            rn = rn < smp ? rn : smp;
-            const std::uint32_t no = a.old_cnt[v];
-            std::uint32_t id;
-            if (i < nn) id = a.new_list[std::size_t(v) * smp + i];
-            else if (i < nn + rn) id = a.rev_new[std::size_t(v) * smp + (i - nn)];
-            else if (i < nn + rn + no) id = a.old_list[std::size_t(v) * 3 * smp + (i - nn - rn)];
-            else id = a.rev_old[std::size_t(v) * smp + (i - nn - rn - no)];
-            s_id[i] = id < a.n ? id : kNone;
+            s_id[i] = a.ids[std::size_t(v) * S + i];
+            s_thr[i] = a.thr[std::size_t(v) * S + i];
         }
         __syncthreads();
-        // keep the first occurrence of every id (A before old)
-        for (std::uint32_t i = tid; i < ns; i += kJoinThreads) {
-            const std::uint32_t id = s_id[i];
-            bool dup = (id == kNone);
-            for (std::uint32_t j = 0; j < i && !dup; ++j) dup = (s_id[j] == id);
-            const float t = (id == kNone) ? 0.0f : a.dist[std::size_t(id) * a.k + a.k - 1];
-            s_thr[i] = dup ? -1.0f : t * t;  // -1 marks a dropped member
+        const std::uint32_t q4 = stride / 4;
+        for (std::uint32_t e = tid; e < ns * q4; e += kJoinThreads) {
+            const std::uint32_t i = e / q4, f = e - i * q4;
+            reinterpret_cast<float4*>(s_rows)[e] =
+                reinterpret_cast<const float4*>(a.pts + std::size_t(s_id[i]) * stride)[f];
         }
         __syncthreads();
-        for (std::uint32_t e = tid; e < ns * stride; e += kJoinThreads) {
-            const std::uint32_t i = e / stride, d = e - i * stride;
-            const std::uint32_t id = s_id[i];
-            s_rows[e] = (s_thr[i] >= 0.0f) ? a.pts[std::size_t(id) * stride + d] : 0.0f;
-        }
-        __syncthreads();
-        for (std::uint32_t e = tid; e < na * ns; e += kJoinThreads) {
-            const std::uint32_t r = e / ns, c = e - r * ns;
-            float val = __builtin_inff();
-            if (r != c && s_thr[r] >= 0.0f && s_thr[c] >= 0.0f)
-                val = l2sqr_exact(s_rows + std::size_t(r) * stride, s_rows + std::size_t(c) * stride,
-                                  stride);
-            s_D[std::size_t(r) * S + c] = val;
-        }
-        __syncthreads();
-        // one member per thread: best `keep` candidates that beat its k-th
-        for (std::uint32_t x = tid; x < ns; x += kJoinThreads) {
-            const float thr = s_thr[x];
-            if (thr < 0.0f) continue;
-            constexpr std::uint32_t kKeepMax = KEEP;
-            float bd[kKeepMax];
-            std::uint32_t bi[kKeepMax];
+        const std::uint32_t cg = tid >> 5;
+        for (std::uint32_t rb = 0; rb < na; rb += 32) {
+            const std::uint32_t r = rb + (tid & 31u);
+            if (r < na) {
+                if constexpr (STRIDE != 0) {
+                    float row[STRIDE];
 #pragma unroll
-            for (std::uint32_t q = 0; q < kKeepMax; ++q) {
-                bd[q] = __builtin_inff();
-                bi[q] = kNone;
-            }
-            const std::uint32_t lim = x < na ? ns : na;
-            for (std::uint32_t y = 0; y < lim; ++y) {
-                const float r = x < na ? s_D[std::size_t(x) * S + y] : s_D[std::size_t(y) * S + x];
-                if (!(r < thr) || !(r < bd[kKeepMax - 1])) continue;
-                // insert (r, y) keeping bd ascending; `keep` caps the used prefix
-                float cr = r;
-                std::uint32_t cy = y;
-#pragma unroll
-                for (std::uint32_t q = 0; q < kKeepMax; ++q) {
-                    if (cr < bd[q]) {
-                        const float td = bd[q];
-                        const std::uint32_t ti = bi[q];
-                        bd[q] = cr;
-                        bi[q] = cy;
-                        cr = td;
-                        cy = ti;
+                    for (std::uint32_t f = 0; f < STRIDE / 4; ++f) {
+                        const float4 t = reinterpret_cast<const float4*>(s_rows + r * STRIDE)[f];
+                        row[4 * f] = t.x;
+                        row[4 * f + 1] = t.y;
+                        row[4 * f + 2] = t.z;
+                        row[4 * f + 3] = t.w;
                     }
+                    for (std::uint32_t c = cg; c < ns; c += 4)
+                        s_D[r * P + c] = (c == r) ? __builtin_inff()
+                                                  : l2sqr_reg<STRIDE>(row, s_rows + c * STRIDE);
+                } else {
+                    for (std::uint32_t c = cg; c < ns; c += 4)
+                        s_D[r * P + c] =
+                            (c == r) ? __builtin_inff()
+                                     : l2sqr_exact(s_rows + std::size_t(r) * stride,
+                                                   s_rows + std::size_t(c) * stride, stride);
                 }
             }
+        }
+        __syncthreads();
+        // Every member proposes each candidate that beats its current k-th
+        // distance (the reference's try_pair test, nndescent.cpp:41-44); the
+        // target's merge keeps the best. Proposals beyond the target's buffer
+        // capacity in this round are dropped.
+        for (std::uint32_t x = tid; x < ns; x += kJoinThreads) {
+            const float thr = s_thr[x];
+            const std::uint32_t lim = x < na ? ns : na;
+            std::uint32_t npass = 0;
+            for (std::uint32_t y = 0; y < lim; ++y) {
+                const float r = x < na ? s_D[x * P + y] : s_D[y * P + x];
+                npass += (r < thr) ? 1u : 0u;
+            }
+            if (npass == 0) continue;
             const std::uint32_t target = s_id[x];
-            std::uint32_t nk = 0;
-#pragma unroll
-            for (std::uint32_t q = 0; q < kKeepMax; ++q) nk += (bi[q] != kNone) ? 1u : 0u;
-            if (nk == 0) continue;
-            const std::uint32_t pos = atomicAdd(&a.fill[target], nk);
+            if (a.fill[target] >= a.cap) continue;  // buffer already full this round
+            std::uint32_t pos = atomicAdd(&a.fill[target], npass);
             unsigned long long* base = a.cand + std::size_t(target) * a.cap;
-#pragma unroll
-            for (std::uint32_t q = 0; q < kKeepMax; ++q) {
-                if (q >= nk || pos + q >= a.cap) break;  // bi is filled front to back
-                const std::uint32_t cand = s_id[bi[q]];
-                const float d = sqrt_exact(bd[q]);
-                base[pos + q] = (static_cast<unsigned long long>(fbits(d)) << 32) | cand;
+            for (std::uint32_t y = 0; y < lim && pos < a.cap; ++y) {
+                const float r = x < na ? s_D[x * P + y] : s_D[y * P + x];
+                if (!(r < thr)) continue;
+                base[pos++] = (static_cast<unsigned long long>(fbits(sqrt_exact(r))) << 32) | s_id[y];
             }
         }
         __syncthreads();
@@ -345,13 +423,16 @@ __global__ void __launch_bounds__(kJoinThreads) join_kernel(JoinArgs a) {
 // merge
 // ---------------------------------------------------------------------------
 
-constexpr std::uint32_t kMaxCap = 256;
+constexpr std::uint32_t kMaxCap = 512;
+constexpr std::uint32_t kDefaultCap = 256;
 
-// One 64-thread group per node. Candidates of the round are in
-// cand[v*cap .. v*cap + min(fill, cap)). A candidate survives if it is not the
-// node itself, not already listed, the first copy of its key, and (for a full
-// list) closer than the current k-th; survivors are ranked by (distance, id)
-// and the best k merged with the list by rank counting.
+// One 64-thread group per node. The round's proposals
+// cand[v*cap .. v*cap + min(fill, cap)) are sorted by (distance, id) with a
+// bitonic network in LDS; a proposal survives if it is the first copy of its
+// key, not the node itself, not already listed and (for a full list) closer
+// than the current k-th -- try_insert's tests (knn_graph.cpp:37-55). The best
+// k survivors are merged with the list by rank counting, existing entries
+// first among equal distances (the reference's sorted insertion).
 __global__ void __launch_bounds__(64)
     merge_kernel(std::uint32_t n, std::uint32_t k, std::uint32_t* __restrict__ ids,
                  float* __restrict__ dist, std::uint8_t* __restrict__ flags,
@@ -359,7 +440,6 @@ __global__ void __launch_bounds__(64)
                  std::uint32_t* __restrict__ fill, std::uint32_t cap,
                  unsigned long long* __restrict__ updates) {
     __shared__ unsigned long long s_key[kMaxCap];
-    __shared__ std::uint8_t s_ok[kMaxCap];
     __shared__ std::uint32_t s_ex[kMaxK];
     __shared__ unsigned long long s_top[kMaxK];
     const std::uint32_t lane = threadIdx.x;
@@ -373,36 +453,60 @@ __global__ void __launch_bounds__(64)
         const float last = (c == k) ? dist[base + k - 1] : __builtin_inff();
         std::uint32_t m = fill[v];
         m = m < cap ? m : cap;
-        if (lane < c) s_ex[lane] = my_id;
-        for (std::uint32_t i = lane; i < m; i += 64) s_key[i] = cand[std::size_t(v) * cap + i];
-        __syncthreads();
-        for (std::uint32_t i = lane; i < m; i += 64) {
-            const unsigned long long key = s_key[i];
-            const std::uint32_t cid = static_cast<std::uint32_t>(key);
-            const float cd = __uint_as_float(static_cast<std::uint32_t>(key >> 32));
-            bool ok = cid != v && cid < n && cd < last;
-            for (std::uint32_t j = 0; j < c && ok; ++j) ok = s_ex[j] != cid;
-            for (std::uint32_t j = 0; j < i && ok; ++j) ok = s_key[j] != key;
-            s_ok[i] = ok ? 1 : 0;
+        std::uint32_t t = 0;
+        if (m > 0) {
+            std::uint32_t P = 64;
+            while (P < m) P <<= 1;
+            if (lane < c) s_ex[lane] = my_id;
+            for (std::uint32_t i = lane; i < P; i += 64)
+                s_key[i] = i < m ? cand[std::size_t(v) * cap + i] : ~0ull;
+            __syncthreads();
+            for (std::uint32_t size = 2; size <= P; size <<= 1) {
+                for (std::uint32_t half = size >> 1; half > 0; half >>= 1) {
+                    for (std::uint32_t q = lane; q < P / 2; q += 64) {
+                        const std::uint32_t lo = 2 * half * (q / half) + (q % half);
+                        const std::uint32_t hi = lo + half;
+                        const bool up = (lo & size) == 0;
+                        const unsigned long long x = s_key[lo], y = s_key[hi];
+                        if ((x > y) == up) {
+                            s_key[lo] = y;
+                            s_key[hi] = x;
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            // survivors in sorted order; the first k of them are the ones
+            // that can enter the list
+            std::uint32_t nvalid = 0;
+            for (std::uint32_t i0 = 0; i0 < m; i0 += 64) {
+                const std::uint32_t i = i0 + lane;
+                bool ok = false;
+                unsigned long long key = ~0ull;
+                if (i < m) {
+                    key = s_key[i];
+                    const std::uint32_t cid = static_cast<std::uint32_t>(key);
+                    const float cd = __uint_as_float(static_cast<std::uint32_t>(key >> 32));
+                    ok = key != ~0ull && (i == 0 || s_key[i - 1] != key) && cid != v &&
+                         cid < n && cd < last;
+                    for (std::uint32_t j = 0; j < c && ok; ++j) ok = s_ex[j] != cid;
+                }
+                const std::uint64_t vm = __ballot(ok);
+                const std::uint32_t r = nvalid + __popcll(vm & ((1ull << lane) - 1ull));
+                if (ok && r < k) s_top[r] = key;
+                nvalid += __popcll(vm);
+            }
+            __syncthreads();
+            t = nvalid < k ? nvalid : k;
+            // Updates as the reference counts them (nndescent.cpp:48-49): every
+            // candidate try_insert would accept against the list as it was
+            // before this merge, even if a closer one of this round evicts it.
+            if (lane == 0) local_upd += nvalid;
         }
-        __syncthreads();
-        std::uint32_t nvalid = 0;
-        for (std::uint32_t i = lane; i < m; i += 64) {
-            if (!s_ok[i]) continue;
-            const unsigned long long key = s_key[i];
-            std::uint32_t rank = 0;
-            for (std::uint32_t j = 0; j < m; ++j) rank += (s_ok[j] && s_key[j] < key) ? 1u : 0u;
-            if (rank < k) s_top[rank] = key;
-            ++nvalid;
-        }
-        nvalid = wave_sum(nvalid);
-        __syncthreads();
-        const std::uint32_t t = nvalid < k ? nvalid : k;  // candidates that can enter
         const bool has_c = lane < t;
         const unsigned long long key = has_c ? s_top[lane] : ~0ull;
         const std::uint32_t cid = static_cast<std::uint32_t>(key);
         const float cd = __uint_as_float(static_cast<std::uint32_t>(key >> 32));
-        // merge: existing entries precede candidates of equal distance
         std::uint32_t pos_e = lane, pos_c = lane;
         for (std::uint32_t j = 0; j < t; ++j) {
             const float od = __shfl(cd, j, 64);
@@ -412,23 +516,22 @@ __global__ void __launch_bounds__(64)
             const float od = __shfl(my_d, j, 64);
             pos_c += (od <= cd) ? 1u : 0u;
         }
-        if (lane < c && pos_e < k) {
-            ids[base + pos_e] = my_id;
-            dist[base + pos_e] = my_d;
-            flags[base + pos_e] = my_f;
-        }
-        const bool ins = has_c && pos_c < k;
-        if (ins) {
-            ids[base + pos_c] = cid;
-            dist[base + pos_c] = cd;
-            flags[base + pos_c] = 1;
+        if (t > 0) {
+            if (lane < c && pos_e < k) {
+                ids[base + pos_e] = my_id;
+                dist[base + pos_e] = my_d;
+                flags[base + pos_e] = my_f;
+            }
+            if (has_c && pos_c < k) {
+                ids[base + pos_c] = cid;
+                dist[base + pos_c] = cd;
+                flags[base + pos_c] = 1;
+            }
         }
         if (lane == 0) {
-            count[v] = (c + t < k) ? c + t : k;
+            if (t > 0) count[v] = (c + t < k) ? c + t : k;
             fill[v] = 0;
         }
-        const unsigned long long inserted = __popcll(__ballot(ins));
-        if (lane == 0) local_upd += inserted;
         __syncthreads();
     }
     if (lane == 0 && local_upd) atomicAdd(updates, local_upd);
@@ -485,8 +588,23 @@ struct DescentBuffers {
     DBuf<std::uint32_t> new_list, new_cnt, old_list, old_cnt;
     DBuf<std::uint32_t> rev_new, rev_new_cnt, rev_old, rev_old_cnt;
     DBuf<unsigned long long> cand, updates;
-    DBuf<std::uint32_t> fill;
+    DBuf<std::uint32_t> fill, gids, gcnt;
+    DBuf<float> gthr;
 };
+
+using JoinFn = void (*)(JoinArgs);
+
+JoinFn pick_join(std::uint32_t stride) {
+    switch (stride) {
+        case 8: return join_kernel<8>;
+        case 16: return join_kernel<16>;
+        case 24: return join_kernel<24>;
+        case 32: return join_kernel<32>;
+        case 48: return join_kernel<48>;
+        case 64: return join_kernel<64>;
+        default: return join_kernel<0>;
+    }
+}
 
 struct DescentResult {
     std::uint32_t rounds = 0;
@@ -497,7 +615,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
                           std::uint32_t k, std::uint32_t* ids, float* dist, std::uint8_t* flags,
                           std::uint32_t* count, std::uint32_t smp, std::uint32_t iters,
                           double delta, std::uint64_t seed, bool random_fill,
-                          std::uint32_t keep, std::uint32_t cap, hipStream_t s) {
+                          std::uint32_t cap, hipStream_t s) {
     DescentResult res;
     if (n == 0) return res;
     if (k > kMaxK) throw std::runtime_error("nndescent: k above 64 is not supported on device");
@@ -525,18 +643,18 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     fill_u32(b.fill.p, n, 0u, s);
 
     const std::uint32_t S = 6u * smp, A = 2u * smp;
-    const std::size_t lds = (2 * std::size_t(S) + std::size_t(S) * stride + std::size_t(A) * S) *
-                            sizeof(float);
+    b.gids.reserve(std::size_t(n) * S);
+    b.gthr.reserve(std::size_t(n) * S);
+    b.gcnt.reserve(n);
+    const std::size_t lds =
+        (2 * std::size_t(S) + std::size_t(S) * stride + std::size_t(A) * (S + 1)) * sizeof(float);
     if (lds > 150 * 1024) throw std::runtime_error("nndescent: candidate tile exceeds LDS");
-    const void* join_fn = keep <= 8    ? reinterpret_cast<const void*>(join_kernel<8>)
-                          : keep <= 16 ? reinterpret_cast<const void*>(join_kernel<16>)
-                                       : reinterpret_cast<const void*>(join_kernel<32>);
+    const JoinFn join_fn = pick_join(stride);
     if (lds > 64 * 1024)
-        CSG_CHECK(hipFuncSetAttribute(join_fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+        CSG_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(join_fn),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                       static_cast<int>(lds)));
-    JoinArgs ja{pts, n, stride, k, smp, dist, b.new_list.p, b.new_cnt.p, b.old_list.p,
-                b.old_cnt.p, b.rev_new.p, b.rev_new_cnt.p, b.rev_old.p, b.rev_old_cnt.p,
-                b.cand.p, b.fill.p, cap, keep};
+    JoinArgs ja{pts, n, stride, smp, b.gids.p, b.gthr.p, b.gcnt.p, b.cand.p, b.fill.p, cap};
     const unsigned join_blocks = static_cast<unsigned>(std::min<std::size_t>(n, 65536));
     const double stop = delta * double(k) * double(n);
     for (std::uint32_t round = 0; round < iters; ++round) {
@@ -553,13 +671,17 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         }
         {
             std::unique_ptr<ScopedKernelTimer> t;
+            if (gp) t.reset(new ScopedKernelTimer(*gp, "nnd_gather"));
+            gather_kernel<<<wave_blocks, 256, 0, s>>>(
+                n, k, smp, dist, b.new_list.p, b.new_cnt.p, b.old_list.p, b.old_cnt.p,
+                b.rev_new.p, b.rev_new_cnt.p, b.rev_old.p, b.rev_old_cnt.p, b.gids.p, b.gthr.p,
+                b.gcnt.p);
+            CSG_LAUNCH_CHECK();
+        }
+        {
+            std::unique_ptr<ScopedKernelTimer> t;
             if (gp) t.reset(new ScopedKernelTimer(*gp, "nnd_join"));
-            if (keep <= 8)
-                join_kernel<8><<<join_blocks, kJoinThreads, lds, s>>>(ja);
-            else if (keep <= 16)
-                join_kernel<16><<<join_blocks, kJoinThreads, lds, s>>>(ja);
-            else
-                join_kernel<32><<<join_blocks, kJoinThreads, lds, s>>>(ja);
+            hipLaunchKernelGGL(join_fn, dim3(join_blocks), dim3(kJoinThreads), lds, s, ja);
             CSG_LAUNCH_CHECK();
         }
         {
@@ -579,15 +701,9 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     return res;
 }
 
-// Proposals per member and node: the list length, within [8, 32]. Fewer leave
-// the descent at a worse fixed point (pairs that become old-old are never
-// compared again).
-std::uint32_t default_keep(std::uint32_t k) { return k < 8 ? 8 : (k > 32 ? 32 : k); }
-
-void check_join_params(std::uint32_t keep, std::uint32_t cap) {
-    if (keep == 0 || keep > 32) throw std::runtime_error("nndescent: join_keep must be in 1..32");
+void check_join_capacity(std::uint32_t cap) {
     if (cap < 16 || cap > kMaxCap)
-        throw std::runtime_error("nndescent: join_bucket must be in 16..256");
+        throw std::runtime_error("nndescent: join_capacity must be in 16..512");
 }
 
 }  // namespace
@@ -608,11 +724,10 @@ void build_knn(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_stat
                                                                g.dist.p, g.flags.p, g.count.p);
         CSG_LAUNCH_CHECK();
     } else {
-        const std::uint32_t keep = p.join_keep ? p.join_keep : default_keep(k);
-        const std::uint32_t cap = p.join_bucket ? p.join_bucket : kMaxCap;
-        check_join_params(keep, cap);
+        const std::uint32_t cap = p.join_capacity ? p.join_capacity : kDefaultCap;
+        check_join_capacity(cap);
         r = run_descent(&g, g.pts.p, n, g.stride, k, g.ids.p, g.dist.p, g.flags.p, g.count.p,
-                        p.sample, p.iters, p.delta, p.seed, true, keep, cap, s);
+                        p.sample, p.iters, p.delta, p.seed, true, cap, s);
     }
     CSG_CHECK(hipStreamSynchronize(s));
     g.knn_set = true;
@@ -654,7 +769,7 @@ void nndescent_host(int device, const coresg_hip_nnd_config& cfg, const float* v
     const DescentResult r =
         run_descent(nullptr, d_pts.p, cfg.n, cfg.stride, cfg.k, d_ids.p, d_dist.p, d_flags.p,
                     d_cnt.p, cfg.sample, cfg.iters, cfg.delta, cfg.seed, cfg.random_fill != 0,
-                    default_keep(cfg.k), kMaxCap, s);
+                    kDefaultCap, s);
     CSG_CHECK(hipMemcpyAsync(ids, d_ids.p, nk * 4, hipMemcpyDeviceToHost, s));
     CSG_CHECK(hipMemcpyAsync(dists, d_dist.p, nk * 4, hipMemcpyDeviceToHost, s));
     CSG_CHECK(hipMemcpyAsync(flags, d_flags.p, nk, hipMemcpyDeviceToHost, s));
