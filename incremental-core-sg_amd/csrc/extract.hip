@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "../../include/coresg_hip.h"
@@ -207,90 +208,150 @@ __device__ __forceinline__ std::uint64_t mrd_key(float w, float cu, float cv, st
     return (std::uint64_t(fbits(wr)) << 32) | e;
 }
 
-// Round 1: every vertex is its own component, so no component lookups.
+// ---- symmetric adjacency (built once per CORE-SG) --------------------------
+
+// Set on an adjacency entry seen from the edge's second endpoint, so the MRD
+// key composes max(core[u], core[v]) in the edge's own operand order.
+constexpr std::uint32_t kSecondEnd = 0x80000000u;
+
+__device__ __forceinline__ std::uint64_t adj_key(std::uint32_t raw, float w, float cx,
+                                                 const float* __restrict__ core,
+                                                 std::uint32_t rank) {
+    const std::uint32_t q = raw & ~kSecondEnd;
+    const float cq = core[q];
+    return (raw & kSecondEnd) ? mrd_key(w, cq, cx, rank) : mrd_key(w, cx, cq, rank);
+}
+
 __global__ void __launch_bounds__(kThreads)
-    round1_kernel(const std::uint32_t* __restrict__ eu, const std::uint32_t* __restrict__ ev,
-                  const float* __restrict__ ew, const float* __restrict__ core,
-                  std::uint64_t num_edges, std::uint64_t* __restrict__ best) {
-    for (std::uint64_t e = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < num_edges;
+    adj_degree_kernel(const std::uint32_t* __restrict__ eu, const std::uint32_t* __restrict__ ev,
+                      std::uint64_t m, std::uint32_t* __restrict__ deg) {
+    for (std::uint64_t e = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
+         e += std::uint64_t(gridDim.x) * blockDim.x) {
+        atomicAdd(&deg[eu[e]], 1u);
+        atomicAdd(&deg[ev[e]], 1u);
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    adj_fill_kernel(const std::uint32_t* __restrict__ eu, const std::uint32_t* __restrict__ ev,
+                    const float* __restrict__ ew, std::uint64_t m,
+                    const std::uint32_t* __restrict__ off, std::uint32_t* __restrict__ cursor,
+                    std::uint32_t* __restrict__ nbr, std::uint32_t* __restrict__ rank,
+                    float* __restrict__ w) {
+    for (std::uint64_t e = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
          e += std::uint64_t(gridDim.x) * blockDim.x) {
         const std::uint32_t u = eu[e], v = ev[e];
-        const std::uint64_t key = mrd_key(ew[e], core[u], core[v], static_cast<std::uint32_t>(e));
-        atomicMin(reinterpret_cast<unsigned long long*>(&best[u]), key);
-        atomicMin(reinterpret_cast<unsigned long long*>(&best[v]), key);
+        const float we = ew[e];
+        const std::uint32_t pu = off[u] + atomicAdd(&cursor[u], 1u);
+        nbr[pu] = v;
+        rank[pu] = static_cast<std::uint32_t>(e);
+        w[pu] = we;
+        const std::uint32_t pv = off[v] + atomicAdd(&cursor[v], 1u);
+        nbr[pv] = u | kSecondEnd;
+        rank[pv] = static_cast<std::uint32_t>(e);
+        w[pv] = we;
     }
 }
 
-// Wave-aggregated append of `keep` lanes to `out` (all 64 lanes must call).
-__device__ __forceinline__ void wave_append(bool keep, const LiveEdge& item, LiveEdge* out,
-                                            std::uint32_t* counter) {
-    const std::uint64_t mask = __ballot(keep);
-    const std::uint32_t lane = __lane_id();
-    std::uint32_t base = 0;
-    if (lane == 0 && mask) base = atomicAdd(counter, static_cast<std::uint32_t>(__popcll(mask)));
-    base = __shfl(base, 0, kWave);
-    if (keep) out[base + __popcll(mask & ((1ull << lane) - 1ull))] = item;
+// ---- Boruvka rounds over vertices ------------------------------------------
+//
+// kLanesPerVertex lanes of a wavefront scan one vertex's adjacency and reduce
+// the packed minimum with shuffles.
+constexpr std::uint32_t kLanesPerVertex = 16;
+
+__device__ __forceinline__ std::uint64_t group_min(std::uint64_t v) {
+#pragma unroll
+    for (std::uint32_t off = kLanesPerVertex / 2; off > 0; off >>= 1) {
+        const std::uint64_t o = __shfl_xor(v, off, kWave);
+        v = o < v ? o : v;
+    }
+    return v;
 }
 
-// Round 2 reads the full sorted list, later rounds the compacted live list.
-// Edges inside one component are dropped for good; survivors are appended.
+// Round 1: every vertex is its own component; its best edge is the minimum
+// MRD key over its adjacency (the reweight happens here), no atomics.
+// active[v] records whether v has any edge at all.
 __global__ void __launch_bounds__(kThreads)
-    round_full_kernel(const std::uint32_t* __restrict__ eu, const std::uint32_t* __restrict__ ev,
-                      const float* __restrict__ ew, const float* __restrict__ core,
-                      std::uint64_t num_edges, const std::uint32_t* __restrict__ comp,
-                      std::uint64_t* __restrict__ best, LiveEdge* __restrict__ out,
-                      std::uint32_t* __restrict__ out_count) {
-    const std::uint64_t stride = std::uint64_t(gridDim.x) * blockDim.x;
-    for (std::uint64_t base = std::uint64_t(blockIdx.x) * blockDim.x; base < num_edges;
-         base += stride) {
-        const std::uint64_t e = base + threadIdx.x;
-        bool keep = false;
-        LiveEdge it{0, 0, 0};
-        if (e < num_edges) {
-            const std::uint32_t u = eu[e], v = ev[e];
-            const std::uint32_t cu = comp[u], cv = comp[v];
-            if (cu != cv) {
-                it.u = u;
-                it.v = v;
-                it.key = mrd_key(ew[e], core[u], core[v], static_cast<std::uint32_t>(e));
-                atomicMin(reinterpret_cast<unsigned long long*>(&best[cu]), it.key);
-                atomicMin(reinterpret_cast<unsigned long long*>(&best[cv]), it.key);
-                keep = true;
+    vertex_round1_kernel(std::uint32_t n, const std::uint32_t* __restrict__ off,
+                         const std::uint32_t* __restrict__ nbr,
+                         const std::uint32_t* __restrict__ rank, const float* __restrict__ w,
+                         const float* __restrict__ core, std::uint64_t* __restrict__ best,
+                         std::uint8_t* __restrict__ active) {
+    const std::uint32_t lane = __lane_id(), sub = lane % kLanesPerVertex;
+    const std::uint32_t per_wave = kWave / kLanesPerVertex;
+    const std::uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const std::uint32_t waves = (gridDim.x * blockDim.x) / kWave;
+    for (std::uint32_t vb = wave * per_wave; vb < n; vb += waves * per_wave) {
+        const std::uint32_t v = vb + lane / kLanesPerVertex;
+        std::uint64_t b = kNone64;
+        if (v < n) {
+            const float cv = core[v];
+            const std::uint32_t e1 = off[v + 1];
+            for (std::uint32_t j = off[v] + sub; j < e1; j += kLanesPerVertex) {
+                const std::uint64_t key = adj_key(nbr[j], w[j], cv, core, rank[j]);
+                b = key < b ? key : b;
             }
         }
-        wave_append(keep, it, out, out_count);
+        b = group_min(b);
+        if (sub == 0 && v < n) {
+            best[v] = b;
+            active[v] = b != kNone64;
+        }
     }
 }
 
+// Later rounds: an active vertex's best edge leaving its component, folded
+// into the component's best with a 64-bit atomic-min (skipped when the stored
+// best is already smaller, which removes most of the contention on large
+// components). A vertex with no edge leaving its component stays inside
+// forever (components only merge), so it is deactivated.
 __global__ void __launch_bounds__(kThreads)
-    round_live_kernel(const LiveEdge* __restrict__ in, const std::uint32_t* __restrict__ in_count,
-                      const std::uint32_t* __restrict__ comp, std::uint64_t* __restrict__ best,
-                      LiveEdge* __restrict__ out, std::uint32_t* __restrict__ out_count) {
-    const std::uint32_t m = *in_count;
-    const std::uint32_t stride = gridDim.x * blockDim.x;
-    for (std::uint32_t base = blockIdx.x * blockDim.x; base < m; base += stride) {
-        const std::uint32_t i = base + threadIdx.x;
-        bool keep = false;
-        LiveEdge it{0, 0, 0};
-        if (i < m) {
-            it = in[i];
-            const std::uint32_t cu = comp[it.u], cv = comp[it.v];
-            if (cu != cv) {
-                atomicMin(reinterpret_cast<unsigned long long*>(&best[cu]), it.key);
-                atomicMin(reinterpret_cast<unsigned long long*>(&best[cv]), it.key);
-                keep = true;
+    vertex_round_kernel(std::uint32_t n, const std::uint32_t* __restrict__ off,
+                        const std::uint32_t* __restrict__ nbr,
+                        const std::uint32_t* __restrict__ rank, const float* __restrict__ w,
+                        const float* __restrict__ core, const std::uint32_t* __restrict__ comp,
+                        std::uint64_t* __restrict__ best, std::uint8_t* __restrict__ active) {
+    const std::uint32_t lane = __lane_id(), sub = lane % kLanesPerVertex;
+    const std::uint32_t per_wave = kWave / kLanesPerVertex;
+    const std::uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const std::uint32_t waves = (gridDim.x * blockDim.x) / kWave;
+    for (std::uint32_t vb = wave * per_wave; vb < n; vb += waves * per_wave) {
+        const std::uint32_t v = vb + lane / kLanesPerVertex;
+        const bool live = v < n && active[v];
+        std::uint32_t cv = 0;
+        std::uint64_t b = kNone64;
+        if (live) {
+            cv = comp[v];
+            const float corev = core[v];
+            const std::uint32_t e1 = off[v + 1];
+            for (std::uint32_t j = off[v] + sub; j < e1; j += kLanesPerVertex) {
+                const std::uint32_t raw = nbr[j];
+                if (comp[raw & ~kSecondEnd] == cv) continue;
+                const std::uint64_t key = adj_key(raw, w[j], corev, core, rank[j]);
+                b = key < b ? key : b;
             }
         }
-        wave_append(keep, it, out, out_count);
+        b = group_min(b);
+        if (sub == 0 && live) {
+            if (b == kNone64) {
+                active[v] = 0;
+            } else if (b < __builtin_nontemporal_load(&best[cv])) {
+                atomicMin(reinterpret_cast<unsigned long long*>(&best[cv]), b);
+            }
+        }
     }
 }
 
 // Every component root with a best edge hooks onto the component across that
 // edge, except the smaller root of a mutual pair. Keys are unique, so hooks
 // form a forest; every hook adds its edge to the tree exactly once.
-// counters[0] = MST edges so far.
+// counters[0] = MST edges so far; counters[kErrorSlot] collects kErr* bits
+// (checked by the host after every round; none is set on a well-formed input).
+constexpr std::uint32_t kErrorSlot = 12;
+constexpr std::uint32_t kErrEdgeIndex = 1u, kErrHookCycle = 2u;
+
 __global__ void __launch_bounds__(kThreads)
-    hook_kernel(std::uint32_t n, const std::uint32_t* __restrict__ eu,
+    hook_kernel(std::uint32_t n, std::uint64_t m, const std::uint32_t* __restrict__ eu,
                 const std::uint32_t* __restrict__ ev, const std::uint32_t* __restrict__ comp,
                 const std::uint64_t* __restrict__ best, std::uint32_t* __restrict__ parent,
                 std::uint64_t* __restrict__ mst_keys, std::uint8_t* __restrict__ mst_flag,
@@ -301,6 +362,10 @@ __global__ void __launch_bounds__(kThreads)
         const std::uint64_t b = best[c];
         if (b == kNone64) continue;
         const std::uint32_t e = static_cast<std::uint32_t>(b);
+        if (e >= m) {
+            atomicOr(&counters[kErrorSlot], kErrEdgeIndex);
+            continue;
+        }
         const std::uint32_t a = comp[eu[e]], bb = comp[ev[e]];
         const std::uint32_t other = (a == c) ? bb : a;
         if (best[other] == b && c < other) continue;
@@ -314,15 +379,17 @@ __global__ void __launch_bounds__(kThreads)
 // Final root of every hooked root (chains end at a root that did not hook).
 __global__ void __launch_bounds__(kThreads)
     jump_kernel(std::uint32_t n, const std::uint32_t* __restrict__ comp,
-                const std::uint32_t* __restrict__ parent, std::uint32_t* __restrict__ newroot) {
+                const std::uint32_t* __restrict__ parent, std::uint32_t* __restrict__ newroot,
+                std::uint32_t* __restrict__ counters) {
     for (std::uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n;
          c += gridDim.x * blockDim.x) {
         if (comp[c] != c) continue;
         // Hooks form a forest (unique keys admit no cycle longer than a
         // mutual pair, which the hook rule breaks); the bound only guards
         // against a corrupted parent array.
-        std::uint32_t r = c;
-        for (std::uint32_t it = 0; parent[r] != r && it < n; ++it) r = parent[r];
+        std::uint32_t r = c, it = 0;
+        for (; parent[r] != r && it < n; ++it) r = parent[r];
+        if (it == n) atomicOr(&counters[kErrorSlot], kErrHookCycle);
         newroot[c] = r;
     }
 }
@@ -360,9 +427,9 @@ __device__ __forceinline__ void odd_decompose(std::uint32_t bits, std::uint32_t*
 // stats64[0] = min exponent + 1024 (as u64 for atomicMin), stats[1] = inf flag
 __global__ void __launch_bounds__(kThreads)
     weight_exp_kernel(const std::uint64_t* __restrict__ keys,
-                      const std::uint32_t* __restrict__ counters,
+                      const std::uint32_t* __restrict__ counters, std::uint32_t n,
                       std::uint32_t* __restrict__ stats) {
-    const std::uint32_t m = counters[0];
+    const std::uint32_t m = min(counters[0], n);  // a forest has at most n-1 edges
     std::uint32_t local = 0xFFFFFFFFu;
     bool inf = false;
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m;
@@ -386,9 +453,9 @@ __global__ void __launch_bounds__(kThreads)
 // Sum of M << (e - e_min) in u64; stats[2] = overflow flag.
 __global__ void __launch_bounds__(kThreads)
     weight_sum_kernel(const std::uint64_t* __restrict__ keys,
-                      const std::uint32_t* __restrict__ counters,
+                      const std::uint32_t* __restrict__ counters, std::uint32_t n,
                       std::uint32_t* __restrict__ stats, unsigned long long* __restrict__ sum) {
-    const std::uint32_t m = counters[0];
+    const std::uint32_t m = min(counters[0], n);
     if (stats[0] == 0xFFFFFFFFu) return;  // no positive finite weight: sum is 0 or inf
     const int emin = int(stats[0]) - 1024;
     unsigned long long local = 0;
@@ -503,7 +570,9 @@ void prepare_edges(Graph& g) {
     const std::uint32_t n = g.n, k = g.k;
     const std::size_t slots = std::size_t(n) * k;
     const std::uint32_t nb = static_cast<std::uint32_t>(g.bu.size());
-    if (slots + nb >= 0xFFFFFFFFull) throw std::runtime_error("edge list exceeds 2^32 entries");
+    if (slots + nb >= 0x7FFFFFFFull)
+        throw std::runtime_error("edge list exceeds 2^31 entries");
+    if (n >= kSecondEnd) throw std::runtime_error("point count exceeds 2^31");
     hipStream_t s = g.stream;
 
     DBuf<std::uint8_t> emit;
@@ -559,6 +628,26 @@ void prepare_edges(Graph& g) {
     segment_sort_kernel<<<capped_blocks(std::size_t(n) * kWave, kThreads, 16384), kThreads, 0,
                           s>>>(n, g.eoff.p, tv.p, tw.p, tsrc.p, g.eu.p, g.ev.p, g.ew.p);
     CSG_LAUNCH_CHECK();
+
+    // Symmetric adjacency for the vertex-centric Boruvka rounds.
+    g.adj_off.reserve(std::size_t(n) + 1);
+    fill_u32(g.adj_off.p, std::size_t(n) + 1, 0u, s);
+    if (m) {
+        adj_degree_kernel<<<capped_blocks(m, kThreads), kThreads, 0, s>>>(g.eu.p, g.ev.p, m,
+                                                                          g.adj_off.p);
+        CSG_LAUNCH_CHECK();
+    }
+    exclusive_scan_u32(g.adj_off.p, std::size_t(n) + 1, nullptr, g.scan_tmp.p, s);
+    g.adj_nbr.reserve(2 * m);
+    g.adj_rank.reserve(2 * m);
+    g.adj_w.reserve(2 * m);
+    if (m) {
+        fill_u32(cursor.p, n, 0u, s);
+        adj_fill_kernel<<<capped_blocks(m, kThreads), kThreads, 0, s>>>(
+            g.eu.p, g.ev.p, g.ew.p, m, g.adj_off.p, cursor.p, g.adj_nbr.p, g.adj_rank.p,
+            g.adj_w.p);
+        CSG_LAUNCH_CHECK();
+    }
     CSG_CHECK(hipStreamSynchronize(s));
     g.num_edges = m;
     std::uint64_t nbv = 0;
@@ -625,8 +714,7 @@ void extract_msts(Graph& g, const std::uint32_t* mpts_list, std::uint32_t count,
     g.mst_keys.reserve(n ? n : 1);
     g.counters.reserve(16);
     g.counters64.reserve(4);
-    g.live_a.reserve(m ? m : 1);
-    g.live_b.reserve(m ? m : 1);
+    g.active.reserve(n ? n : 1);
     const bool want_edges = out_u || out_v || out_w;
     if (want_edges) g.mst_flag.reserve(m ? m : 1);
     if (out_labels) {
@@ -645,7 +733,8 @@ void extract_msts(Graph& g, const std::uint32_t* mpts_list, std::uint32_t count,
         check_mpts(g, &mpts);
         info[qi].mpts = mpts;
         const unsigned nblk = capped_blocks(n, kThreads);
-        const unsigned eblk = capped_blocks(m, kThreads);
+        const unsigned vblk =
+            capped_blocks(std::size_t(n) * kLanesPerVertex, kThreads, 16384);
         fill_u32(g.counters.p, 16, 0u, s);
         fill_u64(g.counters64.p, 4, 0ull, s);
         if (want_edges) fill_u8(g.mst_flag.p, m, 0, s);
@@ -654,49 +743,41 @@ void extract_msts(Graph& g, const std::uint32_t* mpts_list, std::uint32_t count,
         CSG_LAUNCH_CHECK();
 
         std::uint32_t rounds = 0, prev_edges = 0;
-        // counters[0] = MST edges; counters[2], [3] = live counts (ping-pong)
-        LiveEdge* live_in = g.live_a.p;
-        LiveEdge* live_out = g.live_b.p;
-        std::uint32_t* cnt_in = g.counters.p + 2;
-        std::uint32_t* cnt_out = g.counters.p + 3;
         while (true) {
             ++rounds;
             if (rounds == 1) {
                 ScopedKernelTimer t(g, "boruvka_round1");
-                round1_kernel<<<eblk, kThreads, 0, s>>>(g.eu.p, g.ev.p, g.ew.p, g.core.p, m,
-                                                        g.best.p);
+                vertex_round1_kernel<<<vblk, kThreads, 0, s>>>(n, g.adj_off.p, g.adj_nbr.p,
+                                                               g.adj_rank.p, g.adj_w.p, g.core.p,
+                                                               g.best.p, g.active.p);
                 CSG_LAUNCH_CHECK();
-            } else if (rounds == 2) {
-                fill_u32(cnt_out, 1, 0u, s);
-                ScopedKernelTimer t(g, "boruvka_round2");
-                round_full_kernel<<<eblk, kThreads, 0, s>>>(g.eu.p, g.ev.p, g.ew.p, g.core.p, m,
-                                                            g.comp.p, g.best.p, live_out,
-                                                            cnt_out);
-                CSG_LAUNCH_CHECK();
-                std::swap(live_in, live_out);
-                std::swap(cnt_in, cnt_out);
             } else {
-                fill_u32(cnt_out, 1, 0u, s);
-                ScopedKernelTimer t(g, "boruvka_round_live");
-                round_live_kernel<<<eblk, kThreads, 0, s>>>(live_in, cnt_in, g.comp.p, g.best.p,
-                                                            live_out, cnt_out);
+                ScopedKernelTimer t(g, rounds == 2 ? "boruvka_round2" : "boruvka_round_live");
+                vertex_round_kernel<<<vblk, kThreads, 0, s>>>(n, g.adj_off.p, g.adj_nbr.p,
+                                                              g.adj_rank.p, g.adj_w.p, g.core.p,
+                                                              g.comp.p, g.best.p, g.active.p);
                 CSG_LAUNCH_CHECK();
-                std::swap(live_in, live_out);
-                std::swap(cnt_in, cnt_out);
             }
-            hook_kernel<<<nblk, kThreads, 0, s>>>(n, g.eu.p, g.ev.p, g.comp.p, g.best.p,
+            hook_kernel<<<nblk, kThreads, 0, s>>>(n, m, g.eu.p, g.ev.p, g.comp.p, g.best.p,
                                                   g.parent.p, g.mst_keys.p,
                                                   want_edges ? g.mst_flag.p : nullptr,
                                                   g.counters.p);
             CSG_LAUNCH_CHECK();
-            jump_kernel<<<nblk, kThreads, 0, s>>>(n, g.comp.p, g.parent.p, g.newroot.p);
+            jump_kernel<<<nblk, kThreads, 0, s>>>(n, g.comp.p, g.parent.p, g.newroot.p,
+                                                  g.counters.p);
             CSG_LAUNCH_CHECK();
             relabel_kernel<<<nblk, kThreads, 0, s>>>(n, g.comp.p, g.newroot.p, g.parent.p,
                                                      g.best.p);
             CSG_LAUNCH_CHECK();
-            std::uint32_t now = 0;
-            CSG_CHECK(hipMemcpyAsync(&now, g.counters.p, 4, hipMemcpyDeviceToHost, s));
+            std::uint32_t hc[kErrorSlot + 1];
+            CSG_CHECK(hipMemcpyAsync(hc, g.counters.p, sizeof(hc), hipMemcpyDeviceToHost, s));
             CSG_CHECK(hipStreamSynchronize(s));
+            const std::uint32_t now = hc[0];
+            if (hc[kErrorSlot] || now >= n)
+                throw std::runtime_error("extract: Boruvka state corrupted (error bits " +
+                                         std::to_string(hc[kErrorSlot]) + ", " +
+                                         std::to_string(now) + " hooks for " +
+                                         std::to_string(n) + " points)");
             if (now == prev_edges || now + 1 >= n) break;
             prev_edges = now;
         }
@@ -706,10 +787,10 @@ void extract_msts(Graph& g, const std::uint32_t* mpts_list, std::uint32_t count,
             std::uint32_t* stats = g.counters.p + 8;  // [8] emin, [9] inf, [10] overflow
             fill_u32(stats, 1, 0xFFFFFFFFu, s);
             const unsigned wb = capped_blocks(n, kThreads, 1024);
-            weight_exp_kernel<<<wb, kThreads, 0, s>>>(g.mst_keys.p, g.counters.p, stats);
+            weight_exp_kernel<<<wb, kThreads, 0, s>>>(g.mst_keys.p, g.counters.p, n, stats);
             CSG_LAUNCH_CHECK();
             weight_sum_kernel<<<wb, kThreads, 0, s>>>(
-                g.mst_keys.p, g.counters.p, stats,
+                g.mst_keys.p, g.counters.p, n, stats,
                 reinterpret_cast<unsigned long long*>(g.counters64.p));
             CSG_LAUNCH_CHECK();
         }

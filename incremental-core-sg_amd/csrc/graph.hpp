@@ -34,14 +34,6 @@ struct DBuf {
     std::size_t bytes() const { return n * sizeof(T); }
 };
 
-// One live edge of a Boruvka round: endpoints plus the packed
-// (weight bits << 32 | rank) key that totally orders edges as edge_less does.
-struct alignas(16) LiveEdge {
-    std::uint32_t u;
-    std::uint32_t v;
-    std::uint64_t key;
-};
-
 // Accumulated device time of one kernel (HIP events on the graph's stream).
 struct KernelTimer {
     std::vector<hipEvent_t> pending;  // start/stop pairs not yet resolved
@@ -72,12 +64,18 @@ struct Graph {
     std::uint64_t num_edges = 0, knn_edges = 0;
     DBuf<std::uint32_t> eu, ev, eoff;
     DBuf<float> ew;
+    // Symmetric CSR adjacency of the same edges: for vertex x, entries
+    // adj_off[x] .. adj_off[x+1]-1 hold the other endpoint (bit 31 set when x
+    // is the edge's second endpoint), the edge's rank in the sorted list and
+    // its base weight.
+    DBuf<std::uint32_t> adj_off, adj_nbr, adj_rank;
+    DBuf<float> adj_w;
 
     // Extraction scratch.
     DBuf<float> core;
     DBuf<std::uint32_t> comp, parent, newroot, minv, labels_tmp;
     DBuf<std::uint64_t> best, mst_keys;
-    DBuf<LiveEdge> live_a, live_b;
+    DBuf<std::uint8_t> active;  // Boruvka: vertex may still have an outgoing edge
     DBuf<std::uint8_t> mst_flag;
     DBuf<std::uint32_t> counters;     // small device counters
     DBuf<std::uint64_t> counters64;

@@ -424,7 +424,7 @@ __global__ void __launch_bounds__(kJoinThreads) join_kernel(JoinArgs a) {
 // ---------------------------------------------------------------------------
 
 constexpr std::uint32_t kMaxCap = 512;
-constexpr std::uint32_t kDefaultCap = 256;
+constexpr std::uint32_t kDefaultCap = 512;
 
 // One 64-thread group per node. The round's proposals
 // cand[v*cap .. v*cap + min(fill, cap)) are sorted by (distance, id) with a

