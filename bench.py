@@ -54,6 +54,18 @@ def percentile(xs, p):
     return xs[i]
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the same
+    counters on a copy kernel of known size), or None."""
+    path = os.path.join(REPO, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kernel, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
 def cpu_baseline(x_full, k, sample_n, mpts, threads):
     """The reference CPU path (oracle/_ref/coresg_cli, built from the
     reference sources) on a bounded sample of the same workload."""
@@ -95,6 +107,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--recall-queries", type=int, default=300)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the run, copy 1 GiB device-to-device three times (a plain "
+                         "copy kernel of known size, the scale for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -155,29 +170,40 @@ def main():
         if cnt:
             kernels[name] = dict(avg_ms=ms, launches=cnt, total_ms=ms * cnt / args.steps)
     dominant = max(kernels, key=lambda kk: kernels[kk]["total_ms"]) if kernels else None
-
-    # Algorithmic bytes per launch of the Boruvka first round: every edge
-    # (u, v, w: 12 B) read once, two core-distance gathers (8 B), and two
-    # 8-byte atomic-min updates (16 B). E = resident CORE-SG edges.
     ne, ke = g.prepare_edges()
-    roofline = None
-    if dominant:
-        avg_ms = kernels[dominant]["avg_ms"]
-        if dominant == "boruvka_round1":
-            algo = ne * (12 + 8 + 16)
-            roofline = dict(kernel=dominant, bound="hbm", achieved=algo / (avg_ms * 1e-3) / 1e9,
-                            peak=HBM_PEAK_GBS, unit="GB/s", traffic=None)
-        elif dominant == "nnd_join":
-            # per node: candidate rows (<= 6*smp of them) read once; the
-            # bound figure uses the tallied pairs
-            algo = n * 6 * 16 * ((d + 7) // 8 * 8) * 4
-            roofline = dict(kernel=dominant, bound="hbm", achieved=algo / (avg_ms * 1e-3) / 1e9,
-                            peak=HBM_PEAK_GBS, unit="GB/s", traffic=None)
+    stride = (d + 7) // 8 * 8
+    rooflines = {}
+    for name in ("nnd_join", "boruvka_round1"):
+        if name not in kernels:
+            continue
+        w = g.kernel_work(name)
+        launches = kernels[name]["launches"]
+        avg_s = kernels[name]["avg_ms"] * 1e-3
+        if name == "nnd_join":
+            # per launch: each active node's gathered ids + thresholds (8 B per
+            # member) and member rows (4*stride B), the node's count word, and
+            # every proposal written (8 B); FP32 work = 3*stride per distance
+            rows, dists, props, nodes = w
+            algo = (nodes * 4 + rows * (4 * stride + 8) + props * 8) / launches
+            flops = dists * 3 * stride / launches
+            extra = dict(flops_per_launch=flops, fp32_tflops=flops / avg_s / 1e12,
+                         fp32_frac=flops / avg_s / 1e12 / FP32_PEAK_TFLOPS,
+                         members_per_launch=rows / launches, proposals_per_launch=props / launches)
         else:
-            roofline = dict(kernel=dominant, bound="hbm", achieved=None, peak=HBM_PEAK_GBS,
-                            unit="GB/s", traffic=None)
-        if roofline.get("achieved") is not None:
-            roofline["frac"] = roofline["achieved"] / roofline["peak"]
+            # per launch: every adjacency entry (neighbour, rank, base weight:
+            # 12 B) plus the neighbour's core distance (4 B); per vertex its
+            # two offsets, core distance, best key and active flag (21 B)
+            entries, verts = w[0], w[1]
+            algo = (entries * 16 + verts * 21) / launches
+            extra = dict(adjacency_entries=entries / launches)
+        achieved = algo / avg_s / 1e9
+        rooflines[name] = dict(kernel=name, bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS,
+                               unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                               traffic=measured_traffic(name), algo_bytes_per_launch=algo,
+                               avg_launch_ms=kernels[name]["avg_ms"], **extra)
+    roofline = rooflines.get(dominant) or (
+        dict(kernel=dominant, bound="hbm", achieved=None, peak=HBM_PEAK_GBS, unit="GB/s",
+             frac=None, traffic=None) if dominant else None)
 
     recall, ratio = g.sampled_recall(args.recall_queries, 42)
 
@@ -223,9 +249,26 @@ def main():
         "recall_queries": args.recall_queries,
         "kernels_ms_per_step": {kk: round(v["total_ms"], 3) for kk, v in kernels.items()},
         "roofline": roofline,
+        "rooflines": rooflines,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+    if args.copy_probe:
+        # Through the HIP runtime the library itself is linked against (a second
+        # runtime in the process, such as torch's, must not be loaded).
+        import ctypes as C
+        hip = C.CDLL("libamdhip64.so.7")
+        size = C.c_size_t(1 << 30)
+        a_, b_ = C.c_void_p(), C.c_void_p()
+        if hip.hipMalloc(C.byref(a_), size) or hip.hipMalloc(C.byref(b_), size):
+            raise RuntimeError("copy probe: hipMalloc failed")
+        for _ in range(3):
+            if hip.hipMemcpy(b_, a_, size, 3):  # hipMemcpyDeviceToDevice
+                raise RuntimeError("copy probe: hipMemcpy failed")
+        hip.hipDeviceSynchronize()
+        hip.hipFree(a_)
+        hip.hipFree(b_)
+        print("[bench] copy probe: 3 x 1 GiB device-to-device", file=sys.stderr, flush=True)
 
 
 if __name__ == "__main__":

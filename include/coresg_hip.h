@@ -123,7 +123,7 @@ typedef struct coresg_hip_build_params {
     size_t reps_per_component;
     size_t max_reps;
     /* local join: per-node proposal buffer capacity per round (16..512;
-     * 0 = library default, 256) */
+     * 0 = library default, 512) */
     uint32_t join_capacity;
 } coresg_hip_build_params;
 
@@ -214,6 +214,10 @@ int coresg_hip_sampled_recall(coresg_hip_graph* g, size_t queries, uint64_t seed
 int coresg_hip_graph_set_profiling(coresg_hip_graph* g, int on);
 int coresg_hip_graph_kernel_times(coresg_hip_graph* g, const char* kernel, double* avg_ms,
                                   uint64_t* launches);
+/* Work tallies accumulated while profiling, per kernel name (four u64):
+ * "nnd_join": {members staged, exact distances evaluated, proposals
+ * written, nodes}; "boruvka_round1": {adjacency entries, vertices, 0, 0}. */
+int coresg_hip_graph_kernel_work(coresg_hip_graph* g, const char* kernel, uint64_t* work4);
 int coresg_hip_graph_reset_timers(coresg_hip_graph* g);
 
 #ifdef __cplusplus

@@ -107,6 +107,7 @@ SIGNATURES = [
     ("coresg_hip_graph_set_profiling", C.c_int, [_vp, C.c_int]),
     ("coresg_hip_graph_kernel_times", C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double),
                                                 C.POINTER(C.c_uint64)]),
+    ("coresg_hip_graph_kernel_work", C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_uint64)]),
     ("coresg_hip_graph_reset_timers", C.c_int, [_vp]),
 ]
 
@@ -322,6 +323,11 @@ class Graph:
         check(lib().coresg_hip_graph_kernel_times(self.h, name.encode(), C.byref(ms),
                                                   C.byref(n)))
         return ms.value, n.value
+
+    def kernel_work(self, name: str):
+        w = (C.c_uint64 * 4)()
+        check(lib().coresg_hip_graph_kernel_work(self.h, name.encode(), w))
+        return [int(v) for v in w]
 
     def reset_timers(self):
         check(lib().coresg_hip_graph_reset_timers(self.h))

@@ -45,6 +45,15 @@ void Graph::timer_stop(const char* name, hipEvent_t start) {
     t.pending.push_back(e);
 }
 
+void Graph::add_work(const char* name, std::initializer_list<std::uint64_t> w) {
+    KernelTimer& t = timers[name];
+    int i = 0;
+    for (std::uint64_t x : w) {
+        if (i < 4) t.work[i] += x;
+        ++i;
+    }
+}
+
 void Graph::resolve_timers() {
     for (auto& kv : timers) {
         KernelTimer& t = kv.second;
@@ -350,6 +359,14 @@ int coresg_hip_graph_kernel_times(coresg_hip_graph* h, const char* kernel, doubl
             *avg_ms = it->second.launches ? it->second.total_ms / double(it->second.launches)
                                           : 0.0;
         if (launches) *launches = it->second.launches;
+    });
+}
+
+int coresg_hip_graph_kernel_work(coresg_hip_graph* h, const char* kernel, uint64_t* work4) {
+    return guarded([&] {
+        auto it = h->g.timers.find(kernel);
+        for (int i = 0; i < 4; ++i)
+            work4[i] = it == h->g.timers.end() ? 0 : it->second.work[i];
     });
 }
 

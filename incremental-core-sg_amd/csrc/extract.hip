@@ -746,6 +746,7 @@ void extract_msts(Graph& g, const std::uint32_t* mpts_list, std::uint32_t count,
         while (true) {
             ++rounds;
             if (rounds == 1) {
+                if (g.profiling) g.add_work("boruvka_round1", {2 * m, n});
                 ScopedKernelTimer t(g, "boruvka_round1");
                 vertex_round1_kernel<<<vblk, kThreads, 0, s>>>(n, g.adj_off.p, g.adj_nbr.p,
                                                                g.adj_rank.p, g.adj_w.p, g.core.p,

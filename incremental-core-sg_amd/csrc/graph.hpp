@@ -1,6 +1,7 @@
 // Device-resident CORE-SG state shared by the build and extraction paths.
 #pragma once
 
+#include <initializer_list>
 #include <map>
 #include <string>
 #include <vector>
@@ -39,6 +40,7 @@ struct KernelTimer {
     std::vector<hipEvent_t> pending;  // start/stop pairs not yet resolved
     double total_ms = 0.0;
     std::uint64_t launches = 0;
+    std::uint64_t work[4] = {0, 0, 0, 0};  // kernel-specific work tallies
 };
 
 struct Graph {
@@ -100,6 +102,7 @@ struct Graph {
     void timer_start(const char* name, hipEvent_t* start);
     void timer_stop(const char* name, hipEvent_t start);
     void resolve_timers();
+    void add_work(const char* name, std::initializer_list<std::uint64_t> w);
 };
 
 // RAII bracket for a profiled launch.

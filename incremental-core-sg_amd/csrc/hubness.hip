@@ -224,6 +224,9 @@ __global__ void __launch_bounds__(kThreads)
 
 // ---- per-component quota: `per` rounds of a three-stage lexicographic min ---
 
+// Most nodes share a handful of components, so candidates are first reduced
+// per wavefront and component (one atomic per distinct component in the
+// wave, skipped when the stored minimum is already smaller).
 __global__ void __launch_bounds__(kThreads)
     quota_stage_kernel(std::uint32_t n, const double* __restrict__ score,
                        const std::int32_t* __restrict__ rdeg,
@@ -231,19 +234,48 @@ __global__ void __launch_bounds__(kThreads)
                        const std::uint8_t* __restrict__ excluded, int stage,
                        unsigned long long* __restrict__ qs, std::uint32_t* __restrict__ qr,
                        std::uint32_t* __restrict__ qi) {
-    for (std::uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
-         v += gridDim.x * blockDim.x) {
-        if (excluded[v]) continue;
-        const std::uint32_t c = comp[v];
-        if (c >= ncomp) continue;
-        const unsigned long long sb =
-            static_cast<unsigned long long>(__double_as_longlong(score[v]));
-        if (stage == 0) {
-            atomicMin(&qs[c], sb);
-        } else if (stage == 1) {
-            if (sb == qs[c]) atomicMin(&qr[c], static_cast<std::uint32_t>(rdeg[v]));
-        } else {
-            if (sb == qs[c] && static_cast<std::uint32_t>(rdeg[v]) == qr[c]) atomicMin(&qi[c], v);
+    const std::uint32_t lane = __lane_id();
+    const std::uint32_t stride = gridDim.x * blockDim.x;
+    for (std::uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n;
+         base += stride) {
+        const std::uint32_t v = base + lane;
+        bool act = false;
+        std::uint32_t c = 0;
+        std::uint64_t val = ~0ull;
+        if (v < n && !excluded[v]) {
+            c = comp[v];
+            if (c < ncomp) {
+                const unsigned long long sb =
+                    static_cast<unsigned long long>(__double_as_longlong(score[v]));
+                if (stage == 0) {
+                    act = true;
+                    val = sb;
+                } else if (stage == 1) {
+                    act = sb == qs[c];
+                    val = static_cast<std::uint32_t>(rdeg[v]);
+                } else {
+                    act = sb == qs[c] && static_cast<std::uint32_t>(rdeg[v]) == qr[c];
+                    val = v;
+                }
+            }
+        }
+        std::uint64_t pending = __ballot(act);
+        while (pending) {
+            const std::uint32_t leader = static_cast<std::uint32_t>(__builtin_ctzll(pending));
+            const std::uint32_t cl = __shfl(c, leader, 64);
+            const bool mine = act && c == cl;
+            const std::uint64_t m = wave_min_u64(mine ? val : ~0ull);
+            if (lane == leader) {
+                if (stage == 0) {
+                    if (m < qs[cl]) atomicMin(&qs[cl], static_cast<unsigned long long>(m));
+                } else if (stage == 1) {
+                    if (m < qr[cl]) atomicMin(&qr[cl], static_cast<std::uint32_t>(m));
+                } else {
+                    if (m < qi[cl]) atomicMin(&qi[cl], static_cast<std::uint32_t>(m));
+                }
+            }
+            act = act && !mine;
+            pending = __ballot(act);
         }
     }
 }

@@ -438,7 +438,7 @@ __global__ void __launch_bounds__(64)
                  float* __restrict__ dist, std::uint8_t* __restrict__ flags,
                  std::uint32_t* __restrict__ count, const unsigned long long* __restrict__ cand,
                  std::uint32_t* __restrict__ fill, std::uint32_t cap,
-                 unsigned long long* __restrict__ updates) {
+                 unsigned long long* __restrict__ partial) {
     __shared__ unsigned long long s_key[kMaxCap];
     __shared__ std::uint32_t s_ex[kMaxK];
     __shared__ unsigned long long s_top[kMaxK];
@@ -534,7 +534,48 @@ __global__ void __launch_bounds__(64)
         }
         __syncthreads();
     }
-    if (lane == 0 && local_upd) atomicAdd(updates, local_upd);
+    // one partial per block; a single-block pass adds them (a device-wide
+    // atomic per block contends on one address across the XCDs)
+    if (lane == 0) partial[blockIdx.x] = local_upd;
+}
+
+constexpr std::uint32_t kMergeBlocks = 65536;
+
+__global__ void __launch_bounds__(256)
+    sum_partials_kernel(const unsigned long long* __restrict__ partial, std::uint32_t m,
+                        unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long s_w[4];
+    unsigned long long v = 0;
+    for (std::uint32_t i = threadIdx.x; i < m; i += 256) v += partial[i];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63u) == 0) s_w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+// Work tallies of one join launch (profiling only): members staged (rows
+// read), exact distances evaluated, proposals written.
+__global__ void __launch_bounds__(256)
+    join_tally_kernel(std::uint32_t n, const std::uint32_t* __restrict__ cnt,
+                      const std::uint32_t* __restrict__ fill, std::uint32_t cap,
+                      unsigned long long* __restrict__ out) {
+    unsigned long long rows = 0, dists = 0, props = 0;
+    for (std::uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += gridDim.x * blockDim.x) {
+        const std::uint32_t na = cnt[v] & 0xFFFFu, ns = cnt[v] >> 16;
+        if (na == 0) continue;
+        rows += ns;
+        dists += static_cast<unsigned long long>(na) * ns - na;
+        props += fill[v] < cap ? fill[v] : cap;
+    }
+    rows = wave_sum(rows);
+    dists = wave_sum(dists);
+    props = wave_sum(props);
+    if ((threadIdx.x & 63u) == 0) {
+        atomicAdd(&out[0], rows);
+        atomicAdd(&out[1], dists);
+        atomicAdd(&out[2], props);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -587,7 +628,7 @@ __global__ void brute_force_kernel(const float* __restrict__ pts, std::uint32_t 
 struct DescentBuffers {
     DBuf<std::uint32_t> new_list, new_cnt, old_list, old_cnt;
     DBuf<std::uint32_t> rev_new, rev_new_cnt, rev_old, rev_old_cnt;
-    DBuf<unsigned long long> cand, updates;
+    DBuf<unsigned long long> cand, updates, partial, tally;
     DBuf<std::uint32_t> fill, gids, gcnt;
     DBuf<float> gthr;
 };
@@ -640,6 +681,8 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     b.cand.reserve(std::size_t(n) * cap);
     b.fill.reserve(n);
     b.updates.reserve(1);
+    b.partial.reserve(kMergeBlocks);
+    b.tally.reserve(4);
     fill_u32(b.fill.p, n, 0u, s);
 
     const std::uint32_t S = 6u * smp, A = 2u * smp;
@@ -656,11 +699,12 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
                                       static_cast<int>(lds)));
     JoinArgs ja{pts, n, stride, smp, b.gids.p, b.gthr.p, b.gcnt.p, b.cand.p, b.fill.p, cap};
     const unsigned join_blocks = static_cast<unsigned>(std::min<std::size_t>(n, 65536));
+    const unsigned merge_blocks = static_cast<unsigned>(std::min<std::size_t>(n, kMergeBlocks));
+    const bool tallies = gp && gp->profiling;
     const double stop = delta * double(k) * double(n);
     for (std::uint32_t round = 0; round < iters; ++round) {
         fill_u32(b.rev_new_cnt.p, n, 0u, s);
         fill_u32(b.rev_old_cnt.p, n, 0u, s);
-        fill_u64(reinterpret_cast<std::uint64_t*>(b.updates.p), 1, 0ull, s);
         {
             std::unique_ptr<ScopedKernelTimer> t;
             if (gp) t.reset(new ScopedKernelTimer(*gp, "nnd_sample"));
@@ -684,16 +728,25 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
             hipLaunchKernelGGL(join_fn, dim3(join_blocks), dim3(kJoinThreads), lds, s, ja);
             CSG_LAUNCH_CHECK();
         }
+        if (tallies) {
+            fill_u64(reinterpret_cast<std::uint64_t*>(b.tally.p), 3, 0ull, s);
+            join_tally_kernel<<<256, 256, 0, s>>>(n, b.gcnt.p, b.fill.p, cap, b.tally.p);
+            CSG_LAUNCH_CHECK();
+        }
         {
             std::unique_ptr<ScopedKernelTimer> t;
             if (gp) t.reset(new ScopedKernelTimer(*gp, "nnd_merge"));
-            merge_kernel<<<static_cast<unsigned>(std::min<std::size_t>(n, 65536)), 64, 0, s>>>(
-                n, k, ids, dist, flags, count, b.cand.p, b.fill.p, cap, b.updates.p);
+            merge_kernel<<<merge_blocks, 64, 0, s>>>(n, k, ids, dist, flags, count, b.cand.p,
+                                                     b.fill.p, cap, b.partial.p);
             CSG_LAUNCH_CHECK();
         }
-        unsigned long long upd = 0;
+        sum_partials_kernel<<<1, 256, 0, s>>>(b.partial.p, merge_blocks, b.updates.p);
+        CSG_LAUNCH_CHECK();
+        unsigned long long upd = 0, tl[3] = {0, 0, 0};
         CSG_CHECK(hipMemcpyAsync(&upd, b.updates.p, 8, hipMemcpyDeviceToHost, s));
+        if (tallies) CSG_CHECK(hipMemcpyAsync(tl, b.tally.p, sizeof(tl), hipMemcpyDeviceToHost, s));
         CSG_CHECK(hipStreamSynchronize(s));
+        if (tallies) gp->add_work("nnd_join", {tl[0], tl[1], tl[2], n});
         res.updates += upd;
         res.rounds = round + 1;
         if (double(upd) <= stop) break;
