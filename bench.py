@@ -1,15 +1,17 @@
 """CORE-SG benchmark on one MI355X.
 
 A step is one pass of the hot path over one synthetic batch resident in HBM:
-build the CORE-SG (k-NN descent + hubness + anti-hub bridges) and extract the
-full mpts sweep (one MRD reweight + Boruvka MST per mpts). Default workload:
+build the CORE-SG (k-NN descent + hubness + anti-hub bridges), lay out its edge
+list and adjacency, and extract the full mpts sweep (one MRD reweight + Boruvka
+MST per mpts, all mpts in one call). Default workload:
 n = 1,000,000 points, d = 24, a 32-component Gaussian mixture (std 0.2-3.0,
 2% uniform noise, seed 1), k_max = 32, NN-Descent sample 16 / 10 iterations,
 mpts 1..32.
 
 Prints one JSON line. `value` is the build rate in points/s; the sweep rate
-(MSTs/s) and the per-MST extraction latency percentiles sit beside it, with
-the dominant kernel's roofline fraction (device time from HIP events on the
+(MSTs/s), the same sweep as one call per mpts (the plain form), and the
+latency percentiles of single-mpts extract queries sit beside it, with the
+dominant kernel's roofline fraction (device time from HIP events on the
 library's stream) and the reference CPU path timed on this host.
 
     python bench.py [--steps K] [--warmup W] [--n N] [--no-cpu-baseline]
@@ -134,14 +136,22 @@ def main():
         st = g.build(params)
         t1 = time.perf_counter()
         g.prepare_edges()
+        res = g.extract(mpts)  # the whole sweep in one call (batched slots)
+        t2 = time.perf_counter()
+        if record is not None:
+            record.append(dict(build_s=t1 - t0, sweep_s=t2 - t1, stats=st,
+                               weights=[r["weight"] for r in res]))
+
+    def single_queries():
+        """extract(mpts) latency of one-mpts queries against the resident
+        CORE-SG, and the plain form of the sweep (one call per mpts)."""
         lat = []
+        t0 = time.perf_counter()
         for m in mpts:
             ta = time.perf_counter()
             g.extract([m])
             lat.append((time.perf_counter() - ta) * 1e3)
-        t2 = time.perf_counter()
-        if record is not None:
-            record.append(dict(build_s=t1 - t0, sweep_s=t2 - t1, lat_ms=lat, stats=st))
+        return lat, time.perf_counter() - t0
 
     for w in range(args.warmup):
         step(None)
@@ -159,8 +169,13 @@ def main():
 
     build_s = float(np.mean([r["build_s"] for r in rec]))
     sweep_s = float(np.mean([r["sweep_s"] for r in rec]))
-    lat = [v for r in rec for v in r["lat_ms"]]
     st = rec[-1]["stats"]
+    lat, plain_s = [], []
+    for _ in range(args.steps):
+        l1, p1 = single_queries()
+        lat += l1
+        plain_s.append(p1)
+    plain_s = float(np.mean(plain_s))
 
     kernels = {}
     for name in ("nnd_join", "nnd_gather", "nnd_sample", "nnd_merge", "boruvka_round1", "boruvka_round2",
@@ -190,12 +205,14 @@ def main():
                          fp32_frac=flops / avg_s / 1e12 / FP32_PEAK_TFLOPS,
                          members_per_launch=rows / launches, proposals_per_launch=props / launches)
         else:
-            # per launch: every adjacency entry (neighbour, rank, base weight:
-            # 12 B) plus the neighbour's core distance (4 B); per vertex its
-            # two offsets, core distance, best key and active flag (21 B)
-            entries, verts = w[0], w[1]
-            algo = (entries * 16 + verts * 21) / launches
-            extra = dict(adjacency_entries=entries / launches)
+            # per launch (one launch serves B mpts slots): every adjacency
+            # entry (neighbour, rank, base weight: 12 B) plus the neighbour's
+            # core distance per slot (4 B each); per vertex its two offsets
+            # (8 B) and per slot its core distance, best key and active flag
+            # (13 B)
+            entries, verts, slots = w[0], w[1], max(1, w[2] // max(1, launches))
+            algo = (entries * (12 + 4 * slots) + verts * (8 + 13 * slots)) / launches
+            extra = dict(adjacency_entries=entries / launches, slots=slots)
         achieved = algo / avg_s / 1e9
         rooflines[name] = dict(kernel=name, bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS,
                                unit="GB/s", frac=achieved / HBM_PEAK_GBS,
@@ -238,6 +255,8 @@ def main():
         "build_ms": build_s * 1e3,
         "sweep_ms": sweep_s * 1e3,
         "sweep_msts_per_s": len(mpts) / sweep_s,
+        "sweep_plain_ms": plain_s * 1e3,
+        "sweep_fused_over_plain": plain_s / sweep_s,
         "extract_p50_ms": percentile(lat, 50),
         "extract_p99_ms": percentile(lat, 99),
         "knn_rounds": st["rounds"],

@@ -84,6 +84,11 @@ struct Graph {
     std::vector<std::uint32_t> host_counters;
     DBuf<std::uint32_t> scan_tmp;
     DBuf<std::uint32_t> out_pos;
+    // mpts sweep: per-block per-slot partials, per-slot results, slot mpts,
+    // and one slot's outputs
+    DBuf<std::uint64_t> slot_part, slot_out;
+    DBuf<std::uint32_t> slot_mpts, label_buf, out_u, out_v;
+    DBuf<float> out_w;
 
     // Hubness / selection state (filled by the build).
     DBuf<std::int32_t> rdeg;
@@ -120,6 +125,7 @@ struct ScopedKernelTimer {
 
 // extraction (extract.hip)
 void prepare_edges(Graph& g);
+void check_mpts(const Graph& g, std::uint32_t* mpts);
 void extract_msts(Graph& g, const std::uint32_t* mpts, std::uint32_t count, void* info,
                   std::uint32_t* out_u, std::uint32_t* out_v, float* out_w,
                   std::uint32_t* out_labels);

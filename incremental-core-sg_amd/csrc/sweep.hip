@@ -1,0 +1,712 @@
+// The mpts sweep: core distances, MRD reweight and MST for a batch of mpts
+// values in one pass over the resident CORE-SG.
+//
+// Replaces, on device, the reference's per-mpts loop (CoreSG::mst / all_msts,
+// coresg.cpp:408-417; coresg_cli.cpp:553-570), each iteration of which
+// rebuilds the reweighted edge list (coresg.cpp:391-406, knn_graph.cpp:219-235)
+// and runs mst_from_edges (mst.cpp:61-156) on it.
+//
+// Up to kMaxSlots mpts values ("slots") run together. Per-slot state is
+// interleaved by vertex -- comp[v*B + b], core[v*B + b] -- so the values a
+// Boruvka round needs for one neighbour and every slot sit in one cache line,
+// and each adjacency entry read from HBM serves all B slots.
+//
+// Boruvka with the key (weight bits << 32 | rank): rank is the position in the
+// (u, v)-sorted list, so keys are unique and order edges exactly as edge_less
+// (types.hpp:38-42); the tree found is the unique MST under that order, the
+// reference's Kruskal tree edge for edge. Every component root hooks along
+// its minimum outgoing edge (the smaller root of a mutual pair stays), each
+// vertex hooks at most once in its lifetime, and the key it hooked with is
+// kept per (vertex, slot): the MST is exactly the set of those keys.
+//
+// The total weight is the reference's double sum in Kruskal order
+// (types.hpp:58-62), computed exactly: when every MST weight is M * 2^e with
+// M odd and the sum of M * 2^(e - e_min) stays below 2^53, every partial sum of
+// the sequential double accumulation is representable, so the sequential
+// result equals that integer sum. The rare batch that fails the test is summed
+// on the host in Kruskal order.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/coresg_hip.h"
+#include "graph.hpp"
+#include "mrd_key.hpp"
+#include "scan.hpp"
+
+namespace coresg_hip {
+namespace {
+
+constexpr unsigned kThreads = 256;
+constexpr std::uint32_t kMaxSlots = 32;
+constexpr unsigned kReduceBlocks = 1024;  // blocks of the per-slot reducing kernels
+
+constexpr std::uint32_t kErrEdgeIndex = 1u, kErrHookCycle = 2u;
+constexpr std::uint64_t kOver = 1ull << 63;  // saturated checked sum
+constexpr std::uint64_t kExpInf = 1ull << 32;
+
+struct Sweep {
+    std::uint32_t n, k, B, Bp;  // points, list length, slots, slots rounded up to a power of 2
+    std::uint64_t m;            // CORE-SG edges
+    std::uint32_t live;         // bit b: slot b still running
+    const std::uint32_t* off;   // adjacency
+    const std::uint32_t* nbr;
+    const std::uint32_t* rank;
+    const float* w;
+    const std::uint32_t* eu;  // edge list (by rank)
+    const std::uint32_t* ev;
+    float* core;  // n*B, interleaved
+    std::uint32_t* comp;
+    std::uint32_t* parent;
+    std::uint32_t* newroot;
+    std::uint64_t* best;
+    std::uint64_t* mkey;  // key v hooked with as a root, per slot
+    std::uint8_t* active;
+    std::uint64_t* part;  // kReduceBlocks*B per-block partials
+    std::uint32_t* err;
+};
+
+// Elements i = v*B + b are visited with a grid stride that is a multiple of B,
+// so a thread keeps one slot for the whole kernel and can accumulate in
+// registers; threads beyond that stride idle.
+struct SlotStride {
+    std::size_t first, stride;
+    bool idle;
+};
+
+__device__ __forceinline__ SlotStride slot_stride(std::uint32_t B) {
+    const std::size_t total_threads = std::size_t(gridDim.x) * blockDim.x;
+    const std::size_t stride = total_threads - total_threads % B;
+    const std::size_t t = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    return SlotStride{t, stride, t >= stride};
+}
+
+__device__ __forceinline__ std::uint64_t checked_add(std::uint64_t a, std::uint64_t b) {
+    if ((a | b) & kOver) return kOver;
+    const std::uint64_t s = a + b;
+    return (s >= (1ull << 53)) ? kOver : s;
+}
+
+struct OpAdd {
+    __device__ std::uint64_t operator()(std::uint64_t a, std::uint64_t b) const {
+        return checked_add(a, b);
+    }
+};
+// low 32 bits: minimum; bit 32: any-inf flag
+struct OpExp {
+    __device__ std::uint64_t operator()(std::uint64_t a, std::uint64_t b) const {
+        const std::uint64_t la = a & 0xFFFFFFFFull, lb = b & 0xFFFFFFFFull;
+        return (la < lb ? la : lb) | ((a | b) & kExpInf);
+    }
+};
+
+// Per-slot reduction of one value per thread into part[blockIdx.x*B + b].
+// Every thread of the block must call it.
+template <class Op>
+__device__ void slot_block_reduce(std::uint64_t v, std::uint32_t B, std::uint64_t identity,
+                                  Op op, std::uint64_t* part) {
+    __shared__ std::uint64_t s_val[kThreads];
+    s_val[threadIdx.x] = v;
+    __syncthreads();
+    if (threadIdx.x < B) {
+        const std::size_t base = std::size_t(blockIdx.x) * blockDim.x;
+        std::uint64_t acc = identity;
+        for (std::uint32_t t = 0; t < blockDim.x; ++t)
+            if ((base + t) % B == threadIdx.x) acc = op(acc, s_val[t]);
+        part[std::size_t(blockIdx.x) * B + threadIdx.x] = acc;
+    }
+    __syncthreads();
+}
+
+// One block per slot: folds the per-block partials.
+template <class Op>
+__global__ void __launch_bounds__(kThreads)
+    slot_final_kernel(const std::uint64_t* __restrict__ part, std::uint32_t nblocks,
+                      std::uint32_t B, std::uint64_t identity, std::uint64_t* __restrict__ out) {
+    __shared__ std::uint64_t s_val[kThreads];
+    const std::uint32_t b = blockIdx.x;
+    const Op op;
+    std::uint64_t acc = identity;
+    for (std::uint32_t i = threadIdx.x; i < nblocks; i += blockDim.x)
+        acc = op(acc, part[std::size_t(i) * B + b]);
+    s_val[threadIdx.x] = acc;
+    __syncthreads();
+    for (std::uint32_t h = blockDim.x / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) s_val[threadIdx.x] = op(s_val[threadIdx.x], s_val[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[b] = s_val[0];
+}
+
+// core_mpts(v) = dist[v*k + min(mpts, count) - 1], +inf for an empty list
+// (knn_graph.hpp:157-163); every vertex starts as its own component.
+__global__ void __launch_bounds__(kThreads)
+    sweep_init_kernel(Sweep s, const float* __restrict__ dist,
+                      const std::uint32_t* __restrict__ count,
+                      const std::uint32_t* __restrict__ mpts) {
+    const std::size_t total = std::size_t(s.n) * s.B;
+    for (std::size_t i = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += std::size_t(gridDim.x) * blockDim.x) {
+        const std::uint32_t v = static_cast<std::uint32_t>(i / s.B);
+        const std::uint32_t b = static_cast<std::uint32_t>(i - std::size_t(v) * s.B);
+        const std::uint32_t c = count[v];
+        float cd = __builtin_inff();
+        if (c > 0) {
+            const std::uint32_t pos = (mpts[b] < c ? mpts[b] : c) - 1;
+            cd = dist[std::size_t(v) * s.k + pos];
+        }
+        s.core[i] = cd;
+        s.comp[i] = v;
+        s.parent[i] = v;
+        s.best[i] = kNone64;
+        s.mkey[i] = kNone64;
+        s.active[i] = 1;
+    }
+}
+
+// Key of adjacency entry `raw` (bit 31: seen from the second endpoint) for a
+// slot, with the reweight max(w, max(core[u], core[v])) in the edge's order.
+__device__ __forceinline__ std::uint64_t entry_key(std::uint32_t raw, float w, float cx, float cq,
+                                                   std::uint32_t rank) {
+    return (raw & kSecondEnd) ? mrd_key(w, cq, cx, rank) : mrd_key(w, cx, cq, rank);
+}
+
+// One Boruvka round. A group of W lanes serves one vertex: lane sub handles
+// slot sub % Bp and every (W/Bp)-th adjacency entry from sub / Bp, so each
+// entry's neighbour, rank and weight are read once for all slots. The first
+// round stores each vertex's minimum directly (every vertex is a component);
+// later rounds skip neighbours in the same component and fold into the
+// component's best with a read-filtered 64-bit atomic-min. A vertex with no
+// edge leaving its component stays inside for good (components only merge)
+// and is deactivated.
+template <std::uint32_t W, bool FIRST>
+__global__ void __launch_bounds__(kThreads) sweep_round_kernel(Sweep s) {
+    const std::uint32_t lane = __lane_id();
+    const std::uint32_t grp = lane / W, sub = lane % W;
+    const std::uint32_t b = sub % s.Bp, ph = sub / s.Bp, P = W / s.Bp;
+    constexpr std::uint32_t per_wave = kWave / W;
+    const std::uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const std::uint32_t waves = (gridDim.x * blockDim.x) / kWave;
+    const std::uint64_t gmask = (W == 64) ? ~0ull : (((1ull << W) - 1ull) << (grp * W));
+    for (std::uint32_t vb = wave * per_wave; vb < s.n; vb += waves * per_wave) {
+        const std::uint32_t v = vb + grp;
+        const bool slot = v < s.n && b < s.B && ((s.live >> b) & 1u);
+        const std::size_t i = std::size_t(v) * s.B + b;
+        const bool act = slot && (FIRST || s.active[i] != 0);
+        std::uint64_t key_min = kNone64;
+        std::uint32_t cv = v;
+        if (__ballot(act) & gmask) {
+            float cx = 0.0f;
+            if (act) {
+                cx = s.core[i];
+                if (!FIRST) cv = s.comp[i];
+            }
+            const std::uint32_t e1 = s.off[v + 1];
+            std::uint32_t j = s.off[v] + ph;
+            if (act) {
+                // four entries in flight per lane
+                for (; j + 3 * P < e1; j += 4 * P) {
+                    std::uint32_t raw[4], rk[4];
+                    float wj[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        raw[u] = s.nbr[j + u * P];
+                        rk[u] = s.rank[j + u * P];
+                        wj[u] = s.w[j + u * P];
+                    }
+                    std::uint32_t cq_comp[4];
+                    float cq[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const std::size_t iq = std::size_t(raw[u] & ~kSecondEnd) * s.B + b;
+                        cq[u] = s.core[iq];
+                        cq_comp[u] = FIRST ? ~0u : s.comp[iq];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        if (!FIRST && cq_comp[u] == cv) continue;
+                        const std::uint64_t key = entry_key(raw[u], wj[u], cx, cq[u], rk[u]);
+                        key_min = key < key_min ? key : key_min;
+                    }
+                }
+                for (; j < e1; j += P) {
+                    const std::uint32_t raw = s.nbr[j];
+                    const std::size_t iq = std::size_t(raw & ~kSecondEnd) * s.B + b;
+                    if (!FIRST && s.comp[iq] == cv) continue;
+                    const std::uint64_t key = entry_key(raw, s.w[j], cx, s.core[iq], s.rank[j]);
+                    key_min = key < key_min ? key : key_min;
+                }
+            }
+        }
+        for (std::uint32_t o = s.Bp; o < W; o <<= 1) {
+            const std::uint64_t x = __shfl_xor(key_min, o, kWave);
+            key_min = x < key_min ? x : key_min;
+        }
+        if (ph == 0 && act) {
+            if (FIRST) {
+                s.best[i] = key_min;
+                s.active[i] = key_min != kNone64;
+            } else if (key_min == kNone64) {
+                s.active[i] = 0;
+            } else {
+                std::uint64_t* dst = &s.best[std::size_t(cv) * s.B + b];
+                if (key_min < __builtin_nontemporal_load(dst))
+                    atomicMin(reinterpret_cast<unsigned long long*>(dst), key_min);
+            }
+        }
+    }
+}
+
+// Every root with a best edge hooks onto the component across it, except the
+// smaller root of a mutual pair. Keys are unique, so hooks form a forest and
+// every hook adds its edge to the tree exactly once. Hooks per slot are
+// counted into per-block partials.
+__global__ void __launch_bounds__(kThreads) sweep_hook_kernel(Sweep s) {
+    const SlotStride st = slot_stride(s.B);
+    std::uint64_t hooks = 0;
+    const std::size_t total = std::size_t(s.n) * s.B;
+    if (!st.idle) {
+        for (std::size_t i = st.first; i < total; i += st.stride) {
+            const std::uint32_t v = static_cast<std::uint32_t>(i / s.B);
+            const std::uint32_t b = static_cast<std::uint32_t>(i - std::size_t(v) * s.B);
+            if (!((s.live >> b) & 1u) || s.comp[i] != v) continue;
+            const std::uint64_t key = s.best[i];
+            if (key == kNone64) continue;
+            const std::uint32_t e = static_cast<std::uint32_t>(key);
+            if (e >= s.m) {
+                atomicOr(s.err, kErrEdgeIndex);
+                continue;
+            }
+            const std::uint32_t a = s.comp[std::size_t(s.eu[e]) * s.B + b];
+            const std::uint32_t c = s.comp[std::size_t(s.ev[e]) * s.B + b];
+            const std::uint32_t other = (a == v) ? c : a;
+            if (s.best[std::size_t(other) * s.B + b] == key && v < other) continue;
+            s.parent[i] = other;
+            s.mkey[i] = key;
+            ++hooks;
+        }
+    }
+    slot_block_reduce(st.idle ? 0ull : hooks, s.B, 0ull, OpAdd(), s.part);
+}
+
+// Final root of every hooked root (chains end at a root that did not hook).
+__global__ void __launch_bounds__(kThreads) sweep_jump_kernel(Sweep s) {
+    const std::size_t total = std::size_t(s.n) * s.B;
+    for (std::size_t i = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += std::size_t(gridDim.x) * blockDim.x) {
+        const std::uint32_t v = static_cast<std::uint32_t>(i / s.B);
+        const std::uint32_t b = static_cast<std::uint32_t>(i - std::size_t(v) * s.B);
+        if (!((s.live >> b) & 1u) || s.comp[i] != v) continue;
+        // The bound only guards against a corrupted parent array.
+        std::uint32_t r = v, it = 0;
+        for (; it < s.n; ++it) {
+            const std::uint32_t p = s.parent[std::size_t(r) * s.B + b];
+            if (p == r) break;
+            r = p;
+        }
+        if (it == s.n) atomicOr(s.err, kErrHookCycle);
+        s.newroot[i] = r;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) sweep_relabel_kernel(Sweep s) {
+    const std::size_t total = std::size_t(s.n) * s.B;
+    for (std::size_t i = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += std::size_t(gridDim.x) * blockDim.x) {
+        const std::uint32_t v = static_cast<std::uint32_t>(i / s.B);
+        const std::uint32_t b = static_cast<std::uint32_t>(i - std::size_t(v) * s.B);
+        if (!((s.live >> b) & 1u)) continue;
+        s.comp[i] = s.newroot[std::size_t(s.comp[i]) * s.B + b];
+        s.parent[i] = v;
+        s.best[i] = kNone64;
+    }
+}
+
+// Decompose a non-negative finite float as M * 2^e with M odd.
+__device__ __forceinline__ void odd_decompose(std::uint32_t bits, std::uint32_t* M, int* e) {
+    const std::uint32_t E = (bits >> 23) & 0xFFu;
+    std::uint32_t mant = bits & 0x7FFFFFu;
+    int ex;
+    if (E == 0) {
+        ex = -149;
+    } else {
+        mant |= 0x800000u;
+        ex = int(E) - 150;
+    }
+    const int tz = __builtin_ctz(mant);
+    *M = mant >> tz;
+    *e = ex + tz;
+}
+
+// Per slot: smallest exponent of a positive finite MST weight (+1024), and
+// whether any weight is infinite.
+__global__ void __launch_bounds__(kThreads) sweep_wexp_kernel(Sweep s) {
+    const SlotStride st = slot_stride(s.B);
+    std::uint64_t acc = 0xFFFFFFFFull;
+    const std::size_t total = std::size_t(s.n) * s.B;
+    if (!st.idle) {
+        for (std::size_t i = st.first; i < total; i += st.stride) {
+            const std::uint64_t key = s.mkey[i];
+            if (key == kNone64) continue;
+            const std::uint32_t bits = static_cast<std::uint32_t>(key >> 32);
+            if ((bits & 0x7FFFFFFFu) == 0) continue;
+            if ((bits & 0x7F800000u) == 0x7F800000u) {
+                acc |= kExpInf;
+                continue;
+            }
+            std::uint32_t M;
+            int e;
+            odd_decompose(bits, &M, &e);
+            const std::uint64_t enc = static_cast<std::uint32_t>(e + 1024);
+            acc = OpExp()(acc, enc);
+        }
+    }
+    slot_block_reduce(acc, s.B, 0xFFFFFFFFull, OpExp(), s.part);
+}
+
+// Per slot: sum of M << (e - e_min) with saturation at 2^53 (kOver).
+__global__ void __launch_bounds__(kThreads)
+    sweep_wsum_kernel(Sweep s, const std::uint64_t* __restrict__ emin) {
+    const SlotStride st = slot_stride(s.B);
+    std::uint64_t acc = 0;
+    const std::size_t total = std::size_t(s.n) * s.B;
+    if (!st.idle) {
+        const std::uint32_t b = static_cast<std::uint32_t>(st.first % s.B);
+        const std::uint64_t ex = emin[b];
+        const bool none = (ex & 0xFFFFFFFFull) == 0xFFFFFFFFull || (ex & kExpInf);
+        const int e_min = int(ex & 0xFFFFFFFFull) - 1024;
+        if (!none) {
+            for (std::size_t i = st.first; i < total; i += st.stride) {
+                const std::uint64_t key = s.mkey[i];
+                if (key == kNone64) continue;
+                const std::uint32_t bits = static_cast<std::uint32_t>(key >> 32);
+                if ((bits & 0x7FFFFFFFu) == 0) continue;
+                std::uint32_t M;
+                int e;
+                odd_decompose(bits, &M, &e);
+                const int sh = e - e_min;
+                const int len = 32 - __builtin_clz(M);
+                acc = (sh + len > 53) ? kOver
+                                      : checked_add(acc, static_cast<std::uint64_t>(M) << sh);
+            }
+        }
+    }
+    slot_block_reduce(acc, s.B, 0ull, OpAdd(), s.part);
+}
+
+// ---- outputs of one slot ----------------------------------------------------
+
+__global__ void __launch_bounds__(kThreads)
+    slot_flag_kernel(const std::uint64_t* __restrict__ mkey, std::uint32_t n, std::uint32_t B,
+                     std::uint32_t b, std::uint64_t m, std::uint8_t* __restrict__ flag) {
+    for (std::uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += gridDim.x * blockDim.x) {
+        const std::uint64_t key = mkey[std::size_t(v) * B + b];
+        if (key != kNone64 && static_cast<std::uint32_t>(key) < m)
+            flag[static_cast<std::uint32_t>(key)] = 1;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    flag_to_u32_kernel(const std::uint8_t* __restrict__ flag, std::uint64_t m,
+                       std::uint32_t* __restrict__ out) {
+    for (std::uint64_t i = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m;
+         i += std::uint64_t(gridDim.x) * blockDim.x)
+        out[i] = flag[i];
+}
+
+// Ordered compaction of the flagged edges with their reweighted weight.
+__global__ void __launch_bounds__(kThreads)
+    gather_mst_kernel(const std::uint8_t* __restrict__ flag, const std::uint32_t* __restrict__ pos,
+                      std::uint64_t m, const std::uint32_t* __restrict__ eu,
+                      const std::uint32_t* __restrict__ ev, const float* __restrict__ ew,
+                      const float* __restrict__ core, std::uint32_t B, std::uint32_t b,
+                      std::uint32_t* __restrict__ ou, std::uint32_t* __restrict__ ov,
+                      float* __restrict__ ow) {
+    for (std::uint64_t e = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
+         e += std::uint64_t(gridDim.x) * blockDim.x) {
+        if (!flag[e]) continue;
+        const std::uint32_t p = pos[e];
+        const std::uint32_t u = eu[e], v = ev[e];
+        ou[p] = u;
+        ov[p] = v;
+        ow[p] = fmax_ref(ew[e], fmax_ref(core[std::size_t(u) * B + b], core[std::size_t(v) * B + b]));
+    }
+}
+
+// Smallest member of every component of one slot (read-filtered atomic-min:
+// members are visited in ascending order, so most find a smaller one stored).
+__global__ void __launch_bounds__(kThreads)
+    slot_minv_kernel(const std::uint32_t* __restrict__ comp, std::uint32_t n, std::uint32_t B,
+                     std::uint32_t b, std::uint32_t* __restrict__ minv) {
+    for (std::uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += gridDim.x * blockDim.x) {
+        const std::uint32_t c = comp[std::size_t(v) * B + b];
+        if (v < __builtin_nontemporal_load(&minv[c])) atomicMin(&minv[c], v);
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    slot_first_kernel(const std::uint32_t* __restrict__ comp, std::uint32_t n, std::uint32_t B,
+                      std::uint32_t b, const std::uint32_t* __restrict__ minv,
+                      std::uint32_t* __restrict__ first) {
+    for (std::uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += gridDim.x * blockDim.x)
+        first[v] = (minv[comp[std::size_t(v) * B + b]] == v) ? 1u : 0u;
+}
+
+// Dense labels by first appearance (mst.cpp:48-59): the label of v is the
+// number of components whose smallest member precedes that of v's component.
+__global__ void __launch_bounds__(kThreads)
+    slot_label_kernel(const std::uint32_t* __restrict__ comp, std::uint32_t n, std::uint32_t B,
+                      std::uint32_t b, const std::uint32_t* __restrict__ minv,
+                      const std::uint32_t* __restrict__ first_scan,
+                      std::uint32_t* __restrict__ labels) {
+    for (std::uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += gridDim.x * blockDim.x)
+        labels[v] = first_scan[minv[comp[std::size_t(v) * B + b]]];
+}
+
+std::uint32_t pow2_at_least(std::uint32_t x) {
+    std::uint32_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+void launch_round(const Sweep& sw, bool first, unsigned blocks, hipStream_t s) {
+    const bool wide = sw.Bp > 16;
+    if (first) {
+        if (wide)
+            sweep_round_kernel<32, true><<<blocks, kThreads, 0, s>>>(sw);
+        else
+            sweep_round_kernel<16, true><<<blocks, kThreads, 0, s>>>(sw);
+    } else {
+        if (wide)
+            sweep_round_kernel<32, false><<<blocks, kThreads, 0, s>>>(sw);
+        else
+            sweep_round_kernel<16, false><<<blocks, kThreads, 0, s>>>(sw);
+    }
+    CSG_LAUNCH_CHECK();
+}
+
+// Exact double total of one slot's MST from the device statistics, or the
+// host's Kruskal-order sum when the integer sum saturated.
+double slot_weight(Graph& g, const Sweep& sw, std::uint32_t b, std::uint64_t ex,
+                   std::uint64_t sum) {
+    if (ex & kExpInf) return HUGE_VAL;
+    if ((ex & 0xFFFFFFFFull) == 0xFFFFFFFFull) return 0.0;
+    if (!(sum & kOver)) return std::ldexp(static_cast<double>(sum), int(ex & 0xFFFFFFFFull) - 1024);
+    std::vector<std::uint64_t> keys(sw.n);
+    CSG_CHECK(hipMemcpy2DAsync(keys.data(), 8, sw.mkey + b, std::size_t(sw.B) * 8, 8, sw.n,
+                               hipMemcpyDeviceToHost, g.stream));
+    CSG_CHECK(hipStreamSynchronize(g.stream));
+    keys.erase(std::remove(keys.begin(), keys.end(), kNone64), keys.end());
+    std::sort(keys.begin(), keys.end());  // Kruskal order: (weight bits, rank)
+    double total = 0.0;
+    for (std::uint64_t key : keys) {
+        const std::uint32_t bits = static_cast<std::uint32_t>(key >> 32);
+        float w;
+        std::memcpy(&w, &bits, 4);
+        total += static_cast<double>(w);
+    }
+    return total;
+}
+
+void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_mst_info* info,
+               std::uint32_t* out_u, std::uint32_t* out_v, float* out_w,
+               std::uint32_t* out_labels) {
+    const std::uint32_t n = g.n;
+    const std::uint64_t m = g.num_edges;
+    hipStream_t s = g.stream;
+    const std::size_t nb = std::size_t(n) * B;
+    g.core.reserve(nb);
+    g.comp.reserve(nb);
+    g.parent.reserve(nb);
+    g.newroot.reserve(nb);
+    g.best.reserve(nb);
+    g.mst_keys.reserve(nb);
+    g.active.reserve(nb);
+    g.slot_part.reserve(std::size_t(kReduceBlocks) * kMaxSlots);
+    g.slot_out.reserve(4 * kMaxSlots);
+    g.slot_mpts.reserve(kMaxSlots);
+    g.counters.reserve(16);
+
+    std::uint32_t hm[kMaxSlots];
+    for (std::uint32_t b = 0; b < B; ++b) {
+        hm[b] = mpts[b];
+        check_mpts(g, &hm[b]);
+        info[b].mpts = hm[b];
+    }
+    CSG_CHECK(hipMemcpyAsync(g.slot_mpts.p, hm, B * 4, hipMemcpyHostToDevice, s));
+    fill_u32(g.counters.p, 16, 0u, s);
+
+    Sweep sw{};
+    sw.n = n;
+    sw.k = g.k;
+    sw.B = B;
+    sw.Bp = pow2_at_least(B);
+    sw.m = m;
+    sw.live = (B == 32) ? 0xFFFFFFFFu : ((1u << B) - 1u);
+    sw.off = g.adj_off.p;
+    sw.nbr = g.adj_nbr.p;
+    sw.rank = g.adj_rank.p;
+    sw.w = g.adj_w.p;
+    sw.eu = g.eu.p;
+    sw.ev = g.ev.p;
+    sw.core = g.core.p;
+    sw.comp = g.comp.p;
+    sw.parent = g.parent.p;
+    sw.newroot = g.newroot.p;
+    sw.best = g.best.p;
+    sw.mkey = g.mst_keys.p;
+    sw.active = g.active.p;
+    sw.part = g.slot_part.p;
+    sw.err = g.counters.p;
+
+    const unsigned eblk = capped_blocks(nb, kThreads);
+    const unsigned vlanes = sw.Bp > 16 ? 32u : 16u;
+    const unsigned vblk = capped_blocks(std::size_t(n) * vlanes, kThreads, 16384);
+    sweep_init_kernel<<<eblk, kThreads, 0, s>>>(sw, g.dist.p, g.count.p, g.slot_mpts.p);
+    CSG_LAUNCH_CHECK();
+
+    std::uint64_t total[kMaxSlots] = {};
+    std::uint32_t rounds[kMaxSlots] = {};
+    std::uint64_t* d_cnt = g.slot_out.p;
+    for (std::uint32_t round = 1; sw.live; ++round) {
+        {
+            if (g.profiling && round == 1) g.add_work("boruvka_round1", {2 * m, n, B});
+            ScopedKernelTimer t(g, round == 1   ? "boruvka_round1"
+                                   : round == 2 ? "boruvka_round2"
+                                                : "boruvka_round_live");
+            launch_round(sw, round == 1, vblk, s);
+        }
+        sweep_hook_kernel<<<kReduceBlocks, kThreads, 0, s>>>(sw);
+        CSG_LAUNCH_CHECK();
+        slot_final_kernel<OpAdd><<<B, kThreads, 0, s>>>(sw.part, kReduceBlocks, B, 0ull, d_cnt);
+        CSG_LAUNCH_CHECK();
+        sweep_jump_kernel<<<eblk, kThreads, 0, s>>>(sw);
+        CSG_LAUNCH_CHECK();
+        sweep_relabel_kernel<<<eblk, kThreads, 0, s>>>(sw);
+        CSG_LAUNCH_CHECK();
+        std::uint64_t cnt[kMaxSlots];
+        std::uint32_t err = 0;
+        CSG_CHECK(hipMemcpyAsync(cnt, d_cnt, B * 8, hipMemcpyDeviceToHost, s));
+        CSG_CHECK(hipMemcpyAsync(&err, g.counters.p, 4, hipMemcpyDeviceToHost, s));
+        CSG_CHECK(hipStreamSynchronize(s));
+        if (err)
+            throw std::runtime_error("extract: Boruvka state corrupted (error bits " +
+                                     std::to_string(err) + ")");
+        for (std::uint32_t b = 0; b < B; ++b) {
+            if (!((sw.live >> b) & 1u)) continue;
+            if (cnt[b] & kOver) throw std::runtime_error("extract: hook count overflow");
+            total[b] += cnt[b];
+            rounds[b] = round;
+            if (total[b] >= n) throw std::runtime_error("extract: more MST edges than n-1");
+            if (cnt[b] == 0 || total[b] + 1 >= n) sw.live &= ~(1u << b);
+        }
+    }
+
+    // exact totals
+    std::uint64_t* d_exp = g.slot_out.p + kMaxSlots;
+    std::uint64_t* d_sum = g.slot_out.p + 2 * kMaxSlots;
+    sweep_wexp_kernel<<<kReduceBlocks, kThreads, 0, s>>>(sw);
+    CSG_LAUNCH_CHECK();
+    slot_final_kernel<OpExp><<<B, kThreads, 0, s>>>(sw.part, kReduceBlocks, B, 0xFFFFFFFFull,
+                                                    d_exp);
+    CSG_LAUNCH_CHECK();
+    sweep_wsum_kernel<<<kReduceBlocks, kThreads, 0, s>>>(sw, d_exp);
+    CSG_LAUNCH_CHECK();
+    slot_final_kernel<OpAdd><<<B, kThreads, 0, s>>>(sw.part, kReduceBlocks, B, 0ull, d_sum);
+    CSG_LAUNCH_CHECK();
+    std::uint64_t hexp[kMaxSlots], hsum[kMaxSlots];
+    CSG_CHECK(hipMemcpyAsync(hexp, d_exp, B * 8, hipMemcpyDeviceToHost, s));
+    CSG_CHECK(hipMemcpyAsync(hsum, d_sum, B * 8, hipMemcpyDeviceToHost, s));
+    CSG_CHECK(hipStreamSynchronize(s));
+    for (std::uint32_t b = 0; b < B; ++b) {
+        info[b].num_edges = static_cast<std::uint32_t>(total[b]);
+        info[b].num_components = n - static_cast<std::uint32_t>(total[b]);
+        info[b].rounds = rounds[b];
+        info[b].weight = slot_weight(g, sw, b, hexp[b], hsum[b]);
+    }
+
+    const bool want_edges = out_u || out_v || out_w;
+    if (!want_edges && !out_labels) return;
+    const unsigned nblk = capped_blocks(n, kThreads);
+    if (out_labels) {
+        g.minv.reserve(n);
+        g.labels_tmp.reserve(std::size_t(n) + 1);
+        g.label_buf.reserve(n);
+        g.ensure_scan_tmp(std::size_t(n) + 1);
+    }
+    if (want_edges) {
+        g.mst_flag.reserve(m ? m : 1);
+        g.out_pos.reserve(m + 1);
+        g.out_u.reserve(n);
+        g.out_v.reserve(n);
+        g.out_w.reserve(n);
+        g.ensure_scan_tmp(m + 1);
+    }
+    const std::size_t estride = n ? n - 1 : 0;
+    for (std::uint32_t b = 0; b < B; ++b) {
+        if (out_labels) {
+            fill_u32(g.minv.p, n, 0xFFFFFFFFu, s);
+            slot_minv_kernel<<<nblk, kThreads, 0, s>>>(g.comp.p, n, B, b, g.minv.p);
+            CSG_LAUNCH_CHECK();
+            slot_first_kernel<<<nblk, kThreads, 0, s>>>(g.comp.p, n, B, b, g.minv.p,
+                                                        g.labels_tmp.p);
+            CSG_LAUNCH_CHECK();
+            exclusive_scan_u32(g.labels_tmp.p, n, nullptr, g.scan_tmp.p, s);
+            slot_label_kernel<<<nblk, kThreads, 0, s>>>(g.comp.p, n, B, b, g.minv.p,
+                                                        g.labels_tmp.p, g.label_buf.p);
+            CSG_LAUNCH_CHECK();
+            CSG_CHECK(hipMemcpyAsync(out_labels + std::size_t(b) * n, g.label_buf.p,
+                                     std::size_t(n) * 4, hipMemcpyDeviceToHost, s));
+        }
+        if (want_edges && n > 0 && m > 0) {
+            const std::size_t nmst = total[b];
+            fill_u8(g.mst_flag.p, m, 0, s);
+            slot_flag_kernel<<<nblk, kThreads, 0, s>>>(g.mst_keys.p, n, B, b, m, g.mst_flag.p);
+            CSG_LAUNCH_CHECK();
+            flag_to_u32_kernel<<<capped_blocks(m, kThreads), kThreads, 0, s>>>(g.mst_flag.p, m,
+                                                                               g.out_pos.p);
+            CSG_LAUNCH_CHECK();
+            exclusive_scan_u32(g.out_pos.p, m, nullptr, g.scan_tmp.p, s);
+            gather_mst_kernel<<<capped_blocks(m, kThreads), kThreads, 0, s>>>(
+                g.mst_flag.p, g.out_pos.p, m, g.eu.p, g.ev.p, g.ew.p, g.core.p, B, b, g.out_u.p,
+                g.out_v.p, g.out_w.p);
+            CSG_LAUNCH_CHECK();
+            if (out_u)
+                CSG_CHECK(hipMemcpyAsync(out_u + b * estride, g.out_u.p, nmst * 4,
+                                         hipMemcpyDeviceToHost, s));
+            if (out_v)
+                CSG_CHECK(hipMemcpyAsync(out_v + b * estride, g.out_v.p, nmst * 4,
+                                         hipMemcpyDeviceToHost, s));
+            if (out_w)
+                CSG_CHECK(hipMemcpyAsync(out_w + b * estride, g.out_w.p, nmst * 4,
+                                         hipMemcpyDeviceToHost, s));
+        }
+        // the per-slot scratch above is reused by the next slot
+        CSG_CHECK(hipStreamSynchronize(s));
+    }
+}
+
+}  // namespace
+
+void extract_msts(Graph& g, const std::uint32_t* mpts_list, std::uint32_t count, void* info_v,
+                  std::uint32_t* out_u, std::uint32_t* out_v, float* out_w,
+                  std::uint32_t* out_labels) {
+    auto* info = static_cast<coresg_hip_mst_info*>(info_v);
+    prepare_edges(g);
+    const std::size_t n = g.n, estride = n ? n - 1 : 0;
+    for (std::uint32_t q0 = 0; q0 < count; q0 += kMaxSlots) {
+        const std::uint32_t B = std::min<std::uint32_t>(kMaxSlots, count - q0);
+        run_batch(g, mpts_list + q0, B, info + q0, out_u ? out_u + q0 * estride : nullptr,
+                  out_v ? out_v + q0 * estride : nullptr, out_w ? out_w + q0 * estride : nullptr,
+                  out_labels ? out_labels + q0 * n : nullptr);
+    }
+}
+
+}  // namespace coresg_hip

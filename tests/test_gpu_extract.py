@@ -116,3 +116,26 @@ def test_extract_tiny_and_errors():
     assert r[0]["num_edges"] == 0 and r[0]["num_components"] == 3
     assert list(r[0]["labels"]) == [0, 1, 2]
     g.close()
+
+
+def test_batched_sweep_equals_single_queries():
+    """One extract call runs up to 32 mpts values together; the result of every
+    slot (edges, labels, weight, rounds) equals the same mpts extracted alone,
+    across a batch boundary, with repeats and mpts 0."""
+    print("[gpu] batched sweep vs single queries", flush=True)
+    gd = golden("mix_d24")
+    f = coresg_io.loads(gd["coresg_bytes"].tobytes())
+    g = graph_from_file(f)
+    k = f.k_max
+    mpts = [k, 1, 1, 7, 0, 3] + list(range(1, k + 1)) + [5, k, 2]
+    assert len(mpts) > 32
+    batched = g.extract(mpts, edges=True, labels=True)
+    for m, r in zip(mpts, batched):
+        one = g.extract([m], edges=True, labels=True)[0]
+        for key in ("mpts", "num_edges", "num_components", "rounds"):
+            assert r[key] == one[key], (m, key)
+        assert np.float64(r["weight"]).view(np.uint64) == np.float64(one["weight"]).view(np.uint64)
+        assert np.array_equal(r["u"], one["u"]) and np.array_equal(r["v"], one["v"])
+        assert np.array_equal(r["w"].view(np.uint32), one["w"].view(np.uint32))
+        assert np.array_equal(r["labels"], one["labels"])
+    g.close()
