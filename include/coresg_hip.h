@@ -186,6 +186,21 @@ int coresg_hip_build_bridges(coresg_hip_graph* g, const coresg_hip_build_params*
 int coresg_hip_build(coresg_hip_graph* g, const coresg_hip_build_params* p,
                      coresg_hip_build_stats* st);
 
+/* CoreSG::insert (coresg.cpp:325-389; IncrementalKnn::insert,
+ * incremental_knn.cpp:1155-1252): append `count` rows (layout as
+ * set_points) to a built CORE-SG. The new points' lists are their exact k
+ * nearest among all points, by (distance, id) as brute_force_knn orders them
+ * (nndescent.cpp:242-263); every new point is offered to the existing lists it
+ * belongs in (try_insert semantics, knn_graph.cpp:37-55); then hubness,
+ * selection, bridges and repair are recomputed over the grown graph (the
+ * reference refreshes them incrementally with identical results,
+ * test_hubness.cpp:95-163). With n <= brute_force_limit every list is rebuilt
+ * exactly, as the reference does (incremental_knn.cpp:1169-1180). */
+int coresg_hip_insert(coresg_hip_graph* g, const float* rows, uint32_t count, int where,
+                      int padded, const coresg_hip_build_params* p, coresg_hip_build_stats* st);
+/* Current number of points (grows with coresg_hip_insert). */
+uint32_t coresg_hip_graph_size(const coresg_hip_graph* g);
+
 size_t coresg_hip_graph_get_selected(coresg_hip_graph* g, uint32_t* out, size_t cap);
 int coresg_hip_graph_get_hubness(coresg_hip_graph* g, int32_t* rdeg, double* score,
                                  uint32_t* conn_labels, uint32_t* num_components);

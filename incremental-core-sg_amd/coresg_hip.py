@@ -109,6 +109,9 @@ SIGNATURES = [
                                                 C.POINTER(C.c_uint64)]),
     ("coresg_hip_graph_kernel_work", C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_uint64)]),
     ("coresg_hip_graph_reset_timers", C.c_int, [_vp]),
+    ("coresg_hip_insert", C.c_int, [_vp, C.c_void_p, C.c_uint32, C.c_int, C.c_int,
+                                    C.POINTER(BuildParams), C.POINTER(BuildStats)]),
+    ("coresg_hip_graph_size", C.c_uint32, [_vp]),
 ]
 
 
@@ -237,6 +240,20 @@ class Graph:
         p = params or build_params()
         st = BuildStats()
         check(lib().coresg_hip_build(self.h, C.byref(p), C.byref(st)))
+        return st.asdict()
+
+    def insert(self, rows: np.ndarray, params: BuildParams | None = None) -> dict:
+        """CoreSG::insert: append rows (n_new x dim, or x stride padded) and
+        update lists, hubness, selection and bridges."""
+        p = params or build_params()
+        rows = _f32(rows)
+        if rows.ndim != 2 or rows.shape[1] not in (self.dim, self.stride):
+            raise ValueError(f"rows must be (m, {self.dim}) or (m, {self.stride})")
+        padded = int(rows.shape[1] == self.stride and self.stride != self.dim)
+        st = BuildStats()
+        check(lib().coresg_hip_insert(self.h, ptr(rows), rows.shape[0], HOST, padded,
+                                      C.byref(p), C.byref(st)))
+        self.n = int(lib().coresg_hip_graph_size(self.h))
         return st.asdict()
 
     def build_knn(self, params: BuildParams | None = None) -> dict:

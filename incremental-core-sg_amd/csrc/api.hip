@@ -379,6 +379,17 @@ int coresg_hip_graph_reset_timers(coresg_hip_graph* h) {
     });
 }
 
+int coresg_hip_insert(coresg_hip_graph* h, const float* rows, uint32_t count, int where,
+                      int padded, const coresg_hip_build_params* p, coresg_hip_build_stats* st) {
+    return guarded([&] {
+        bind(h->g);
+        if (st) std::memset(st, 0, sizeof(*st));
+        insert_points(h->g, rows, count, where, padded, *p, st);
+    });
+}
+
+uint32_t coresg_hip_graph_size(const coresg_hip_graph* h) { return h ? h->g.n : 0; }
+
 int coresg_hip_build_knn(coresg_hip_graph* h, const coresg_hip_build_params* p,
                          coresg_hip_build_stats* st) {
     return guarded([&] {

@@ -9,6 +9,11 @@ namespace coresg_hip {
 
 void build_knn(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_stats* st);
 void build_bridges(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_stats* st);
+// CoreSG::insert (coresg.cpp:325-389): append rows, exact k-NN of the batch
+// against every point with the batch offered to the existing lists, then
+// hubness, selection, bridges and repair over the grown graph.
+void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, int padded,
+                   const coresg_hip_build_params& p, coresg_hip_build_stats* st);
 void sampled_recall(Graph& g, std::size_t queries, std::uint64_t seed, double* recall,
                     double* ratio);
 

@@ -1,6 +1,7 @@
 // Device-resident CORE-SG state shared by the build and extraction paths.
 #pragma once
 
+#include <algorithm>
 #include <initializer_list>
 #include <map>
 #include <string>
@@ -32,7 +33,32 @@ struct DBuf {
         CSG_CHECK(hipMalloc(reinterpret_cast<void**>(&p), bytes));
         n = count;
     }
+    // Grow to at least `count` elements keeping the first `keep` (copied on
+    // `s`); capacity grows by half again at least, so a stream of inserts
+    // reallocates O(log n) times.
+    void grow(std::size_t count, std::size_t keep, hipStream_t s) {
+        if (count <= n && p) return;
+        const std::size_t cap = std::max(count, n + n / 2);
+        T* q = nullptr;
+        CSG_CHECK(hipMalloc(reinterpret_cast<void**>(&q), (cap ? cap : 1) * sizeof(T)));
+        if (p && keep)
+            CSG_CHECK(hipMemcpyAsync(q, p, keep * sizeof(T), hipMemcpyDeviceToDevice, s));
+        CSG_CHECK(hipStreamSynchronize(s));
+        release();
+        p = q;
+        n = cap;
+    }
     std::size_t bytes() const { return n * sizeof(T); }
+};
+
+// NN-Descent round scratch: sampled lists, bounded reverse lists, gathered
+// candidates and the per-node proposal buffers (kept across builds).
+struct DescentBuffers {
+    DBuf<std::uint32_t> new_list, new_cnt, old_list, old_cnt;
+    DBuf<std::uint32_t> rev_new, rev_new_cnt, rev_old, rev_old_cnt;
+    DBuf<unsigned long long> cand, updates, partial, tally;
+    DBuf<std::uint32_t> fill, gids, gcnt;
+    DBuf<float> gthr;
 };
 
 // Accumulated device time of one kernel (HIP events on the graph's stream).
@@ -89,6 +115,8 @@ struct Graph {
     DBuf<std::uint64_t> slot_part, slot_out;
     DBuf<std::uint32_t> slot_mpts, label_buf, out_u, out_v;
     DBuf<float> out_w;
+
+    DescentBuffers nnd;
 
     // Hubness / selection state (filled by the build).
     DBuf<std::int32_t> rdeg;

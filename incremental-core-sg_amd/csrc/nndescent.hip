@@ -426,7 +426,8 @@ __global__ void __launch_bounds__(kJoinThreads) join_kernel(JoinArgs a) {
 constexpr std::uint32_t kMaxCap = 512;
 constexpr std::uint32_t kDefaultCap = 512;
 
-// One 64-thread group per node. The round's proposals
+// One 64-thread group per node (nodes [0, n), candidate ids below id_limit).
+// The round's proposals
 // cand[v*cap .. v*cap + min(fill, cap)) are sorted by (distance, id) with a
 // bitonic network in LDS; a proposal survives if it is the first copy of its
 // key, not the node itself, not already listed and (for a full list) closer
@@ -434,7 +435,8 @@ constexpr std::uint32_t kDefaultCap = 512;
 // k survivors are merged with the list by rank counting, existing entries
 // first among equal distances (the reference's sorted insertion).
 __global__ void __launch_bounds__(64)
-    merge_kernel(std::uint32_t n, std::uint32_t k, std::uint32_t* __restrict__ ids,
+    merge_kernel(std::uint32_t n, std::uint32_t id_limit, std::uint32_t k,
+                 std::uint32_t* __restrict__ ids,
                  float* __restrict__ dist, std::uint8_t* __restrict__ flags,
                  std::uint32_t* __restrict__ count, const unsigned long long* __restrict__ cand,
                  std::uint32_t* __restrict__ fill, std::uint32_t cap,
@@ -488,7 +490,7 @@ __global__ void __launch_bounds__(64)
                     const std::uint32_t cid = static_cast<std::uint32_t>(key);
                     const float cd = __uint_as_float(static_cast<std::uint32_t>(key >> 32));
                     ok = key != ~0ull && (i == 0 || s_key[i - 1] != key) && cid != v &&
-                         cid < n && cd < last;
+                         cid < id_limit && cd < last;
                     for (std::uint32_t j = 0; j < c && ok; ++j) ok = s_ex[j] != cid;
                 }
                 const std::uint64_t vm = __ballot(ok);
@@ -625,14 +627,6 @@ __global__ void brute_force_kernel(const float* __restrict__ pts, std::uint32_t 
     }
 }
 
-struct DescentBuffers {
-    DBuf<std::uint32_t> new_list, new_cnt, old_list, old_cnt;
-    DBuf<std::uint32_t> rev_new, rev_new_cnt, rev_old, rev_old_cnt;
-    DBuf<unsigned long long> cand, updates, partial, tally;
-    DBuf<std::uint32_t> fill, gids, gcnt;
-    DBuf<float> gthr;
-};
-
 using JoinFn = void (*)(JoinArgs);
 
 JoinFn pick_join(std::uint32_t stride) {
@@ -669,7 +663,8 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         CSG_LAUNCH_CHECK();
     }
     if (iters == 0) return res;
-    DescentBuffers b;
+    DescentBuffers local;
+    DescentBuffers& b = gp ? gp->nnd : local;
     b.new_list.reserve(std::size_t(n) * smp);
     b.new_cnt.reserve(n);
     b.old_list.reserve(std::size_t(n) * 3 * smp);
@@ -736,7 +731,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         {
             std::unique_ptr<ScopedKernelTimer> t;
             if (gp) t.reset(new ScopedKernelTimer(*gp, "nnd_merge"));
-            merge_kernel<<<merge_blocks, 64, 0, s>>>(n, k, ids, dist, flags, count, b.cand.p,
+            merge_kernel<<<merge_blocks, 64, 0, s>>>(n, n, k, ids, dist, flags, count, b.cand.p,
                                                      b.fill.p, cap, b.partial.p);
             CSG_LAUNCH_CHECK();
         }
@@ -752,6 +747,212 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         if (double(upd) <= stop) break;
     }
     return res;
+}
+
+// ---------------------------------------------------------------------------
+// batch insertion: exact k-NN of a batch of new points against every point,
+// and the new points proposed to the existing lists they belong in
+// ---------------------------------------------------------------------------
+
+constexpr unsigned kInsThreads = 256;  // 4 waves; lane = new point, wave = column phase
+constexpr unsigned kInsRows = 64;
+constexpr unsigned kInsTile = 128;     // columns staged in LDS per step
+
+struct InsertArgs {
+    const float* pts;
+    std::uint32_t n0, n1, k;  // existing points [0, n0), new points [n0, n1)
+    std::uint32_t* ids;
+    float* dist;
+    std::uint8_t* flags;
+    std::uint32_t* count;
+    unsigned long long* cand;  // proposal buffers of the existing points
+    std::uint32_t* fill;
+    std::uint32_t cap;
+};
+
+// (d, id) lexicographic order on metric distances: the reference's brute
+// force keeps the lower id among equal distances (nndescent.cpp:242-263).
+__device__ __forceinline__ bool key_less(float d, std::uint32_t i, float e, std::uint32_t j) {
+    return d < e || (d == e && i < j);
+}
+
+// Insert (d, id) into a sorted register list (unrolled: no dynamic indexing).
+template <std::uint32_t KP>
+__device__ __forceinline__ void list_insert(float (&td)[KP], std::uint32_t (&ti)[KP], float d,
+                                            std::uint32_t id) {
+#pragma unroll
+    for (int i = int(KP) - 1; i > 0; --i) {
+        const bool shift = key_less(d, id, td[i - 1], ti[i - 1]);
+        const bool here = !shift && key_less(d, id, td[i], ti[i]);
+        const float nd = shift ? td[i - 1] : (here ? d : td[i]);
+        const std::uint32_t ni = shift ? ti[i - 1] : (here ? id : ti[i]);
+        td[i] = nd;
+        ti[i] = ni;
+    }
+    if (key_less(d, id, td[0], ti[0])) {
+        td[0] = d;
+        ti[0] = id;
+    }
+}
+
+// One block per 64 new points. Every step stages kInsTile points in LDS; a
+// lane holds its new point's row in registers and computes the exact squared
+// distance (reference summation order) to every 4th staged point, keeping its
+// best KP in a sorted register list. The same distance offers the new point
+// to the staged point's list when that one is an existing point and the
+// distance beats its current k-th. The four partial lists of a new point are
+// merged at the end (waves 2,3 -> 0,1 -> 0) and its first k entries written.
+template <std::uint32_t STRIDE, std::uint32_t KP>
+__global__ void __launch_bounds__(kInsThreads) insert_scan_kernel(InsertArgs a) {
+    extern __shared__ float lds[];
+    float* s_tile = lds;                                 // kInsTile * STRIDE
+    float* s_thr = lds + kInsTile * STRIDE;              // kInsTile
+    float* s_ld = lds;                                   // merge phase: 2 * kInsRows * KP
+    std::uint32_t* s_li = reinterpret_cast<std::uint32_t*>(lds + 2 * kInsRows * KP);
+    const std::uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    for (std::uint32_t rb = a.n0 + blockIdx.x * kInsRows; rb < a.n1; rb += gridDim.x * kInsRows) {
+        const std::uint32_t p = rb + lane;
+        const bool has = p < a.n1;
+        float row[STRIDE];
+#pragma unroll
+        for (std::uint32_t f = 0; f < STRIDE / 4; ++f) {
+            const float4 t = has ? reinterpret_cast<const float4*>(a.pts + std::size_t(p) * STRIDE)[f]
+                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+            row[4 * f] = t.x;
+            row[4 * f + 1] = t.y;
+            row[4 * f + 2] = t.z;
+            row[4 * f + 3] = t.w;
+        }
+        float td[KP];  // metric distances, sorted by (d, id)
+        std::uint32_t ti[KP];
+#pragma unroll
+        for (std::uint32_t i = 0; i < KP; ++i) {
+            td[i] = __builtin_inff();
+            ti[i] = kNone;
+        }
+        // squared prefilter for td[KP-1], loosened by 2^-20 so that no pair
+        // whose rounded sqrt ties or beats it is skipped
+        float thr2 = __builtin_inff();
+        for (std::uint32_t c0 = 0; c0 < a.n1; c0 += kInsTile) {
+            __syncthreads();  // the previous tile is consumed
+            const std::uint32_t cend = min(kInsTile, a.n1 - c0);
+            for (std::uint32_t e = threadIdx.x; e < cend * (STRIDE / 4); e += kInsThreads) {
+                const std::uint32_t c = e / (STRIDE / 4), f = e - c * (STRIDE / 4);
+                reinterpret_cast<float4*>(s_tile)[e] =
+                    reinterpret_cast<const float4*>(a.pts + std::size_t(c0 + c) * STRIDE)[f];
+            }
+            for (std::uint32_t c = threadIdx.x; c < cend; c += kInsThreads) {
+                const std::uint32_t j = c0 + c;
+                float t = -1.0f;  // new points take no proposals here
+                if (j < a.n0) {
+                    t = __builtin_inff();
+                    if (a.count[j] >= a.k) {
+                        const float dk = a.dist[std::size_t(j) * a.k + a.k - 1];
+                        t = dk * dk * (1.0f + 0x1p-20f);  // the merge makes the exact test
+                    }
+                }
+                s_thr[c] = t;
+            }
+            __syncthreads();
+            for (std::uint32_t c = w; c < cend; c += 4) {
+                const std::uint32_t j = c0 + c;
+                const float d2 = l2sqr_reg<STRIDE>(row, s_tile + c * STRIDE);
+                if (!has || j == p) continue;
+                if (d2 <= thr2) {
+                    const float d = sqrt_exact(d2);
+                    if (key_less(d, j, td[KP - 1], ti[KP - 1])) {
+                        list_insert<KP>(td, ti, d, j);
+                        thr2 = td[KP - 1] * td[KP - 1] * (1.0f + 0x1p-20f);
+                    }
+                }
+                if (d2 < s_thr[c]) {
+                    const std::uint32_t pos = atomicAdd(&a.fill[j], 1u);
+                    if (pos < a.cap)
+                        a.cand[std::size_t(j) * a.cap + pos] =
+                            (static_cast<unsigned long long>(fbits(sqrt_exact(d2))) << 32) | p;
+                }
+            }
+        }
+        // merge the four partial lists of each new point
+        __syncthreads();
+        if (w >= 2) {
+#pragma unroll
+            for (std::uint32_t i = 0; i < KP; ++i) {
+                s_ld[((w - 2) * kInsRows + lane) * KP + i] = td[i];
+                s_li[((w - 2) * kInsRows + lane) * KP + i] = ti[i];
+            }
+        }
+        __syncthreads();
+        if (w < 2) {
+            for (std::uint32_t i = 0; i < KP; ++i) {
+                const float d = s_ld[(w * kInsRows + lane) * KP + i];
+                const std::uint32_t id = s_li[(w * kInsRows + lane) * KP + i];
+                if (id == kNone || !key_less(d, id, td[KP - 1], ti[KP - 1])) break;
+                list_insert<KP>(td, ti, d, id);
+            }
+        }
+        __syncthreads();
+        if (w == 1) {
+#pragma unroll
+            for (std::uint32_t i = 0; i < KP; ++i) {
+                s_ld[lane * KP + i] = td[i];
+                s_li[lane * KP + i] = ti[i];
+            }
+        }
+        __syncthreads();
+        if (w == 0) {
+            for (std::uint32_t i = 0; i < KP; ++i) {
+                const float d = s_ld[lane * KP + i];
+                const std::uint32_t id = s_li[lane * KP + i];
+                if (id == kNone || !key_less(d, id, td[KP - 1], ti[KP - 1])) break;
+                list_insert<KP>(td, ti, d, id);
+            }
+            if (has) {
+                std::uint32_t c = 0;
+#pragma unroll
+                for (std::uint32_t i = 0; i < KP; ++i) {
+                    if (i < a.k && ti[i] != kNone) {
+                        const std::size_t o = std::size_t(p) * a.k + i;
+                        a.ids[o] = ti[i];
+                        a.dist[o] = td[i];
+                        a.flags[o] = 1;
+                        c = i + 1;
+                    }
+                }
+                a.count[p] = c;
+            }
+        }
+    }
+}
+
+using InsertFn = void (*)(InsertArgs);
+
+template <std::uint32_t KP>
+InsertFn pick_insert_stride(std::uint32_t stride) {
+    switch (stride) {
+        case 8: return insert_scan_kernel<8, KP>;
+        case 16: return insert_scan_kernel<16, KP>;
+        case 24: return insert_scan_kernel<24, KP>;
+        case 32: return insert_scan_kernel<32, KP>;
+        case 40: return insert_scan_kernel<40, KP>;
+        case 48: return insert_scan_kernel<48, KP>;
+        case 56: return insert_scan_kernel<56, KP>;
+        case 64: return insert_scan_kernel<64, KP>;
+        default: return nullptr;
+    }
+}
+
+InsertFn pick_insert(std::uint32_t stride, std::uint32_t k, std::uint32_t* kp) {
+    if (k <= 16) {
+        *kp = 16;
+        return pick_insert_stride<16>(stride);
+    }
+    if (k <= 32) {
+        *kp = 32;
+        return pick_insert_stride<32>(stride);
+    }
+    *kp = 64;
+    return pick_insert_stride<64>(stride);
 }
 
 void check_join_capacity(std::uint32_t cap) {
@@ -830,6 +1031,105 @@ void nndescent_host(int device, const coresg_hip_nnd_config& cfg, const float* v
     CSG_CHECK(hipStreamSynchronize(s));
     CSG_CHECK(hipStreamDestroy(s));
     if (updates) *updates = r.updates;
+}
+
+void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, int padded,
+                   const coresg_hip_build_params& p, coresg_hip_build_stats* st) {
+    if (!g.knn_set) throw std::runtime_error("insert: the CORE-SG is not built");
+    const auto t0 = std::chrono::steady_clock::now();
+    if (count == 0) return;
+    const std::uint32_t n0 = g.n, k = g.k, stride = g.stride;
+    const std::uint64_t n1_64 = std::uint64_t(n0) + count;
+    if (n1_64 >= 0x7FFFFFFFull) throw std::runtime_error("insert: point count exceeds 2^31");
+    const std::uint32_t n1 = static_cast<std::uint32_t>(n1_64);
+    std::uint32_t kp = 0;
+    const InsertFn fn = pick_insert(stride, k, &kp);
+    const bool brute = p.brute_force_limit && n1 <= p.brute_force_limit;
+    if (!brute && !fn)
+        throw std::runtime_error("insert: rows wider than 64 floats are not supported on device");
+    if (k > kMaxK) throw std::runtime_error("insert: k above 64 is not supported on device");
+    hipStream_t s = g.stream;
+
+    // grow the resident arrays, keeping the existing points and lists
+    g.pts.grow(std::size_t(n1) * stride, std::size_t(n0) * stride, s);
+    g.ids.grow(std::size_t(n1) * k, std::size_t(n0) * k, s);
+    g.dist.grow(std::size_t(n1) * k, std::size_t(n0) * k, s);
+    g.flags.grow(std::size_t(n1) * k, std::size_t(n0) * k, s);
+    g.count.grow(n1, n0, s);
+    float* dst = g.pts.p + std::size_t(n0) * stride;
+    const hipMemcpyKind kind =
+        where == CORESG_HIP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (padded || g.dim == stride) {
+        CSG_CHECK(hipMemcpyAsync(dst, rows, std::size_t(count) * stride * 4, kind, s));
+    } else {
+        fill_u32(reinterpret_cast<std::uint32_t*>(dst), std::size_t(count) * stride, 0u, s);
+        CSG_CHECK(hipMemcpy2DAsync(dst, std::size_t(stride) * 4, rows, std::size_t(g.dim) * 4,
+                                   std::size_t(g.dim) * 4, count, kind, s));
+    }
+    const std::size_t nk_new = std::size_t(count) * k, base = std::size_t(n0) * k;
+    fill_u32(g.ids.p + base, nk_new, kNone, s);
+    fill_u32(reinterpret_cast<std::uint32_t*>(g.dist.p) + base, nk_new, 0x7F800000u, s);
+    fill_u8(g.flags.p + base, nk_new, 0, s);
+    fill_u32(g.count.p + n0, count, 0u, s);
+    g.n = n1;
+    g.knn_set = true;
+    g.edges_ready = false;
+    g.hub_ready = false;
+
+    std::uint64_t applied = 0;
+    if (brute) {
+        // exact lists for every point (incremental_knn.cpp:1169-1180)
+        brute_force_kernel<<<capped_blocks(n1, 64), 64, 0, s>>>(g.pts.p, n1, stride, k, g.ids.p,
+                                                                g.dist.p, g.flags.p, g.count.p);
+        CSG_LAUNCH_CHECK();
+    } else {
+        DescentBuffers& b = g.nnd;
+        const std::uint32_t cap = kDefaultCap;
+        b.cand.reserve(std::size_t(n0 ? n0 : 1) * cap);
+        b.fill.reserve(n0 ? n0 : 1);
+        b.partial.reserve(kMergeBlocks);
+        b.updates.reserve(1);
+        fill_u32(b.fill.p, n0 ? n0 : 1, 0u, s);
+        InsertArgs ia{g.pts.p, n0, n1, k, g.ids.p, g.dist.p, g.flags.p, g.count.p,
+                      b.cand.p, b.fill.p, cap};
+        const std::size_t lds_tile = (std::size_t(kInsTile) * stride + kInsTile) * sizeof(float);
+        const std::size_t lds_merge = std::size_t(4) * kInsRows * kp * sizeof(float);
+        const std::size_t lds = std::max(lds_tile, lds_merge);
+        if (lds > 64 * 1024)
+            CSG_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          static_cast<int>(lds)));
+        const unsigned blocks =
+            static_cast<unsigned>(std::min<std::size_t>((count + kInsRows - 1) / kInsRows, 65535));
+        {
+            ScopedKernelTimer t(g, "insert_scan");
+            hipLaunchKernelGGL(fn, dim3(blocks), dim3(kInsThreads), lds, s, ia);
+            CSG_LAUNCH_CHECK();
+        }
+        if (n0) {
+            const unsigned merge_blocks = static_cast<unsigned>(std::min<std::size_t>(n0, kMergeBlocks));
+            merge_kernel<<<merge_blocks, 64, 0, s>>>(n0, n1, k, g.ids.p, g.dist.p, g.flags.p,
+                                                     g.count.p, b.cand.p, b.fill.p, cap,
+                                                     b.partial.p);
+            CSG_LAUNCH_CHECK();
+            sum_partials_kernel<<<1, 256, 0, s>>>(b.partial.p, merge_blocks, b.updates.p);
+            CSG_LAUNCH_CHECK();
+            unsigned long long upd = 0;
+            CSG_CHECK(hipMemcpyAsync(&upd, b.updates.p, 8, hipMemcpyDeviceToHost, s));
+            CSG_CHECK(hipStreamSynchronize(s));
+            applied = upd;
+        }
+    }
+    CSG_CHECK(hipStreamSynchronize(s));
+    const double knn_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    build_bridges(g, p, st);
+    if (st) {
+        st->knn_seconds = knn_s;
+        st->rounds = 0;
+        st->updates = applied;
+        st->total_seconds =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
 }
 
 }  // namespace coresg_hip
