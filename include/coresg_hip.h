@@ -219,6 +219,73 @@ int coresg_hip_extract(coresg_hip_graph* g, const uint32_t* mpts, uint32_t count
                        coresg_hip_mst_info* info, uint32_t* out_u, uint32_t* out_v,
                        float* out_w, uint32_t* out_labels);
 
+/* ---- 3. serving session -------------------------------------------------- */
+
+/* One device-resident CORE-SG behind a request queue served by a worker
+ * thread that owns the device. The reference serves inserts and MST queries
+ * as separate CLI runs that reload the CORE-SG each time
+ * (coresg_cli.cpp:507-570); here the graph stays in HBM. Requests take effect
+ * in submission order: an extract sees exactly the inserts submitted before
+ * it. Extract requests waiting together are admitted into one fused sweep
+ * (up to 32 mpts values); every mpts is computed independently, so a result
+ * is the same alone and in a mix. An insert is admitted only while the
+ * graph's projected device footprint stays within hbm_budget_bytes
+ * (0 = no limit); otherwise it fails with a message naming the budget. */
+typedef struct coresg_hip_session coresg_hip_session;
+
+enum { CORESG_HIP_REQ_INSERT = 1, CORESG_HIP_REQ_EXTRACT = 2 };
+enum {
+    CORESG_HIP_REQ_QUEUED = 0,
+    CORESG_HIP_REQ_RUNNING = 1,
+    CORESG_HIP_REQ_DONE = 2,
+    CORESG_HIP_REQ_FAILED = 3
+};
+
+typedef struct coresg_hip_request_status {
+    uint64_t id;
+    int kind;
+    int state;
+    uint32_t graph_n;       /* points in the graph when the request completed */
+    uint64_t graph_version; /* inserts applied before it completed */
+    uint32_t num_msts;      /* extract: mpts values requested */
+    uint32_t batch_requests; /* requests served by the same device pass */
+    double queued_ms;       /* submit -> start */
+    double run_ms;          /* start -> done */
+    double latency_ms;      /* submit -> done */
+    coresg_hip_build_stats insert_stats;
+    char error[256];
+} coresg_hip_request_status;
+
+/* Build a CORE-SG from host rows (CoreSG::build) and start serving it. */
+coresg_hip_session* coresg_hip_session_create(int device, const float* rows, uint32_t n,
+                                              uint32_t dim, uint32_t k_max,
+                                              const coresg_hip_build_params* p,
+                                              size_t hbm_budget_bytes,
+                                              coresg_hip_build_stats* st);
+/* Serve an existing built graph; the session takes ownership of it. */
+coresg_hip_session* coresg_hip_session_adopt(coresg_hip_graph* g, const coresg_hip_build_params* p,
+                                             size_t hbm_budget_bytes);
+/* Queue a request (rows / mpts are copied); returns its id, 0 on failure. */
+uint64_t coresg_hip_session_submit_insert(coresg_hip_session* s, const float* rows,
+                                          uint32_t count);
+uint64_t coresg_hip_session_submit_extract(coresg_hip_session* s, const uint32_t* mpts,
+                                           uint32_t count, int want_edges, int want_labels);
+/* Wait up to timeout_s (< 0: no limit): 1 finished (done or failed), 0 not
+ * yet, -1 error (unknown id). `st` receives the request's status. */
+int coresg_hip_session_wait(coresg_hip_session* s, uint64_t id, double timeout_s,
+                            coresg_hip_request_status* st);
+/* A completed extract's results, laid out as coresg_hip_extract's. */
+int coresg_hip_session_extract_result(coresg_hip_session* s, uint64_t id,
+                                      coresg_hip_mst_info* info, uint32_t* out_u,
+                                      uint32_t* out_v, float* out_w, uint32_t* out_labels);
+/* Forget a finished request. */
+int coresg_hip_session_release(coresg_hip_session* s, uint64_t id);
+/* The served graph, once the queue has drained (for inspection only; submit
+ * nothing while it is in use). */
+coresg_hip_graph* coresg_hip_session_graph(coresg_hip_session* s);
+/* Finish the queued requests, stop the worker, free the graph. */
+void coresg_hip_session_destroy(coresg_hip_session* s);
+
 /* Sampled k-NN recall on device (eval.cpp:24-97; queries chosen on the
  * host by the same mt19937_64 draw). */
 int coresg_hip_sampled_recall(coresg_hip_graph* g, size_t queries, uint64_t seed,

@@ -61,6 +61,23 @@ class MstInfo(C.Structure):
                 ("num_components", C.c_uint32), ("rounds", C.c_uint32), ("weight", C.c_double)]
 
 
+class RequestStatus(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("kind", C.c_int), ("state", C.c_int),
+                ("graph_n", C.c_uint32), ("graph_version", C.c_uint64), ("num_msts", C.c_uint32),
+                ("batch_requests", C.c_uint32), ("queued_ms", C.c_double),
+                ("run_ms", C.c_double), ("latency_ms", C.c_double),
+                ("insert_stats", BuildStats), ("error", C.c_char * 256)]
+
+    def asdict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f not in ("insert_stats", "error")}
+        d["insert_stats"] = self.insert_stats.asdict()
+        d["error"] = self.error.decode(errors="replace")
+        return d
+
+
+REQ_INSERT, REQ_EXTRACT = 1, 2
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+
 _vp = C.c_void_p
 _lib = None
 
@@ -112,6 +129,18 @@ SIGNATURES = [
     ("coresg_hip_insert", C.c_int, [_vp, C.c_void_p, C.c_uint32, C.c_int, C.c_int,
                                     C.POINTER(BuildParams), C.POINTER(BuildStats)]),
     ("coresg_hip_graph_size", C.c_uint32, [_vp]),
+    ("coresg_hip_session_create", _vp, [C.c_int, _vp, C.c_uint32, C.c_uint32, C.c_uint32,
+                                         C.POINTER(BuildParams), C.c_size_t,
+                                         C.POINTER(BuildStats)]),
+    ("coresg_hip_session_adopt", _vp, [_vp, C.POINTER(BuildParams), C.c_size_t]),
+    ("coresg_hip_session_submit_insert", C.c_uint64, [_vp, _vp, C.c_uint32]),
+    ("coresg_hip_session_submit_extract", C.c_uint64, [_vp, _vp, C.c_uint32, C.c_int, C.c_int]),
+    ("coresg_hip_session_wait", C.c_int, [_vp, C.c_uint64, C.c_double,
+                                          C.POINTER(RequestStatus)]),
+    ("coresg_hip_session_extract_result", C.c_int, [_vp, C.c_uint64, _vp, _vp, _vp, _vp, _vp]),
+    ("coresg_hip_session_release", C.c_int, [_vp, C.c_uint64]),
+    ("coresg_hip_session_graph", _vp, [_vp]),
+    ("coresg_hip_session_destroy", None, [_vp]),
 ]
 
 
@@ -381,3 +410,96 @@ def nndescent(vectors_padded: np.ndarray, dim: int, k: int, sample: int = 16, it
     check(lib().coresg_hip_nndescent(device, C.byref(cfg), ptr(vec), ptr(ids), ptr(dist),
                                      ptr(flags), ptr(count), C.byref(upd)))
     return (ids, dist, flags, count), upd.value
+
+
+class Session:
+    """A serving session over one device-resident CORE-SG (insert and
+    extract requests, executed in submission order by the library's worker
+    thread; waiting extracts share one fused sweep)."""
+
+    def __init__(self, rows: np.ndarray, k_max: int, params: BuildParams | None = None,
+                 hbm_budget: int = 0, device: int = 0, graph: "Graph | None" = None):
+        p = params or build_params()
+        self.params = p
+        if graph is not None:
+            h = lib().coresg_hip_session_adopt(graph.h, C.byref(p), hbm_budget)
+            if not h:
+                raise CoresgHipError(last_error())
+            graph.h = None  # owned by the session now
+            self.dim, self.k = graph.dim, graph.k
+            self.build_stats = None
+        else:
+            rows = _f32(rows)
+            self.dim, self.k = rows.shape[1], int(k_max)
+            st = BuildStats()
+            h = lib().coresg_hip_session_create(device, ptr(rows), rows.shape[0], rows.shape[1],
+                                                self.k, C.byref(p), hbm_budget, C.byref(st))
+            if not h:
+                raise CoresgHipError(last_error())
+            self.build_stats = st.asdict()
+        self.h = C.c_void_p(h)
+
+    def submit_insert(self, rows: np.ndarray) -> int:
+        rows = _f32(rows)
+        if rows.ndim != 2 or rows.shape[1] != self.dim:
+            raise ValueError(f"rows must be (m, {self.dim})")
+        rid = lib().coresg_hip_session_submit_insert(self.h, ptr(rows), rows.shape[0])
+        if not rid:
+            raise CoresgHipError(last_error())
+        return int(rid)
+
+    def submit_extract(self, mpts, edges: bool = False, labels: bool = False) -> int:
+        mp = _u32(list(mpts))
+        rid = lib().coresg_hip_session_submit_extract(self.h, ptr(mp), mp.size, int(edges),
+                                                      int(labels))
+        if not rid:
+            raise CoresgHipError(last_error())
+        return int(rid)
+
+    def wait(self, rid: int, timeout: float = -1.0) -> dict | None:
+        st = RequestStatus()
+        r = lib().coresg_hip_session_wait(self.h, rid, timeout, C.byref(st))
+        if r < 0:
+            raise CoresgHipError(last_error())
+        return st.asdict() if r == 1 else None
+
+    def result(self, rid: int, edges: bool = False, labels: bool = False) -> list:
+        st = self.wait(rid)
+        if st["state"] != REQ_DONE:
+            raise CoresgHipError(st["error"])
+        q, n = st["num_msts"], st["graph_n"]
+        info = (MstInfo * q)()
+        es = max(n - 1, 0)
+        ou = np.empty(q * es, np.uint32) if edges else None
+        ov = np.empty(q * es, np.uint32) if edges else None
+        ow = np.empty(q * es, np.float32) if edges else None
+        ol = np.empty(q * n, np.uint32) if labels else None
+        check(lib().coresg_hip_session_extract_result(self.h, rid, info, ptr(ou), ptr(ov), ptr(ow),
+                                                      ptr(ol)))
+        out = []
+        for i in range(q):
+            r = dict(mpts=info[i].mpts, num_edges=info[i].num_edges,
+                     num_components=info[i].num_components, rounds=info[i].rounds,
+                     weight=info[i].weight)
+            if edges:
+                t = info[i].num_edges
+                r["u"], r["v"], r["w"] = (ou[i * es:i * es + t], ov[i * es:i * es + t],
+                                          ow[i * es:i * es + t])
+            if labels:
+                r["labels"] = ol[i * n:(i + 1) * n]
+            out.append(r)
+        return out
+
+    def release(self, rid: int) -> None:
+        check(lib().coresg_hip_session_release(self.h, rid))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().coresg_hip_session_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

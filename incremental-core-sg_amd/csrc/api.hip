@@ -7,6 +7,7 @@
 #include "../../include/coresg_hip.h"
 #include "build.hpp"
 #include "graph.hpp"
+#include "handle.hpp"
 #include "scan.hpp"
 
 namespace coresg_hip {
@@ -75,29 +76,7 @@ void Graph::resolve_timers() {
 
 using namespace coresg_hip;
 
-struct coresg_hip_graph {
-    Graph g;
-};
-
 namespace {
-
-// Runs `f`, converting exceptions into the 0-return + last_error convention.
-template <typename F>
-int guarded(F&& f) {
-    try {
-        f();
-        return 1;
-    } catch (const HipError& e) {
-        std::string msg = e.what();
-        if (e.code == hipErrorOutOfMemory) msg += " (out of memory)";
-        set_last_error(msg);
-    } catch (const std::bad_alloc&) {
-        set_last_error("host allocation failed (out of memory)");
-    } catch (const std::exception& e) {
-        set_last_error(e.what());
-    }
-    return 0;
-}
 
 void bind(const Graph& g) { CSG_CHECK(hipSetDevice(g.device)); }
 

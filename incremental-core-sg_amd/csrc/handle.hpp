@@ -1,0 +1,37 @@
+// The opaque ABI handle of a resident graph and the ABI's error convention,
+// shared by the translation units that implement extern "C" entry points.
+#pragma once
+
+#include <new>
+#include <string>
+
+#include "graph.hpp"
+
+struct coresg_hip_graph {
+    coresg_hip::Graph g;
+};
+
+namespace coresg_hip {
+
+void set_last_error(const std::string& msg);
+const char* get_last_error();
+
+// Runs `f`, converting exceptions into the 0-return + last_error convention.
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 1;
+    } catch (const HipError& e) {
+        std::string msg = e.what();
+        if (e.code == hipErrorOutOfMemory) msg += " (out of memory)";
+        set_last_error(msg);
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed (out of memory)");
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+    }
+    return 0;
+}
+
+}  // namespace coresg_hip
