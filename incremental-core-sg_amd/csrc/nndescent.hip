@@ -330,25 +330,34 @@ __device__ __forceinline__ float l2sqr_reg(const float (&a)[STRIDE],
     return (l0 + l1) + (l2 + l3);
 }
 
-// One 128-thread group per node. LDS: ids[S], thr[S], rows[S*stride] and the
-// distance tile D[A][S+1] (odd row pitch: the A rows are written column-wise
-// and read row-wise without bank conflicts). Thread t owns A-row
-// rb + (t & 31) in registers and computes its distances to every 4th member
-// (broadcast LDS reads). Then each member appends the candidates that beat
-// its threshold to its own proposal buffer.
-// STRIDE = 0 selects the runtime-stride path (rows read from LDS).
+// One 128-thread group (two wavefronts) per node. LDS: ids[S], thr[S],
+// rows[S*stride] and the distance tile D[A][S+1] (odd pitch: rows and
+// columns are both read without bank conflicts).
+//
+// Distances: the (new i, member j) pairs are spread densely over the threads,
+// each unordered new x new pair once (mirrored into the tile), both rows read
+// from LDS in the reference's exact summation order.
+//
+// Proposals (the reference's try_pair test, nndescent.cpp:41-44): new member
+// x gets every member y with D(x,y) < thr_x (row targets), old member x gets
+// every new member y with D(y,x) < thr_x (column targets). Wavefront w takes
+// the rows x = w (mod 2) -- one ballot per row chunk -- and one chunk of 64
+// columns, a lane per column. All counts are taken first, one batch of
+// atomics reserves the targets' buffer ranges, then the tile is read again to
+// write the keys (distance bits << 32 | id). Proposals at or beyond a target's
+// capacity in this round are dropped.
 template <std::uint32_t STRIDE>
 __global__ void __launch_bounds__(kJoinThreads) join_kernel(JoinArgs a) {
     extern __shared__ float lds[];
     const std::uint32_t smp = a.smp;
-    const std::uint32_t S = 6u * smp, A = 2u * smp, P = S + 1;
+    const std::uint32_t S = 6u * smp, P = S + 1;
     const std::uint32_t stride = STRIDE ? STRIDE : a.stride;
     std::uint32_t* s_id = reinterpret_cast<std::uint32_t*>(lds);
     float* s_thr = lds + S;
     float* s_rows = lds + 2 * S;
     float* s_D = s_rows + std::size_t(S) * stride;
-    (void)A;
-    const std::uint32_t tid = threadIdx.x;
+    const std::uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    const std::uint64_t below = (1ull << lane) - 1ull;
     for (std::uint32_t v = blockIdx.x; v < a.n; v += gridDim.x) {
         const std::uint32_t cnt = a.cnt[v];
         const std::uint32_t na = cnt & 0xFFFFu, ns = cnt >> 16;
@@ -365,54 +374,84 @@ __global__ void __launch_bounds__(kJoinThreads) join_kernel(JoinArgs a) {
                 reinterpret_cast<const float4*>(a.pts + std::size_t(s_id[i]) * stride)[f];
         }
         __syncthreads();
-        const std::uint32_t cg = tid >> 5;
-        for (std::uint32_t rb = 0; rb < na; rb += 32) {
-            const std::uint32_t r = rb + (tid & 31u);
-            if (r < na) {
-                if constexpr (STRIDE != 0) {
-                    float row[STRIDE];
-#pragma unroll
-                    for (std::uint32_t f = 0; f < STRIDE / 4; ++f) {
-                        const float4 t = reinterpret_cast<const float4*>(s_rows + r * STRIDE)[f];
-                        row[4 * f] = t.x;
-                        row[4 * f + 1] = t.y;
-                        row[4 * f + 2] = t.z;
-                        row[4 * f + 3] = t.w;
-                    }
-                    for (std::uint32_t c = cg; c < ns; c += 4)
-                        s_D[r * P + c] = (c == r) ? __builtin_inff()
-                                                  : l2sqr_reg<STRIDE>(row, s_rows + c * STRIDE);
-                } else {
-                    for (std::uint32_t c = cg; c < ns; c += 4)
-                        s_D[r * P + c] =
-                            (c == r) ? __builtin_inff()
-                                     : l2sqr_exact(s_rows + std::size_t(r) * stride,
-                                                   s_rows + std::size_t(c) * stride, stride);
-                }
+        // pair p -> (i, j) = (p / ns, p % ns); the rounded-up reciprocal
+        // gives the exact quotient for p < 2^14 and ns <= 192
+        const std::uint32_t npair = na * ns;
+        const std::uint64_t recip = ((1ull << 24) + ns - 1) / ns;
+        for (std::uint32_t p = tid; p < npair; p += kJoinThreads) {
+            const std::uint32_t i = static_cast<std::uint32_t>((std::uint64_t(p) * recip) >> 24);
+            const std::uint32_t j = p - i * ns;
+            if (j < na && j <= i) continue;
+            const float d = l2sqr_exact(s_rows + std::size_t(i) * stride,
+                                        s_rows + std::size_t(j) * stride, stride);
+            s_D[i * P + j] = d;
+            if (j < na) s_D[j * P + i] = d;
+        }
+        for (std::uint32_t i = tid; i < na; i += kJoinThreads) s_D[i * P + i] = __builtin_inff();
+        __syncthreads();
+        // counts: rows x = w (mod 2), lane x/2 keeps row x's; columns
+        // na + 64w + lane
+        std::uint32_t rcount = 0;
+        for (std::uint32_t x = w; x < na; x += 2) {
+            const float thr = s_thr[x];
+            std::uint32_t c = 0;
+            for (std::uint32_t y0 = 0; y0 < ns; y0 += 64) {
+                const std::uint32_t y = y0 + lane;
+                c += __popcll(__ballot(y < ns && s_D[x * P + y] < thr));
+            }
+            if (lane == (x >> 1)) rcount = c;
+        }
+        const std::uint32_t xc = na + 64u * w + lane;
+        const bool has_col = xc < ns;
+        const float thr_c = has_col ? s_thr[xc] : -1.0f;
+        std::uint32_t ccount = 0;
+        if (has_col)
+            for (std::uint32_t y = 0; y < na; ++y) ccount += s_D[y * P + xc] < thr_c ? 1u : 0u;
+        // reserve
+        const std::uint32_t xr = 2u * lane + w;
+        std::uint32_t rbase = a.cap, cbase = a.cap;
+        if (rcount) rbase = atomicAdd(&a.fill[s_id[xr]], rcount);
+        if (ccount) cbase = atomicAdd(&a.fill[s_id[xc]], ccount);
+        // write row targets
+        for (std::uint32_t x = w; x < na; x += 2) {
+            const float thr = s_thr[x];
+            std::uint32_t pos0 = __shfl(rbase, static_cast<int>(x >> 1), 64);
+            unsigned long long* tgt = a.cand + std::size_t(s_id[x]) * a.cap;
+            for (std::uint32_t y0 = 0; y0 < ns; y0 += 64) {
+                const std::uint32_t y = y0 + lane;
+                const float d = y < ns ? s_D[x * P + y] : __builtin_inff();
+                const bool pass = d < thr;
+                const std::uint64_t m = __ballot(pass);
+                const std::uint32_t pos = pos0 + __popcll(m & below);
+                if (pass && pos < a.cap)
+                    tgt[pos] = (static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32) | s_id[y];
+                pos0 += __popcll(m);
             }
         }
-        __syncthreads();
-        // Every member proposes each candidate that beats its current k-th
-        // distance (the reference's try_pair test, nndescent.cpp:41-44); the
-        // target's merge keeps the best. Proposals beyond the target's buffer
-        // capacity in this round are dropped.
-        for (std::uint32_t x = tid; x < ns; x += kJoinThreads) {
-            const float thr = s_thr[x];
-            const std::uint32_t lim = x < na ? ns : na;
-            std::uint32_t npass = 0;
-            for (std::uint32_t y = 0; y < lim; ++y) {
-                const float r = x < na ? s_D[x * P + y] : s_D[y * P + x];
-                npass += (r < thr) ? 1u : 0u;
+        // write column targets
+        if (ccount) {
+            unsigned long long* tgt = a.cand + std::size_t(s_id[xc]) * a.cap;
+            std::uint32_t pos = cbase;
+            for (std::uint32_t y = 0; y < na && pos < a.cap; ++y) {
+                const float d = s_D[y * P + xc];
+                if (!(d < thr_c)) continue;
+                tgt[pos++] = (static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32) | s_id[y];
             }
-            if (npass == 0) continue;
-            const std::uint32_t target = s_id[x];
-            if (a.fill[target] >= a.cap) continue;  // buffer already full this round
-            std::uint32_t pos = atomicAdd(&a.fill[target], npass);
-            unsigned long long* base = a.cand + std::size_t(target) * a.cap;
-            for (std::uint32_t y = 0; y < lim && pos < a.cap; ++y) {
-                const float r = x < na ? s_D[x * P + y] : s_D[y * P + x];
-                if (!(r < thr)) continue;
-                base[pos++] = (static_cast<unsigned long long>(fbits(sqrt_exact(r))) << 32) | s_id[y];
+        }
+        // columns beyond the first 128 (sample > 21 only)
+        for (std::uint32_t xb = na + 128u + 64u * w; xb < ns; xb += 128u) {
+            const std::uint32_t x = xb + lane;
+            if (x >= ns) continue;
+            const float thr = s_thr[x];
+            std::uint32_t c = 0;
+            for (std::uint32_t y = 0; y < na; ++y) c += s_D[y * P + x] < thr ? 1u : 0u;
+            if (c == 0) continue;
+            std::uint32_t pos = atomicAdd(&a.fill[s_id[x]], c);
+            unsigned long long* tgt = a.cand + std::size_t(s_id[x]) * a.cap;
+            for (std::uint32_t y = 0; y < na && pos < a.cap; ++y) {
+                const float d = s_D[y * P + x];
+                if (!(d < thr)) continue;
+                tgt[pos++] = (static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32) | s_id[y];
             }
         }
         __syncthreads();
@@ -426,14 +465,32 @@ __global__ void __launch_bounds__(kJoinThreads) join_kernel(JoinArgs a) {
 constexpr std::uint32_t kMaxCap = 512;
 constexpr std::uint32_t kDefaultCap = 512;
 
+// Ascending bitonic sort of one 64-bit key per lane across a wavefront.
+__device__ __forceinline__ unsigned long long wave_sort_u64(unsigned long long key) {
+    const std::uint32_t lane = __lane_id();
+#pragma unroll
+    for (std::uint32_t size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+        for (std::uint32_t half = size >> 1; half > 0; half >>= 1) {
+            const unsigned long long other = __shfl_xor(key, half, 64);
+            const bool up = (lane & size) == 0;
+            const bool lo = (lane & half) == 0;
+            const bool keep_min = lo == up;
+            key = keep_min ? (other < key ? other : key) : (other > key ? other : key);
+        }
+    }
+    return key;
+}
+
 // One 64-thread group per node (nodes [0, n), candidate ids below id_limit).
-// The round's proposals
-// cand[v*cap .. v*cap + min(fill, cap)) are sorted by (distance, id) with a
-// bitonic network in LDS; a proposal survives if it is the first copy of its
-// key, not the node itself, not already listed and (for a full list) closer
-// than the current k-th -- try_insert's tests (knn_graph.cpp:37-55). The best
-// k survivors are merged with the list by rank counting, existing entries
-// first among equal distances (the reference's sorted insertion).
+// The round's proposals cand[v*cap .. v*cap + min(fill, cap)) are first
+// filtered by try_insert's per-candidate tests (knn_graph.cpp:37-55): not the
+// node itself, not already listed, and (for a full list) closer than the
+// current k-th. The survivors are sorted by (distance, id) -- in registers
+// when at most 64 remain, else by a bitonic network in LDS -- and the first
+// copy of each key is kept. The best k are merged with the list by rank
+// counting, existing entries first among equal distances (the reference's
+// sorted insertion).
 __global__ void __launch_bounds__(64)
     merge_kernel(std::uint32_t n, std::uint32_t id_limit, std::uint32_t k,
                  std::uint32_t* __restrict__ ids,
@@ -442,9 +499,9 @@ __global__ void __launch_bounds__(64)
                  std::uint32_t* __restrict__ fill, std::uint32_t cap,
                  unsigned long long* __restrict__ partial) {
     __shared__ unsigned long long s_key[kMaxCap];
-    __shared__ std::uint32_t s_ex[kMaxK];
     __shared__ unsigned long long s_top[kMaxK];
     const std::uint32_t lane = threadIdx.x;
+    const std::uint64_t lt = (1ull << lane) - 1ull;
     unsigned long long local_upd = 0;
     for (std::uint32_t v = blockIdx.x; v < n; v += gridDim.x) {
         const std::size_t base = std::size_t(v) * k;
@@ -457,46 +514,62 @@ __global__ void __launch_bounds__(64)
         m = m < cap ? m : cap;
         std::uint32_t t = 0;
         if (m > 0) {
-            std::uint32_t P = 64;
-            while (P < m) P <<= 1;
-            if (lane < c) s_ex[lane] = my_id;
-            for (std::uint32_t i = lane; i < P; i += 64)
-                s_key[i] = i < m ? cand[std::size_t(v) * cap + i] : ~0ull;
-            __syncthreads();
-            for (std::uint32_t size = 2; size <= P; size <<= 1) {
-                for (std::uint32_t half = size >> 1; half > 0; half >>= 1) {
-                    for (std::uint32_t q = lane; q < P / 2; q += 64) {
-                        const std::uint32_t lo = 2 * half * (q / half) + (q % half);
-                        const std::uint32_t hi = lo + half;
-                        const bool up = (lo & size) == 0;
-                        const unsigned long long x = s_key[lo], y = s_key[hi];
-                        if ((x > y) == up) {
-                            s_key[lo] = y;
-                            s_key[hi] = x;
-                        }
-                    }
-                    __syncthreads();
-                }
-            }
-            // survivors in sorted order; the first k of them are the ones
-            // that can enter the list
-            std::uint32_t nvalid = 0;
+            // 1. per-candidate tests, survivors compacted into s_key
+            std::uint32_t nv = 0;
             for (std::uint32_t i0 = 0; i0 < m; i0 += 64) {
                 const std::uint32_t i = i0 + lane;
-                bool ok = false;
-                unsigned long long key = ~0ull;
-                if (i < m) {
-                    key = s_key[i];
-                    const std::uint32_t cid = static_cast<std::uint32_t>(key);
-                    const float cd = __uint_as_float(static_cast<std::uint32_t>(key >> 32));
-                    ok = key != ~0ull && (i == 0 || s_key[i - 1] != key) && cid != v &&
-                         cid < id_limit && cd < last;
-                    for (std::uint32_t j = 0; j < c && ok; ++j) ok = s_ex[j] != cid;
-                }
+                const unsigned long long key = i < m ? cand[std::size_t(v) * cap + i] : ~0ull;
+                const std::uint32_t cid = static_cast<std::uint32_t>(key);
+                const float cd = __uint_as_float(static_cast<std::uint32_t>(key >> 32));
+                bool ok = key != ~0ull && cid != v && cid < id_limit && cd < last;
+                for (std::uint32_t j = 0; j < c; ++j)
+                    ok = ok && static_cast<std::uint32_t>(__builtin_amdgcn_readlane(
+                                   static_cast<int>(my_id), static_cast<int>(j))) != cid;
                 const std::uint64_t vm = __ballot(ok);
-                const std::uint32_t r = nvalid + __popcll(vm & ((1ull << lane) - 1ull));
+                if (ok) s_key[nv + __popcll(vm & lt)] = key;
+                nv += __popcll(vm);
+            }
+            __syncthreads();
+            // 2. sort, keep first copies, rank
+            std::uint32_t nvalid = 0;
+            if (nv > 0 && nv <= 64) {
+                unsigned long long key = lane < nv ? s_key[lane] : ~0ull;
+                key = wave_sort_u64(key);
+                const unsigned long long prev = __shfl_up(key, 1, 64);
+                const bool ok = key != ~0ull && (lane == 0 || prev != key);
+                const std::uint64_t vm = __ballot(ok);
+                const std::uint32_t r = __popcll(vm & lt);
                 if (ok && r < k) s_top[r] = key;
-                nvalid += __popcll(vm);
+                nvalid = __popcll(vm);
+            } else if (nv > 64) {
+                std::uint32_t P = 128;
+                while (P < nv) P <<= 1;
+                for (std::uint32_t i = nv + lane; i < P; i += 64) s_key[i] = ~0ull;
+                __syncthreads();
+                for (std::uint32_t size = 2; size <= P; size <<= 1) {
+                    for (std::uint32_t half = size >> 1; half > 0; half >>= 1) {
+                        for (std::uint32_t q = lane; q < P / 2; q += 64) {
+                            const std::uint32_t lo = 2 * half * (q / half) + (q % half);
+                            const std::uint32_t hi = lo + half;
+                            const bool up = (lo & size) == 0;
+                            const unsigned long long x = s_key[lo], y = s_key[hi];
+                            if ((x > y) == up) {
+                                s_key[lo] = y;
+                                s_key[hi] = x;
+                            }
+                        }
+                        __syncthreads();
+                    }
+                }
+                for (std::uint32_t i0 = 0; i0 < nv; i0 += 64) {
+                    const std::uint32_t i = i0 + lane;
+                    const unsigned long long key = i < nv ? s_key[i] : ~0ull;
+                    const bool ok = i < nv && (i == 0 || s_key[i - 1] != key);
+                    const std::uint64_t vm = __ballot(ok);
+                    const std::uint32_t r = nvalid + __popcll(vm & lt);
+                    if (ok && r < k) s_top[r] = key;
+                    nvalid += __popcll(vm);
+                }
             }
             __syncthreads();
             t = nvalid < k ? nvalid : k;
@@ -684,16 +757,17 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     b.gids.reserve(std::size_t(n) * S);
     b.gthr.reserve(std::size_t(n) * S);
     b.gcnt.reserve(n);
+    const JoinFn join_fn = pick_join(stride);
     const std::size_t lds =
         (2 * std::size_t(S) + std::size_t(S) * stride + std::size_t(A) * (S + 1)) * sizeof(float);
+    const unsigned join_threads = kJoinThreads;
+    const unsigned join_grid = static_cast<unsigned>(std::min<std::size_t>(n, 65536));
     if (lds > 150 * 1024) throw std::runtime_error("nndescent: candidate tile exceeds LDS");
-    const JoinFn join_fn = pick_join(stride);
     if (lds > 64 * 1024)
         CSG_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(join_fn),
                                       hipFuncAttributeMaxDynamicSharedMemorySize,
                                       static_cast<int>(lds)));
     JoinArgs ja{pts, n, stride, smp, b.gids.p, b.gthr.p, b.gcnt.p, b.cand.p, b.fill.p, cap};
-    const unsigned join_blocks = static_cast<unsigned>(std::min<std::size_t>(n, 65536));
     const unsigned merge_blocks = static_cast<unsigned>(std::min<std::size_t>(n, kMergeBlocks));
     const bool tallies = gp && gp->profiling;
     const double stop = delta * double(k) * double(n);
@@ -720,7 +794,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         {
             std::unique_ptr<ScopedKernelTimer> t;
             if (gp) t.reset(new ScopedKernelTimer(*gp, "nnd_join"));
-            hipLaunchKernelGGL(join_fn, dim3(join_blocks), dim3(kJoinThreads), lds, s, ja);
+            hipLaunchKernelGGL(join_fn, dim3(join_grid), dim3(join_threads), lds, s, ja);
             CSG_LAUNCH_CHECK();
         }
         if (tallies) {
