@@ -60,7 +60,9 @@ __global__ void __launch_bounds__(256)
                        std::uint32_t k, std::uint64_t seed, std::uint32_t* __restrict__ ids,
                        float* __restrict__ dist, std::uint8_t* __restrict__ flags,
                        std::uint32_t* __restrict__ count) {
-    const std::uint32_t lane = threadIdx.x & 63u;
+    __shared__ std::uint32_t s_stage_id[4][64];
+    __shared__ float s_stage_d[4][64];
+    const std::uint32_t lane = threadIdx.x & 63u, wib = threadIdx.x >> 6;
     const std::uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const std::uint32_t waves = (gridDim.x * blockDim.x) >> 6;
     for (std::uint32_t node = wave; node < n; node += waves) {
@@ -102,8 +104,10 @@ __global__ void __launch_bounds__(256)
                 ok = ok && (oid != cand);
             }
             // first occurrence in this chunk
+#pragma unroll
             for (std::uint32_t j = 0; j < 64; ++j) {
-                const std::uint32_t oc = __shfl(cand, j, 64);
+                const std::uint32_t oc = static_cast<std::uint32_t>(
+                    __builtin_amdgcn_readlane(static_cast<int>(cand), static_cast<int>(j)));
                 ok = ok && !(j < lane && oc == cand);
             }
             const std::uint64_t mask = __ballot(ok);
@@ -113,21 +117,24 @@ __global__ void __launch_bounds__(256)
             if (take)
                 d = sqrt_exact(l2sqr_exact(pts + std::size_t(node) * stride,
                                            pts + std::size_t(cand) * stride, stride));
-            // move taken candidates into lanes [accepted, accepted + taken)
+            // move taken candidates into lanes [accepted, accepted + taken):
+            // each writes its slot (its rank) in the wave's staging area
             const std::uint64_t tmask = __ballot(take);
             const std::uint32_t taken = __popcll(tmask);
-            for (std::uint32_t r = 0; r < taken; ++r) {
-                // lane holding the r-th taken candidate
-                std::uint64_t mm = tmask;
-                for (std::uint32_t q = 0; q < r; ++q) mm &= mm - 1ull;
-                const int src = __ffsll(static_cast<unsigned long long>(mm)) - 1;
-                const std::uint32_t cid = __shfl(cand, src, 64);
-                const float cd = __shfl(d, src, 64);
-                if (lane == accepted + r) {
-                    acc_id = cid;
-                    acc_d = cd;
-                }
+            if (take) {
+                s_stage_id[wib][rank] = cand;
+                s_stage_d[wib][rank] = d;
             }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane >= accepted && lane < accepted + taken) {
+                acc_id = s_stage_id[wib][lane];
+                acc_d = s_stage_d[wib][lane];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             accepted += taken;
         }
         // stable merge: existing entries (sorted) first among equals, then the
@@ -465,32 +472,108 @@ __global__ void __launch_bounds__(kJoinThreads) join_kernel(JoinArgs a) {
 constexpr std::uint32_t kMaxCap = 512;
 constexpr std::uint32_t kDefaultCap = 512;
 
-// Ascending bitonic sort of one 64-bit key per lane across a wavefront.
-__device__ __forceinline__ unsigned long long wave_sort_u64(unsigned long long key) {
+// Ascending bitonic sort of 64*R keys held R per lane (element e = lane*R + r):
+// partners closer than R are swapped inside a lane, farther ones across lanes
+// by shuffles. Fully unrolled: no LDS, no barriers, no loop overhead.
+template <int R>
+__device__ __forceinline__ void wave_sort_regs(unsigned long long (&key)[R]) {
     const std::uint32_t lane = __lane_id();
 #pragma unroll
-    for (std::uint32_t size = 2; size <= 64; size <<= 1) {
+    for (std::uint32_t size = 2; size <= 64u * R; size <<= 1) {
 #pragma unroll
         for (std::uint32_t half = size >> 1; half > 0; half >>= 1) {
-            const unsigned long long other = __shfl_xor(key, half, 64);
-            const bool up = (lane & size) == 0;
-            const bool lo = (lane & half) == 0;
-            const bool keep_min = lo == up;
-            key = keep_min ? (other < key ? other : key) : (other > key ? other : key);
+            if (half < static_cast<std::uint32_t>(R)) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int q = r ^ static_cast<int>(half);
+                    if (q > r) {
+                        const bool up = ((lane * R + r) & size) == 0;
+                        const unsigned long long x = key[r], y = key[q];
+                        const bool sw = (x > y) == up;
+                        key[r] = sw ? y : x;
+                        key[q] = sw ? x : y;
+                    }
+                }
+            } else {
+                const int lh = static_cast<int>(half / R);
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const unsigned long long other = __shfl_xor(key[r], lh, 64);
+                    const std::uint32_t e = lane * R + r;
+                    const bool up = (e & size) == 0;
+                    const bool lo = (e & half) == 0;
+                    const unsigned long long mn = other < key[r] ? other : key[r];
+                    const unsigned long long mx = other < key[r] ? key[r] : other;
+                    key[r] = (lo == up) ? mn : mx;
+                }
+            }
         }
     }
-    return key;
 }
 
-// One 64-thread group per node (nodes [0, n), candidate ids below id_limit).
-// The round's proposals cand[v*cap .. v*cap + min(fill, cap)) are first
-// filtered by try_insert's per-candidate tests (knn_graph.cpp:37-55): not the
-// node itself, not already listed, and (for a full list) closer than the
-// current k-th. The survivors are sorted by (distance, id) -- in registers
-// when at most 64 remain, else by a bitonic network in LDS -- and the first
-// copy of each key is kept. The best k are merged with the list by rank
-// counting, existing entries first among equal distances (the reference's
-// sorted insertion).
+// Sort the nv compacted survivors s_key[0, nv) (nv <= 64R), keep the first copy
+// of every key and store the first k in s_top; returns the number of distinct
+// survivors.
+template <int R>
+__device__ __forceinline__ std::uint32_t sort_unique_top(const unsigned long long* s_key,
+                                                         std::uint32_t nv, std::uint32_t k,
+                                                         unsigned long long* s_top) {
+    const std::uint32_t lane = __lane_id();
+    unsigned long long key[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const std::uint32_t e = lane * R + r;
+        key[r] = e < nv ? s_key[e] : ~0ull;
+    }
+    wave_sort_regs<R>(key);
+    const unsigned long long prev_lane = __shfl_up(key[R - 1], 1, 64);
+    bool ok[R];
+    std::uint32_t mine = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const unsigned long long prev = r > 0 ? key[r - 1] : prev_lane;
+        const bool first = (r == 0 && lane == 0) || prev != key[r];
+        ok[r] = key[r] != ~0ull && first;
+        mine += ok[r] ? 1u : 0u;
+    }
+    const std::uint32_t incl = wave_inclusive_scan(mine);
+    std::uint32_t rank = incl - mine;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (ok[r]) {
+            if (rank < k) s_top[rank] = key[r];
+            ++rank;
+        }
+    }
+    return static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
+}
+
+// Number of entries of the ascending a[0, len) (len <= 64) below x (STRICT) or
+// at most x.
+template <bool STRICT>
+__device__ __forceinline__ std::uint32_t count_before(const float* a, std::uint32_t len, float x) {
+    std::uint32_t pos = 0;
+#pragma unroll
+    for (std::uint32_t step = 64; step > 0; step >>= 1) {
+        if (pos + step <= len) {
+            const float y = a[pos + step - 1];
+            if (STRICT ? (y < x) : (y <= x)) pos += step;
+        }
+    }
+    return pos;
+}
+
+// One 64-thread group per node (nodes [0, n), candidate ids below id_limit),
+// lists of at most KC entries. The round's proposals
+// cand[v*cap .. v*cap + min(fill, cap)) are first filtered by try_insert's
+// per-candidate tests (knn_graph.cpp:37-55): not the node itself, not already
+// listed, and (for a full list) closer than the current k-th. The survivors
+// are sorted by (distance, id) in registers and the first copy of each key is
+// kept. The best k are merged with the list: an entry's new position is its
+// rank plus the number of entries of the other sorted list before it --
+// existing entries first among equal distances (the reference's sorted
+// insertion) -- found by binary search.
+template <std::uint32_t KC>
 __global__ void __launch_bounds__(64)
     merge_kernel(std::uint32_t n, std::uint32_t id_limit, std::uint32_t k,
                  std::uint32_t* __restrict__ ids,
@@ -500,6 +583,7 @@ __global__ void __launch_bounds__(64)
                  unsigned long long* __restrict__ partial) {
     __shared__ unsigned long long s_key[kMaxCap];
     __shared__ unsigned long long s_top[kMaxK];
+    __shared__ float s_ld[kMaxK], s_td[kMaxK];
     const std::uint32_t lane = threadIdx.x;
     const std::uint64_t lt = (1ull << lane) - 1ull;
     unsigned long long local_upd = 0;
@@ -522,9 +606,12 @@ __global__ void __launch_bounds__(64)
                 const std::uint32_t cid = static_cast<std::uint32_t>(key);
                 const float cd = __uint_as_float(static_cast<std::uint32_t>(key >> 32));
                 bool ok = key != ~0ull && cid != v && cid < id_limit && cd < last;
-                for (std::uint32_t j = 0; j < c; ++j)
-                    ok = ok && static_cast<std::uint32_t>(__builtin_amdgcn_readlane(
-                                   static_cast<int>(my_id), static_cast<int>(j))) != cid;
+#pragma unroll
+                for (std::uint32_t j = 0; j < KC; ++j) {
+                    const std::uint32_t lj = static_cast<std::uint32_t>(
+                        __builtin_amdgcn_readlane(static_cast<int>(my_id), static_cast<int>(j)));
+                    ok = ok && lj != cid;  // unlisted slots hold kNone
+                }
                 const std::uint64_t vm = __ballot(ok);
                 if (ok) s_key[nv + __popcll(vm & lt)] = key;
                 nv += __popcll(vm);
@@ -532,75 +619,40 @@ __global__ void __launch_bounds__(64)
             __syncthreads();
             // 2. sort, keep first copies, rank
             std::uint32_t nvalid = 0;
-            if (nv > 0 && nv <= 64) {
-                unsigned long long key = lane < nv ? s_key[lane] : ~0ull;
-                key = wave_sort_u64(key);
-                const unsigned long long prev = __shfl_up(key, 1, 64);
-                const bool ok = key != ~0ull && (lane == 0 || prev != key);
-                const std::uint64_t vm = __ballot(ok);
-                const std::uint32_t r = __popcll(vm & lt);
-                if (ok && r < k) s_top[r] = key;
-                nvalid = __popcll(vm);
-            } else if (nv > 64) {
-                std::uint32_t P = 128;
-                while (P < nv) P <<= 1;
-                for (std::uint32_t i = nv + lane; i < P; i += 64) s_key[i] = ~0ull;
-                __syncthreads();
-                for (std::uint32_t size = 2; size <= P; size <<= 1) {
-                    for (std::uint32_t half = size >> 1; half > 0; half >>= 1) {
-                        for (std::uint32_t q = lane; q < P / 2; q += 64) {
-                            const std::uint32_t lo = 2 * half * (q / half) + (q % half);
-                            const std::uint32_t hi = lo + half;
-                            const bool up = (lo & size) == 0;
-                            const unsigned long long x = s_key[lo], y = s_key[hi];
-                            if ((x > y) == up) {
-                                s_key[lo] = y;
-                                s_key[hi] = x;
-                            }
-                        }
-                        __syncthreads();
-                    }
-                }
-                for (std::uint32_t i0 = 0; i0 < nv; i0 += 64) {
-                    const std::uint32_t i = i0 + lane;
-                    const unsigned long long key = i < nv ? s_key[i] : ~0ull;
-                    const bool ok = i < nv && (i == 0 || s_key[i - 1] != key);
-                    const std::uint64_t vm = __ballot(ok);
-                    const std::uint32_t r = nvalid + __popcll(vm & lt);
-                    if (ok && r < k) s_top[r] = key;
-                    nvalid += __popcll(vm);
-                }
+            if (nv > 0) {
+                if (nv <= 64) nvalid = sort_unique_top<1>(s_key, nv, k, s_top);
+                else if (nv <= 128) nvalid = sort_unique_top<2>(s_key, nv, k, s_top);
+                else if (nv <= 256) nvalid = sort_unique_top<4>(s_key, nv, k, s_top);
+                else nvalid = sort_unique_top<8>(s_key, nv, k, s_top);
             }
-            __syncthreads();
             t = nvalid < k ? nvalid : k;
+            __syncthreads();  // s_top complete
             // Updates as the reference counts them (nndescent.cpp:48-49): every
             // candidate try_insert would accept against the list as it was
             // before this merge, even if a closer one of this round evicts it.
             if (lane == 0) local_upd += nvalid;
         }
-        const bool has_c = lane < t;
-        const unsigned long long key = has_c ? s_top[lane] : ~0ull;
-        const std::uint32_t cid = static_cast<std::uint32_t>(key);
-        const float cd = __uint_as_float(static_cast<std::uint32_t>(key >> 32));
-        std::uint32_t pos_e = lane, pos_c = lane;
-        for (std::uint32_t j = 0; j < t; ++j) {
-            const float od = __shfl(cd, j, 64);
-            pos_e += (od < my_d) ? 1u : 0u;
-        }
-        for (std::uint32_t j = 0; j < c; ++j) {
-            const float od = __shfl(my_d, j, 64);
-            pos_c += (od <= cd) ? 1u : 0u;
-        }
         if (t > 0) {
-            if (lane < c && pos_e < k) {
-                ids[base + pos_e] = my_id;
-                dist[base + pos_e] = my_d;
-                flags[base + pos_e] = my_f;
+            if (lane < c) s_ld[lane] = my_d;
+            const unsigned long long tk = lane < t ? s_top[lane] : ~0ull;
+            if (lane < t) s_td[lane] = __uint_as_float(static_cast<std::uint32_t>(tk >> 32));
+            __syncthreads();
+            if (lane < c) {
+                const std::uint32_t pos_e = lane + count_before<true>(s_td, t, my_d);
+                if (pos_e < k) {
+                    ids[base + pos_e] = my_id;
+                    dist[base + pos_e] = my_d;
+                    flags[base + pos_e] = my_f;
+                }
             }
-            if (has_c && pos_c < k) {
-                ids[base + pos_c] = cid;
-                dist[base + pos_c] = cd;
-                flags[base + pos_c] = 1;
+            if (lane < t) {
+                const float cd = __uint_as_float(static_cast<std::uint32_t>(tk >> 32));
+                const std::uint32_t pos_c = lane + count_before<false>(s_ld, c, cd);
+                if (pos_c < k) {
+                    ids[base + pos_c] = static_cast<std::uint32_t>(tk);
+                    dist[base + pos_c] = cd;
+                    flags[base + pos_c] = 1;
+                }
             }
         }
         if (lane == 0) {
@@ -612,6 +664,16 @@ __global__ void __launch_bounds__(64)
     // one partial per block; a single-block pass adds them (a device-wide
     // atomic per block contends on one address across the XCDs)
     if (lane == 0) partial[blockIdx.x] = local_upd;
+}
+
+using MergeFn = void (*)(std::uint32_t, std::uint32_t, std::uint32_t, std::uint32_t*, float*,
+                         std::uint8_t*, std::uint32_t*, const unsigned long long*, std::uint32_t*,
+                         std::uint32_t, unsigned long long*);
+
+MergeFn pick_merge(std::uint32_t k) {
+    if (k <= 16) return merge_kernel<16>;
+    if (k <= 32) return merge_kernel<32>;
+    return merge_kernel<64>;
 }
 
 constexpr std::uint32_t kMergeBlocks = 65536;
@@ -731,6 +793,8 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     if (smp > kMaxSample) smp = kMaxSample;
     const unsigned wave_blocks = capped_blocks(std::size_t(n) * 64, 256, 16384);
     if (random_fill && n >= 2) {
+        std::unique_ptr<ScopedKernelTimer> t;
+        if (gp) t.reset(new ScopedKernelTimer(*gp, "nnd_init"));
         init_random_kernel<<<wave_blocks, 256, 0, s>>>(pts, n, stride, k, seed, ids, dist, flags,
                                                        count);
         CSG_LAUNCH_CHECK();
@@ -805,7 +869,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         {
             std::unique_ptr<ScopedKernelTimer> t;
             if (gp) t.reset(new ScopedKernelTimer(*gp, "nnd_merge"));
-            merge_kernel<<<merge_blocks, 64, 0, s>>>(n, n, k, ids, dist, flags, count, b.cand.p,
+            pick_merge(k)<<<merge_blocks, 64, 0, s>>>(n, n, k, ids, dist, flags, count, b.cand.p,
                                                      b.fill.p, cap, b.partial.p);
             CSG_LAUNCH_CHECK();
         }
@@ -1182,7 +1246,7 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
         }
         if (n0) {
             const unsigned merge_blocks = static_cast<unsigned>(std::min<std::size_t>(n0, kMergeBlocks));
-            merge_kernel<<<merge_blocks, 64, 0, s>>>(n0, n1, k, g.ids.p, g.dist.p, g.flags.p,
+            pick_merge(k)<<<merge_blocks, 64, 0, s>>>(n0, n1, k, g.ids.p, g.dist.p, g.flags.p,
                                                      g.count.p, b.cand.p, b.fill.p, cap,
                                                      b.partial.p);
             CSG_LAUNCH_CHECK();
