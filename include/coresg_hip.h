@@ -125,6 +125,12 @@ typedef struct coresg_hip_build_params {
     /* local join: per-node proposal buffer capacity per round (16..512;
      * 0 = library default, 512) */
     uint32_t join_capacity;
+    /* local-join distances: 0 = library choice, 1 = exact FP32 for every
+     * pair, 2 = FP16 Gram on the matrix cores as a filter, 3 = BF16 Gram as a
+     * filter. The filters use a rigorous error bound to skip only pairs that
+     * cannot pass either threshold test and compute every other pair in exact
+     * FP32, so all modes propose the same candidates. */
+    uint32_t gram;
 } coresg_hip_build_params;
 
 typedef struct coresg_hip_build_stats {
@@ -298,9 +304,17 @@ int coresg_hip_graph_kernel_times(coresg_hip_graph* g, const char* kernel, doubl
                                   uint64_t* launches);
 /* Work tallies accumulated while profiling, per kernel name (four u64):
  * "nnd_join": {members staged, exact distances evaluated, proposals
- * written, nodes}; "boruvka_round1": {adjacency entries, vertices, 0, 0}. */
+ * written, nodes}; "boruvka_round1": {adjacency entries, vertices, 0, 0};
+ * "nnd_gram" (Gram-filter joins): {pairs visited, pairs computed exactly,
+ * the largest relative error of a fast distance against FP64 (float bits),
+ * the largest ratio of the bound's lower distance to the FP64 distance
+ * (float bits; at most 1 when the bound holds)}. */
 int coresg_hip_graph_kernel_work(coresg_hip_graph* g, const char* kernel, uint64_t* work4);
 int coresg_hip_graph_reset_timers(coresg_hip_graph* g);
+/* While profiling, also audit the Gram-filter join: an FP64 distance for
+ * every pair it visits, the worst relative error of the fast distance and the
+ * worst bound ratio recorded under "nnd_gram" (off by default; slow). */
+int coresg_hip_graph_set_audit(coresg_hip_graph* g, int on);
 
 #ifdef __cplusplus
 }

@@ -41,7 +41,7 @@ class BuildParams(C.Structure):
                 ("min_selected", C.c_size_t), ("max_selected", C.c_size_t),
                 ("min_per_component", C.c_size_t), ("ensure_connected", C.c_int),
                 ("reps_per_component", C.c_size_t), ("max_reps", C.c_size_t),
-                ("join_capacity", C.c_uint32)]
+                ("join_capacity", C.c_uint32), ("gram", C.c_uint32)]
 
 
 class BuildStats(C.Structure):
@@ -126,6 +126,7 @@ SIGNATURES = [
                                                 C.POINTER(C.c_uint64)]),
     ("coresg_hip_graph_kernel_work", C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_uint64)]),
     ("coresg_hip_graph_reset_timers", C.c_int, [_vp]),
+    ("coresg_hip_graph_set_audit", C.c_int, [_vp, C.c_int]),
     ("coresg_hip_insert", C.c_int, [_vp, C.c_void_p, C.c_uint32, C.c_int, C.c_int,
                                     C.POINTER(BuildParams), C.POINTER(BuildStats)]),
     ("coresg_hip_graph_size", C.c_uint32, [_vp]),
@@ -374,6 +375,9 @@ class Graph:
         w = (C.c_uint64 * 4)()
         check(lib().coresg_hip_graph_kernel_work(self.h, name.encode(), w))
         return [int(v) for v in w]
+
+    def set_audit(self, on: bool):
+        check(lib().coresg_hip_graph_set_audit(self.h, int(bool(on))))
 
     def reset_timers(self):
         check(lib().coresg_hip_graph_reset_timers(self.h))

@@ -46,6 +46,11 @@ void Graph::timer_stop(const char* name, hipEvent_t start) {
     t.pending.push_back(e);
 }
 
+void Graph::max_work(const char* name, int index, std::uint64_t value) {
+    KernelTimer& t = timers[name];
+    if (index >= 0 && index < 4 && value > t.work[index]) t.work[index] = value;
+}
+
 void Graph::add_work(const char* name, std::initializer_list<std::uint64_t> w) {
     KernelTimer& t = timers[name];
     int i = 0;
@@ -144,6 +149,7 @@ void coresg_hip_default_build_params(coresg_hip_build_params* p) {
     p->reps_per_component = 4;
     p->max_reps = 4096;
     p->join_capacity = 0;
+    p->gram = 0;
 }
 
 coresg_hip_graph* coresg_hip_graph_create(int device, uint32_t n, uint32_t dim, uint32_t k_max) {
@@ -348,6 +354,13 @@ int coresg_hip_graph_kernel_work(coresg_hip_graph* h, const char* kernel, uint64
         auto it = h->g.timers.find(kernel);
         for (int i = 0; i < 4; ++i)
             work4[i] = it == h->g.timers.end() ? 0 : it->second.work[i];
+    });
+}
+
+int coresg_hip_graph_set_audit(coresg_hip_graph* h, int on) {
+    return guarded([&] {
+        if (!h) throw std::runtime_error("set_audit: null graph");
+        h->g.audit = on != 0;
     });
 }
 

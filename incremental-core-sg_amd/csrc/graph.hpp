@@ -59,6 +59,8 @@ struct DescentBuffers {
     DBuf<unsigned long long> cand, updates, partial, tally;
     DBuf<std::uint32_t> fill, gids, gcnt;
     DBuf<float> gthr;
+    DBuf<unsigned long long> gram_tally;
+    DBuf<unsigned int> gram_audit;
 };
 
 // Accumulated device time of one kernel (HIP events on the graph's stream).
@@ -126,8 +128,9 @@ struct Graph {
     std::vector<std::uint32_t> selected;
     bool hub_ready = false;
 
-    // Profiling.
+    // Profiling (audit: FP64 check of the Gram-filter join's distances).
     bool profiling = false;
+    bool audit = false;
     std::map<std::string, KernelTimer> timers;
 
     ~Graph();
@@ -136,6 +139,7 @@ struct Graph {
     void timer_stop(const char* name, hipEvent_t start);
     void resolve_timers();
     void add_work(const char* name, std::initializer_list<std::uint64_t> w);
+    void max_work(const char* name, int index, std::uint64_t value);
 };
 
 // RAII bracket for a profiled launch.

@@ -292,6 +292,11 @@ struct JoinArgs {
     unsigned long long* cand;  // n x cap proposal buffer
     std::uint32_t* fill;       // proposals offered per node this round
     std::uint32_t cap;
+    // Gram-filter join only: work tallies (pairs, exact recomputations) and
+    // the audit maxima (fast-distance relative error vs FP64, error / bound),
+    // both nullptr when not profiling
+    unsigned long long* tally;
+    unsigned int* audit;
 };
 
 // Exact squared distance with `a` held in registers (compile-time stride),
@@ -337,58 +342,152 @@ __device__ __forceinline__ float l2sqr_reg(const float (&a)[STRIDE],
     return (l0 + l1) + (l2 + l3);
 }
 
-// One 128-thread group (two wavefronts) per node. LDS: ids[S], thr[S],
-// rows[S*stride] and the distance tile D[A][S+1] (odd pitch: rows and
-// columns are both read without bank conflicts).
-//
-// Distances: the (new i, member j) pairs are spread densely over the threads,
-// each unordered new x new pair once (mirrored into the tile), both rows read
-// from LDS in the reference's exact summation order.
-//
-// Proposals (the reference's try_pair test, nndescent.cpp:41-44): new member
-// x gets every member y with D(x,y) < thr_x (row targets), old member x gets
-// every new member y with D(y,x) < thr_x (column targets). Wavefront w takes
-// the rows x = w (mod 2) -- one ballot per row chunk -- and one chunk of 64
-// columns, a lane per column. All counts are taken first, one batch of
-// atomics reserves the targets' buffer ranges, then the tile is read again to
-// write the keys (distance bits << 32 | id). Proposals at or beyond a target's
-// capacity in this round are dropped.
+// Shared phases of the two local-join kernels. One 128-thread group (two
+// wavefronts) per node. LDS: ids[S], thr[S], rows[S*stride] and the distance
+// tile D[A][S+1] (odd pitch: rows and columns are both read without bank
+// conflicts).
+
+// Stage the node's members (ids, thresholds, rows). Returns false (and touches
+// nothing) for a node without new members.
+__device__ __forceinline__ bool join_stage(const JoinArgs& a, std::uint32_t v, std::uint32_t S,
+                                           std::uint32_t stride, std::uint32_t* s_id,
+                                           float* s_thr, float* s_rows, std::uint32_t* na_out,
+                                           std::uint32_t* ns_out) {
+    const std::uint32_t tid = threadIdx.x;
+    const std::uint32_t cnt = a.cnt[v];
+    const std::uint32_t na = cnt & 0xFFFFu, ns = cnt >> 16;
+    *na_out = na;
+    *ns_out = ns;
+    if (na == 0) return false;  // block-uniform
+    for (std::uint32_t i = tid; i < ns; i += kJoinThreads) {
+        s_id[i] = a.ids[std::size_t(v) * S + i];
+        s_thr[i] = a.thr[std::size_t(v) * S + i];
+    }
+    __syncthreads();
+    const std::uint32_t q4 = stride / 4;
+    for (std::uint32_t e = tid; e < ns * q4; e += kJoinThreads) {
+        const std::uint32_t i = e / q4, f = e - i * q4;
+        reinterpret_cast<float4*>(s_rows)[e] =
+            reinterpret_cast<const float4*>(a.pts + std::size_t(s_id[i]) * stride)[f];
+    }
+    __syncthreads();
+    return true;
+}
+
+// Pair p -> (i, j) = (p / ns, p % ns) over the (new i, member j) pairs; the
+// rounded-up reciprocal gives the exact quotient for p < 2^14 and ns <= 192.
+// Unordered new x new pairs are visited once (i < j).
+struct PairMap {
+    std::uint32_t ns, na;
+    std::uint64_t recip;
+    __device__ PairMap(std::uint32_t ns_, std::uint32_t na_)
+        : ns(ns_), na(na_), recip(((1ull << 24) + ns_ - 1) / ns_) {}
+    __device__ bool operator()(std::uint32_t p, std::uint32_t* i, std::uint32_t* j) const {
+        *i = static_cast<std::uint32_t>((std::uint64_t(p) * recip) >> 24);
+        *j = p - *i * ns;
+        return !(*j < na && *j <= *i);
+    }
+};
+
+// Proposals from the completed tile (the reference's try_pair test,
+// nndescent.cpp:41-44): new member x gets every member y with D(x,y) < thr_x
+// (row targets), old member x gets every new member y with D(y,x) < thr_x
+// (column targets). Wavefront w takes the rows x = w (mod 2) -- one ballot
+// per row chunk -- and one chunk of 64 columns, a lane per column. All counts
+// are taken first, one batch of atomics reserves the targets' buffer ranges,
+// then the tile is read again to write the keys (distance bits << 32 | id).
+// Proposals at or beyond a target's capacity in this round are dropped.
+__device__ __forceinline__ void join_propose(const JoinArgs& a, const std::uint32_t* s_id,
+                                             const float* s_thr, const float* s_D,
+                                             std::uint32_t P, std::uint32_t na,
+                                             std::uint32_t ns) {
+    const std::uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    const std::uint64_t below = (1ull << lane) - 1ull;
+    std::uint32_t rcount = 0;
+    for (std::uint32_t x = w; x < na; x += 2) {
+        const float thr = s_thr[x];
+        std::uint32_t c = 0;
+        for (std::uint32_t y0 = 0; y0 < ns; y0 += 64) {
+            const std::uint32_t y = y0 + lane;
+            c += __popcll(__ballot(y < ns && s_D[x * P + y] < thr));
+        }
+        if (lane == (x >> 1)) rcount = c;
+    }
+    const std::uint32_t xc = na + 64u * w + lane;
+    const bool has_col = xc < ns;
+    const float thr_c = has_col ? s_thr[xc] : -1.0f;
+    std::uint32_t ccount = 0;
+    if (has_col)
+        for (std::uint32_t y = 0; y < na; ++y) ccount += s_D[y * P + xc] < thr_c ? 1u : 0u;
+    // reserve
+    const std::uint32_t xr = 2u * lane + w;
+    std::uint32_t rbase = a.cap, cbase = a.cap;
+    if (rcount) rbase = atomicAdd(&a.fill[s_id[xr]], rcount);
+    if (ccount) cbase = atomicAdd(&a.fill[s_id[xc]], ccount);
+    // write row targets
+    for (std::uint32_t x = w; x < na; x += 2) {
+        const float thr = s_thr[x];
+        std::uint32_t pos0 = __shfl(rbase, static_cast<int>(x >> 1), 64);
+        unsigned long long* tgt = a.cand + std::size_t(s_id[x]) * a.cap;
+        for (std::uint32_t y0 = 0; y0 < ns; y0 += 64) {
+            const std::uint32_t y = y0 + lane;
+            const float d = y < ns ? s_D[x * P + y] : __builtin_inff();
+            const bool pass = d < thr;
+            const std::uint64_t m = __ballot(pass);
+            const std::uint32_t pos = pos0 + __popcll(m & below);
+            if (pass && pos < a.cap)
+                tgt[pos] = (static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32) | s_id[y];
+            pos0 += __popcll(m);
+        }
+    }
+    // write column targets
+    if (ccount) {
+        unsigned long long* tgt = a.cand + std::size_t(s_id[xc]) * a.cap;
+        std::uint32_t pos = cbase;
+        for (std::uint32_t y = 0; y < na && pos < a.cap; ++y) {
+            const float d = s_D[y * P + xc];
+            if (!(d < thr_c)) continue;
+            tgt[pos++] = (static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32) | s_id[y];
+        }
+    }
+    // columns beyond the first 128 (sample > 21 only)
+    for (std::uint32_t xb = na + 128u + 64u * w; xb < ns; xb += 128u) {
+        const std::uint32_t x = xb + lane;
+        if (x >= ns) continue;
+        const float thr = s_thr[x];
+        std::uint32_t c = 0;
+        for (std::uint32_t y = 0; y < na; ++y) c += s_D[y * P + x] < thr ? 1u : 0u;
+        if (c == 0) continue;
+        std::uint32_t pos = atomicAdd(&a.fill[s_id[x]], c);
+        unsigned long long* tgt = a.cand + std::size_t(s_id[x]) * a.cap;
+        for (std::uint32_t y = 0; y < na && pos < a.cap; ++y) {
+            const float d = s_D[y * P + x];
+            if (!(d < thr)) continue;
+            tgt[pos++] = (static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32) | s_id[y];
+        }
+    }
+}
+
+// Exact local join: every (new i, member j) pair's distance is computed in
+// the reference's exact summation order (both rows from LDS), the pairs
+// spread densely over the threads.
 template <std::uint32_t STRIDE>
 __global__ void __launch_bounds__(kJoinThreads) join_kernel(JoinArgs a) {
     extern __shared__ float lds[];
-    const std::uint32_t smp = a.smp;
-    const std::uint32_t S = 6u * smp, P = S + 1;
+    const std::uint32_t S = 6u * a.smp, P = S + 1;
     const std::uint32_t stride = STRIDE ? STRIDE : a.stride;
     std::uint32_t* s_id = reinterpret_cast<std::uint32_t*>(lds);
     float* s_thr = lds + S;
     float* s_rows = lds + 2 * S;
     float* s_D = s_rows + std::size_t(S) * stride;
-    const std::uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-    const std::uint64_t below = (1ull << lane) - 1ull;
+    const std::uint32_t tid = threadIdx.x;
     for (std::uint32_t v = blockIdx.x; v < a.n; v += gridDim.x) {
-        const std::uint32_t cnt = a.cnt[v];
-        const std::uint32_t na = cnt & 0xFFFFu, ns = cnt >> 16;
-        if (na == 0) continue;  // block-uniform; LDS untouched this iteration
-        for (std::uint32_t i = tid; i < ns; i += kJoinThreads) {
-            s_id[i] = a.ids[std::size_t(v) * S + i];
-            s_thr[i] = a.thr[std::size_t(v) * S + i];
-        }
-        __syncthreads();
-        const std::uint32_t q4 = stride / 4;
-        for (std::uint32_t e = tid; e < ns * q4; e += kJoinThreads) {
-            const std::uint32_t i = e / q4, f = e - i * q4;
-            reinterpret_cast<float4*>(s_rows)[e] =
-                reinterpret_cast<const float4*>(a.pts + std::size_t(s_id[i]) * stride)[f];
-        }
-        __syncthreads();
-        // pair p -> (i, j) = (p / ns, p % ns); the rounded-up reciprocal
-        // gives the exact quotient for p < 2^14 and ns <= 192
-        const std::uint32_t npair = na * ns;
-        const std::uint64_t recip = ((1ull << 24) + ns - 1) / ns;
-        for (std::uint32_t p = tid; p < npair; p += kJoinThreads) {
-            const std::uint32_t i = static_cast<std::uint32_t>((std::uint64_t(p) * recip) >> 24);
-            const std::uint32_t j = p - i * ns;
-            if (j < na && j <= i) continue;
+        std::uint32_t na, ns;
+        if (!join_stage(a, v, S, stride, s_id, s_thr, s_rows, &na, &ns)) continue;
+        const PairMap pm(ns, na);
+        for (std::uint32_t p = tid; p < na * ns; p += kJoinThreads) {
+            std::uint32_t i, j;
+            if (!pm(p, &i, &j)) continue;
             const float d = l2sqr_exact(s_rows + std::size_t(i) * stride,
                                         s_rows + std::size_t(j) * stride, stride);
             s_D[i * P + j] = d;
@@ -396,72 +495,233 @@ __global__ void __launch_bounds__(kJoinThreads) join_kernel(JoinArgs a) {
         }
         for (std::uint32_t i = tid; i < na; i += kJoinThreads) s_D[i * P + i] = __builtin_inff();
         __syncthreads();
-        // counts: rows x = w (mod 2), lane x/2 keeps row x's; columns
-        // na + 64w + lane
-        std::uint32_t rcount = 0;
-        for (std::uint32_t x = w; x < na; x += 2) {
-            const float thr = s_thr[x];
-            std::uint32_t c = 0;
-            for (std::uint32_t y0 = 0; y0 < ns; y0 += 64) {
-                const std::uint32_t y = y0 + lane;
-                c += __popcll(__ballot(y < ns && s_D[x * P + y] < thr));
-            }
-            if (lane == (x >> 1)) rcount = c;
+        join_propose(a, s_id, s_thr, s_D, P, na, ns);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gram-filter local join (matrix cores)
+// ---------------------------------------------------------------------------
+//
+// The rows of a node's members are centred on its first member, rounded to
+// FP16 (or BF16) and contracted on the matrix cores (16x16x16 MFMA, FP32
+// accumulation): D~(i,j) = N_i + N_j - 2 G_ij with N the squared norms of
+// the rounded rows. A rigorous bound turns D~ into a lower bound on the
+// exact distance r that the reference's summation would produce:
+//   |S - T| <= delta = rho (|a_i| + |a_j|) + 2 abs     (rounding of the rows)
+//   |D~ - S^2| <= eps = cG (|a_i| + |a_j|)^2           (FP32 accumulation)
+//   r >= T^2 (1 - eta)                                  (the exact sum's rounding)
+// with T the true distance, S the distance of the rounded rows, rho the
+// format's unit roundoff (plus the centring's), abs its flush-to-zero floor per
+// coordinate (times sqrt(K)), cG = 2 (K + 8) 2^-24 and eta = (stride + 24)
+// 2^-24 + 2^-18; every bound is rounded outward. A pair whose lower bound
+// reaches both members' thresholds cannot pass either try_pair test and gets
+// +inf; every other pair gets its exact distance. The proposals are therefore
+// exactly those of the exact join: the matrix cores only decide which exact
+// distances are needed.
+
+typedef _Float16 gram_f16x4 __attribute__((ext_vector_type(4)));
+typedef short gram_i16x4 __attribute__((ext_vector_type(4)));
+typedef float gram_f32x4 __attribute__((ext_vector_type(4)));
+
+// fp32 -> bf16 bits, round to nearest even (finite inputs).
+__device__ __forceinline__ std::uint32_t bf16_bits(float x) {
+    std::uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+
+template <bool BF16>
+__device__ __forceinline__ float gram_round(float x) {
+    if constexpr (BF16) return __uint_as_float(bf16_bits(x) << 16);
+    else return static_cast<float>(static_cast<_Float16>(x));
+}
+
+template <bool BF16>
+struct GramConst {
+    // unit roundoff of the format plus the centring's fp32 rounding
+    static constexpr float rho = BF16 ? (0x1p-8f + 0x1p-23f) : (0x1p-11f + 0x1p-23f);
+    // per-coordinate absolute floor (a flushed subnormal)
+    static constexpr float absc = BF16 ? 0x1p-126f : 0x1p-14f;
+};
+
+template <bool BF16>
+__device__ __forceinline__ gram_f32x4 gram_mfma(const float (&x)[4], const float (&y)[4],
+                                                gram_f32x4 acc) {
+    if constexpr (BF16) {
+        gram_i16x4 av, bv;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            av[q] = static_cast<short>(bf16_bits(x[q]));
+            bv[q] = static_cast<short>(bf16_bits(y[q]));
         }
-        const std::uint32_t xc = na + 64u * w + lane;
-        const bool has_col = xc < ns;
-        const float thr_c = has_col ? s_thr[xc] : -1.0f;
-        std::uint32_t ccount = 0;
-        if (has_col)
-            for (std::uint32_t y = 0; y < na; ++y) ccount += s_D[y * P + xc] < thr_c ? 1u : 0u;
-        // reserve
-        const std::uint32_t xr = 2u * lane + w;
-        std::uint32_t rbase = a.cap, cbase = a.cap;
-        if (rcount) rbase = atomicAdd(&a.fill[s_id[xr]], rcount);
-        if (ccount) cbase = atomicAdd(&a.fill[s_id[xc]], ccount);
-        // write row targets
-        for (std::uint32_t x = w; x < na; x += 2) {
-            const float thr = s_thr[x];
-            std::uint32_t pos0 = __shfl(rbase, static_cast<int>(x >> 1), 64);
-            unsigned long long* tgt = a.cand + std::size_t(s_id[x]) * a.cap;
-            for (std::uint32_t y0 = 0; y0 < ns; y0 += 64) {
-                const std::uint32_t y = y0 + lane;
-                const float d = y < ns ? s_D[x * P + y] : __builtin_inff();
-                const bool pass = d < thr;
-                const std::uint64_t m = __ballot(pass);
-                const std::uint32_t pos = pos0 + __popcll(m & below);
-                if (pass && pos < a.cap)
-                    tgt[pos] = (static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32) | s_id[y];
-                pos0 += __popcll(m);
-            }
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    } else {
+        gram_f16x4 av, bv;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            av[q] = static_cast<_Float16>(x[q]);
+            bv[q] = static_cast<_Float16>(y[q]);
         }
-        // write column targets
-        if (ccount) {
-            unsigned long long* tgt = a.cand + std::size_t(s_id[xc]) * a.cap;
-            std::uint32_t pos = cbase;
-            for (std::uint32_t y = 0; y < na && pos < a.cap; ++y) {
-                const float d = s_D[y * P + xc];
-                if (!(d < thr_c)) continue;
-                tgt[pos++] = (static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32) | s_id[y];
+        return __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, acc, 0, 0, 0);
+    }
+}
+
+// Four centred coordinates k..k+3 of LDS row r (zeros past the stride or for
+// rows beyond `rows`).
+__device__ __forceinline__ void centred4(const float* s_rows, std::uint32_t stride,
+                                         std::uint32_t r, std::uint32_t rows, std::uint32_t k,
+                                         float (&out)[4]) {
+    if (r < rows && k < stride) {
+        const float4 x = *reinterpret_cast<const float4*>(s_rows + std::size_t(r) * stride + k);
+        const float4 c = *reinterpret_cast<const float4*>(s_rows + k);
+        out[0] = x.x - c.x;
+        out[1] = x.y - c.y;
+        out[2] = x.z - c.z;
+        out[3] = x.w - c.w;
+    } else {
+        out[0] = out[1] = out[2] = out[3] = 0.0f;
+    }
+}
+
+__device__ __forceinline__ unsigned int wave_max_u32(unsigned int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned int o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// LDS: ids[S], thr[S], rows[S*stride], D[A][S+1], N[S] (squared norms of the
+// rounded centred rows), nb[S] (outward bound on both norms of a row).
+template <std::uint32_t STRIDE, bool BF16>
+__global__ void __launch_bounds__(kJoinThreads) join_gram_kernel(JoinArgs a) {
+    extern __shared__ float lds[];
+    const std::uint32_t S = 6u * a.smp, A = 2u * a.smp, P = S + 1;
+    const std::uint32_t stride = STRIDE ? STRIDE : a.stride;
+    const std::uint32_t KP = (stride + 15u) & ~15u;
+    std::uint32_t* s_id = reinterpret_cast<std::uint32_t*>(lds);
+    float* s_thr = lds + S;
+    float* s_rows = lds + 2 * S;
+    float* s_D = s_rows + std::size_t(S) * stride;
+    float* s_N = s_D + std::size_t(A) * P;
+    float* s_nb = s_N + S;
+    const std::uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    const float rho = GramConst<BF16>::rho;
+    const float abs_norm = GramConst<BF16>::absc * sqrtf(float(KP)) * 1.01f;
+    const float cG = 2.0f * float(KP + 8) * 0x1p-24f;
+    const float eta = float(stride + 24) * 0x1p-24f + 0x1p-18f;
+    const float nscale = 1.0f + float(KP + 4) * 0x1p-24f;
+    unsigned long long n_pairs = 0, n_exact = 0;
+    unsigned int worst_rel = 0, worst_ratio = 0;
+    for (std::uint32_t v = blockIdx.x; v < a.n; v += gridDim.x) {
+        std::uint32_t na, ns;
+        if (!join_stage(a, v, S, stride, s_id, s_thr, s_rows, &na, &ns)) continue;
+        // 1. squared norms of the rounded centred rows and their outward bound
+        for (std::uint32_t j = tid; j < ns; j += kJoinThreads) {
+            float N = 0.0f;
+            for (std::uint32_t k = 0; k < stride; k += 4) {
+                float c4[4];
+                centred4(s_rows, stride, j, ns, k, c4);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float h = gram_round<BF16>(c4[q]);
+                    N = __builtin_fmaf(h, h, N);
+                }
             }
+            s_N[j] = N;
+            s_nb[j] = (sqrtf(N) * nscale + abs_norm) * (1.0f + 2.0f * rho);
         }
-        // columns beyond the first 128 (sample > 21 only)
-        for (std::uint32_t xb = na + 128u + 64u * w; xb < ns; xb += 128u) {
-            const std::uint32_t x = xb + lane;
-            if (x >= ns) continue;
-            const float thr = s_thr[x];
-            std::uint32_t c = 0;
-            for (std::uint32_t y = 0; y < na; ++y) c += s_D[y * P + x] < thr ? 1u : 0u;
-            if (c == 0) continue;
-            std::uint32_t pos = atomicAdd(&a.fill[s_id[x]], c);
-            unsigned long long* tgt = a.cand + std::size_t(s_id[x]) * a.cap;
-            for (std::uint32_t y = 0; y < na && pos < a.cap; ++y) {
-                const float d = s_D[y * P + x];
-                if (!(d < thr)) continue;
-                tgt[pos++] = (static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32) | s_id[y];
+        __syncthreads();
+        // 2. Gram tiles of 16 new rows x 16 members; wavefront w takes tiles
+        // w, w+2, ... Lane (li, lk) supplies row/column li, coordinates
+        // lk..lk+3 of each 16-wide step and receives G[4*(lane/16)+q][li].
+        const std::uint32_t rt = (na + 15u) / 16u, ct = (ns + 15u) / 16u;
+        const std::uint32_t li = lane & 15u, lk = 4u * (lane >> 4);
+        for (std::uint32_t t = w; t < rt * ct; t += 2) {
+            const std::uint32_t r0 = 16u * (t / ct), c0 = 16u * (t % ct);
+            gram_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (std::uint32_t k0 = 0; k0 < KP; k0 += 16) {
+                float x[4], y[4];
+                centred4(s_rows, stride, r0 + li, na, k0 + lk, x);
+                centred4(s_rows, stride, c0 + li, ns, k0 + lk, y);
+                acc = gram_mfma<BF16>(x, y, acc);
+            }
+            const std::uint32_t j = c0 + li;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const std::uint32_t i = r0 + lk + static_cast<std::uint32_t>(q);
+                if (i < na && j < ns) s_D[i * P + j] = (s_N[i] + s_N[j]) - 2.0f * acc[q];
             }
         }
         __syncthreads();
+        // 3. pair pass: +inf where the bound rules out both tests, else the
+        // exact distance
+        const PairMap pm(ns, na);
+        for (std::uint32_t p = tid; p < na * ns; p += kJoinThreads) {
+            std::uint32_t i, j;
+            if (!pm(p, &i, &j)) continue;
+            const float dt = s_D[i * P + j];
+            const float nbs = s_nb[i] + s_nb[j];
+            float tlo = 0.0f;
+            const float s2 = dt - cG * nbs * nbs;
+            if (s2 > 0.0f) {
+                const float slo = sqrtf(s2) * (1.0f - 0x1p-20f);
+                const float delta = (rho * nbs + 2.0f * abs_norm) * (1.0f + 0x1p-16f) + slo * 0x1p-21f;
+                tlo = slo - delta;
+            }
+            bool reject = false;
+            if (tlo > 0.0f) {
+                const float rlo = tlo * tlo * (1.0f - eta);
+                reject = rlo >= s_thr[i] && rlo >= s_thr[j];
+            }
+            const float* ri = s_rows + std::size_t(i) * stride;
+            const float* rj = s_rows + std::size_t(j) * stride;
+            const float d = reject ? __builtin_inff() : l2sqr_exact(ri, rj, stride);
+            s_D[i * P + j] = d;
+            if (j < na) s_D[j * P + i] = d;
+            if (a.tally) {
+                ++n_pairs;
+                n_exact += reject ? 0u : 1u;
+            }
+            if (a.audit) {
+                double t2 = 0.0;
+                for (std::uint32_t k = 0; k < stride; ++k) {
+                    const double e = double(ri[k]) - double(rj[k]);
+                    t2 += e * e;
+                }
+                const double T = sqrt(t2);
+                if (T > 0.0) {
+                    const double fast = sqrt(dt > 0.0f ? double(dt) : 0.0);
+                    const float rel = static_cast<float>(fabs(fast - T) / T);
+                    const float ratio = static_cast<float>(double(tlo > 0.0f ? tlo : 0.0f) / T);
+                    worst_rel = max(worst_rel, __float_as_uint(rel));
+                    worst_ratio = max(worst_ratio, __float_as_uint(ratio));
+                }
+            }
+        }
+        for (std::uint32_t i = tid; i < na; i += kJoinThreads) s_D[i * P + i] = __builtin_inff();
+        __syncthreads();
+        join_propose(a, s_id, s_thr, s_D, P, na, ns);
+        __syncthreads();
+    }
+    if (a.tally) {
+        n_pairs = wave_sum(n_pairs);
+        n_exact = wave_sum(n_exact);
+        if (lane == 0) {
+            atomicAdd(&a.tally[0], n_pairs);
+            atomicAdd(&a.tally[1], n_exact);
+        }
+    }
+    if (a.audit) {
+        worst_rel = wave_max_u32(worst_rel);
+        worst_ratio = wave_max_u32(worst_ratio);
+        if (lane == 0) {
+            atomicMax(&a.audit[0], worst_rel);
+            atomicMax(&a.audit[1], worst_ratio);
+        }
     }
 }
 
@@ -772,8 +1032,38 @@ JoinFn pick_join(std::uint32_t stride) {
         case 32: return join_kernel<32>;
         case 48: return join_kernel<48>;
         case 64: return join_kernel<64>;
+        case 128: return join_kernel<128>;
         default: return join_kernel<0>;
     }
+}
+
+template <bool BF16>
+JoinFn pick_join_gram_t(std::uint32_t stride) {
+    switch (stride) {
+        case 8: return join_gram_kernel<8, BF16>;
+        case 16: return join_gram_kernel<16, BF16>;
+        case 24: return join_gram_kernel<24, BF16>;
+        case 32: return join_gram_kernel<32, BF16>;
+        case 48: return join_gram_kernel<48, BF16>;
+        case 64: return join_gram_kernel<64, BF16>;
+        case 128: return join_gram_kernel<128, BF16>;
+        default: return join_gram_kernel<0, BF16>;
+    }
+}
+
+// Local-join distance modes (coresg_hip_build_params::gram).
+enum : std::uint32_t { kGramAuto = 0, kGramExact = 1, kGramF16 = 2, kGramBF16 = 3 };
+
+// The library's choice is the exact join: in NN-Descent a third (d = 24) to
+// almost half (d = 128) of the visited pairs pass a threshold test and need
+// their exact distance anyway, and the Gram phase, the norms and the bound
+// cost more than the exact distances they save (MI355X, n = 1M, d = 24:
+// 51 ms exact vs 78 ms FP16 filter per build; n = 200k, d = 128: 54 vs 68 ms).
+std::uint32_t resolve_gram(std::uint32_t mode, std::uint32_t stride) {
+    (void)stride;
+    if (mode == kGramAuto) return kGramExact;
+    if (mode > kGramBF16) throw std::runtime_error("nndescent: gram must be 0..3");
+    return mode;
 }
 
 struct DescentResult {
@@ -785,7 +1075,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
                           std::uint32_t k, std::uint32_t* ids, float* dist, std::uint8_t* flags,
                           std::uint32_t* count, std::uint32_t smp, std::uint32_t iters,
                           double delta, std::uint64_t seed, bool random_fill,
-                          std::uint32_t cap, hipStream_t s) {
+                          std::uint32_t cap, std::uint32_t gram, hipStream_t s) {
     DescentResult res;
     if (n == 0) return res;
     if (k > kMaxK) throw std::runtime_error("nndescent: k above 64 is not supported on device");
@@ -821,9 +1111,14 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     b.gids.reserve(std::size_t(n) * S);
     b.gthr.reserve(std::size_t(n) * S);
     b.gcnt.reserve(n);
-    const JoinFn join_fn = pick_join(stride);
+    gram = resolve_gram(gram, stride);
+    const JoinFn join_fn = gram == kGramExact ? pick_join(stride)
+                           : gram == kGramF16 ? pick_join_gram_t<false>(stride)
+                                              : pick_join_gram_t<true>(stride);
     const std::size_t lds =
-        (2 * std::size_t(S) + std::size_t(S) * stride + std::size_t(A) * (S + 1)) * sizeof(float);
+        (2 * std::size_t(S) + std::size_t(S) * stride + std::size_t(A) * (S + 1) +
+         (gram == kGramExact ? 0 : 2 * std::size_t(S))) *
+        sizeof(float);
     const unsigned join_threads = kJoinThreads;
     const unsigned join_grid = static_cast<unsigned>(std::min<std::size_t>(n, 65536));
     if (lds > 150 * 1024) throw std::runtime_error("nndescent: candidate tile exceeds LDS");
@@ -831,9 +1126,17 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         CSG_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(join_fn),
                                       hipFuncAttributeMaxDynamicSharedMemorySize,
                                       static_cast<int>(lds)));
-    JoinArgs ja{pts, n, stride, smp, b.gids.p, b.gthr.p, b.gcnt.p, b.cand.p, b.fill.p, cap};
+    JoinArgs ja{pts, n, stride, smp, b.gids.p, b.gthr.p, b.gcnt.p, b.cand.p, b.fill.p, cap,
+                nullptr, nullptr};
     const unsigned merge_blocks = static_cast<unsigned>(std::min<std::size_t>(n, kMergeBlocks));
     const bool tallies = gp && gp->profiling;
+    const bool gram_audit = tallies && gram != kGramExact;
+    if (gram_audit) {
+        b.gram_tally.reserve(2);
+        b.gram_audit.reserve(2);
+        ja.tally = b.gram_tally.p;
+        ja.audit = gp->audit ? b.gram_audit.p : nullptr;
+    }
     const double stop = delta * double(k) * double(n);
     for (std::uint32_t round = 0; round < iters; ++round) {
         fill_u32(b.rev_new_cnt.p, n, 0u, s);
@@ -854,6 +1157,10 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
                 b.rev_new.p, b.rev_new_cnt.p, b.rev_old.p, b.rev_old_cnt.p, b.gids.p, b.gthr.p,
                 b.gcnt.p);
             CSG_LAUNCH_CHECK();
+        }
+        if (gram_audit) {
+            fill_u64(reinterpret_cast<std::uint64_t*>(b.gram_tally.p), 2, 0ull, s);
+            fill_u32(b.gram_audit.p, 2, 0u, s);
         }
         {
             std::unique_ptr<ScopedKernelTimer> t;
@@ -878,8 +1185,19 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         unsigned long long upd = 0, tl[3] = {0, 0, 0};
         CSG_CHECK(hipMemcpyAsync(&upd, b.updates.p, 8, hipMemcpyDeviceToHost, s));
         if (tallies) CSG_CHECK(hipMemcpyAsync(tl, b.tally.p, sizeof(tl), hipMemcpyDeviceToHost, s));
+        unsigned long long gt[2] = {0, 0};
+        unsigned int ga[2] = {0, 0};
+        if (gram_audit) {
+            CSG_CHECK(hipMemcpyAsync(gt, b.gram_tally.p, sizeof(gt), hipMemcpyDeviceToHost, s));
+            CSG_CHECK(hipMemcpyAsync(ga, b.gram_audit.p, sizeof(ga), hipMemcpyDeviceToHost, s));
+        }
         CSG_CHECK(hipStreamSynchronize(s));
         if (tallies) gp->add_work("nnd_join", {tl[0], tl[1], tl[2], n});
+        if (gram_audit) {
+            gp->add_work("nnd_gram", {gt[0], gt[1], 0, 0});
+            gp->max_work("nnd_gram", 2, ga[0]);
+            gp->max_work("nnd_gram", 3, ga[1]);
+        }
         res.updates += upd;
         res.rounds = round + 1;
         if (double(upd) <= stop) break;
@@ -1119,7 +1437,7 @@ void build_knn(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_stat
         const std::uint32_t cap = p.join_capacity ? p.join_capacity : kDefaultCap;
         check_join_capacity(cap);
         r = run_descent(&g, g.pts.p, n, g.stride, k, g.ids.p, g.dist.p, g.flags.p, g.count.p,
-                        p.sample, p.iters, p.delta, p.seed, true, cap, s);
+                        p.sample, p.iters, p.delta, p.seed, true, cap, p.gram, s);
     }
     CSG_CHECK(hipStreamSynchronize(s));
     g.knn_set = true;
@@ -1161,7 +1479,7 @@ void nndescent_host(int device, const coresg_hip_nnd_config& cfg, const float* v
     const DescentResult r =
         run_descent(nullptr, d_pts.p, cfg.n, cfg.stride, cfg.k, d_ids.p, d_dist.p, d_flags.p,
                     d_cnt.p, cfg.sample, cfg.iters, cfg.delta, cfg.seed, cfg.random_fill != 0,
-                    kDefaultCap, s);
+                    kDefaultCap, kGramAuto, s);
     CSG_CHECK(hipMemcpyAsync(ids, d_ids.p, nk * 4, hipMemcpyDeviceToHost, s));
     CSG_CHECK(hipMemcpyAsync(dists, d_dist.p, nk * 4, hipMemcpyDeviceToHost, s));
     CSG_CHECK(hipMemcpyAsync(flags, d_flags.p, nk, hipMemcpyDeviceToHost, s));

@@ -151,6 +151,73 @@ def test_nndescent_recall_at_least_reference(name):
     assert ours >= ref - 1e-9
 
 
+def _gram_builds(x, k, gram, runs=5):
+    """Builds with the given local-join mode; returns the exact recalls, the
+    'nnd_gram' work tallies of the last build and checks stored distances."""
+    import struct
+    xp = oracle.pad(x)
+    recalls, work = [], None
+    for _ in range(runs):
+        g = chip.Graph(x.shape[0], x.shape[1], k)
+        g.set_points(x)
+        g.set_profiling(True)
+        g.set_audit(True)
+        g.build_knn(chip.build_params(gram=gram))
+        if gram != 1:
+            work = g.kernel_work("nnd_gram")
+        ids, dist, flags, count = g.get_knn()
+        g.close()
+        nn = np.repeat(np.arange(x.shape[0]), k)
+        sel = ids != 0xFFFFFFFF
+        _, d = oracle.l2sqr_pairs(xp, np.stack([nn[sel], ids[sel]], 1).astype(np.uint32))
+        assert np.array_equal(d.view(np.uint32), dist[sel].view(np.uint32))
+        recalls.append(exact_recall(ids, count, xp, k))
+    if work is not None:
+        f = lambda b: struct.unpack("f", struct.pack("I", b & 0xFFFFFFFF))[0]
+        work = dict(pairs=work[0], exact=work[1], max_rel=f(work[2]), max_ratio=f(work[3]))
+    return recalls, work
+
+
+@pytest.mark.parametrize("gram", [2, 3])
+@pytest.mark.parametrize("name", ["mix_d16", "mix_d24"])
+def test_gram_filter_join_bound_and_recall(name, gram):
+    """The FP16 (2) / BF16 (3) Gram filter skips only pairs whose rigorous
+    lower bound reaches both thresholds, so it proposes exactly what the exact
+    join proposes. Checked: on every visited pair the bound's lower distance
+    stays at or below the FP64 distance (audit ratio <= 1), every stored
+    distance is the host's exact one, and the median recall of five builds is
+    at least the reference's serial descent."""
+    print(f"[gpu] gram filter join: {name} gram={gram}", flush=True)
+    gd = golden(name)
+    x = gd["data"]
+    k = int(gd["k"])
+    f = coresg_io.loads(gd["coresg_bytes"].tobytes())
+    ref = exact_recall(f.ids, f.count, oracle.pad(x), k)
+    recalls, work = _gram_builds(x, k, gram)
+    print(f"  recall median={np.median(recalls):.5f} reference={ref:.5f} audit={work}", flush=True)
+    assert work["pairs"] > 0 and 0 < work["exact"] <= work["pairs"]
+    assert work["max_ratio"] <= 1.0
+    # the fast distances' worst relative error is reported, not gated: it is
+    # large for near-duplicate pairs, where the bound's absolute terms dominate
+    assert 0.0 < work["max_rel"] < 1.0
+    assert float(np.median(recalls)) >= ref - 1e-9
+
+
+def test_gram_filter_join_d128():
+    """d = 128 (the mixed-precision configuration's row length) on a small
+    anisotropic mixture: BF16 filter audit bound and recall against the exact
+    join's (same gate as above, the exact mode standing in for the reference)."""
+    print("[gpu] gram filter join: anisotropic d=128", flush=True)
+    x = datagen.anisotropic_mixture(6000, 128, 16, 0.01, 4.0, seed=4)
+    k = 32
+    exact, _ = _gram_builds(x, k, 1, runs=3)
+    bf16, work = _gram_builds(x, k, 3, runs=3)
+    print(f"  recall exact={np.median(exact):.5f} bf16={np.median(bf16):.5f} audit={work}",
+          flush=True)
+    assert work["max_ratio"] <= 1.0
+    assert float(np.median(bf16)) >= float(np.median(exact)) - 0.005
+
+
 REF_CLI = os.path.join(REPO, "oracle", "_ref", "coresg_cli")
 REF_DUMP = os.path.join(REPO, "oracle", "_ref", "ref_dump")
 
