@@ -962,7 +962,8 @@ __global__ void __launch_bounds__(256)
         const std::uint32_t na = cnt[v] & 0xFFFFu, ns = cnt[v] >> 16;
         if (na == 0) continue;
         rows += ns;
-        dists += static_cast<unsigned long long>(na) * ns - na;
+        // new x new pairs once, new x old pairs
+        dists += static_cast<unsigned long long>(na) * ns - static_cast<unsigned long long>(na) * (na + 1) / 2;
         props += fill[v] < cap ? fill[v] : cap;
     }
     rows = wave_sum(rows);
