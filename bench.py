@@ -99,6 +99,85 @@ def cpu_baseline(x_full, k, sample_n, mpts, threads):
                 msts_per_s=len(mpts) / extract_wall)
 
 
+def serving_run(chip, datagen, k, seed_n, batches, insert_rate, extract_rate, budget, seed=3):
+    """The incremental serving workload on one device-resident CORE-SG: a seed
+    graph, then a stream of insert batches (sizes log-uniform in [1k, 50k],
+    every fourth from components the seed does not have) and extract queries
+    (a random subset of 1..8 mpts values from 2..32), each kind arriving as a
+    Poisson process, submitted by an open-loop client while the session's
+    worker serves them. Latency = submission to completion, per request."""
+    rng = np.random.default_rng(seed)
+    d = 24
+    seed_pts = datagen.gaussian_mixture(seed_n, d, 32, 0.2, 3.0, 0.02, seed=seed)
+    sizes = np.exp(rng.uniform(np.log(1_000), np.log(50_000), size=batches)).astype(int)
+    inserts = []
+    for b, m in enumerate(sizes):
+        # same centre box; new components (a fresh generator seed) every fourth batch
+        src_seed = seed if b % 4 else 1000 + b
+        inserts.append(datagen.gaussian_mixture(int(m), d, 32 if b % 4 else 4, 0.2, 3.0, 0.02,
+                                                seed=src_seed * 7919 + b))
+    t_ins = np.cumsum(rng.exponential(1.0 / insert_rate, size=batches))
+    horizon = float(t_ins[-1])
+    n_ext = max(1, int(rng.poisson(extract_rate * horizon)))
+    t_ext = np.sort(rng.uniform(0.0, horizon, size=n_ext))
+    ext_sets = [sorted(rng.choice(np.arange(2, 33), size=int(rng.integers(1, 9)), replace=False))
+                for _ in range(n_ext)]
+    events = sorted([(float(t), "insert", i) for i, t in enumerate(t_ins)] +
+                    [(float(t), "extract", i) for i, t in enumerate(t_ext)])
+    params = chip.build_params(sample=16, iters=10)
+    t0 = time.perf_counter()
+    sess = chip.Session(seed_pts, k, params, hbm_budget=budget)
+    seed_build_s = time.perf_counter() - t0
+    print(f"[bench] serving: seed {seed_n} points built in {seed_build_s * 1e3:.1f} ms; "
+          f"{batches} inserts ({int(sizes.sum())} points), {n_ext} extracts over {horizon:.2f} s",
+          file=sys.stderr, flush=True)
+    ids = []
+    start = time.perf_counter()
+    for t, kind, i in events:
+        delay = start + t - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        rid = sess.submit_insert(inserts[i]) if kind == "insert" else sess.submit_extract(ext_sets[i])
+        ids.append((kind, rid))
+    lat = {"insert": [], "extract": []}
+    run_ms = {"insert": [], "extract": []}
+    phases = {"knn_seconds": [], "hubness_seconds": [], "bridge_seconds": [], "total_seconds": []}
+    batch_sizes, failed, msts = [], 0, 0
+    for kind, rid in ids:
+        st = sess.wait(rid)
+        if st["state"] != chip.REQ_DONE:
+            failed += 1
+            continue
+        lat[kind].append(st["latency_ms"])
+        run_ms[kind].append(st["run_ms"])
+        if kind == "insert":
+            for key in phases:
+                phases[key].append(st["insert_stats"][key])
+        if kind == "extract":
+            batch_sizes.append(st["batch_requests"])
+            msts += st["num_msts"]
+        sess.release(rid)
+    wall = time.perf_counter() - start
+    info = sess.graph_info()
+    final_n, dev_bytes = info["n"], info["device_bytes"]
+    sess.close()
+    return dict(
+        workload=f"serving: seed {seed_n} (d=24) + {batches} log-uniform insert batches "
+                 f"[1k,50k] + Poisson extract queries (1..8 mpts from 2..32)",
+        seed_build_ms=seed_build_s * 1e3, inserted_points=int(sizes.sum()), final_points=final_n,
+        inserts=len(lat["insert"]), extracts=len(lat["extract"]), failed=failed,
+        insert_rate_per_s=insert_rate, extract_rate_per_s=extract_rate, wall_s=wall,
+        insert_p50_ms=percentile(lat["insert"], 50), insert_p99_ms=percentile(lat["insert"], 99),
+        extract_p50_ms=percentile(lat["extract"], 50),
+        extract_p99_ms=percentile(lat["extract"], 99),
+        extract_msts=msts, extracts_per_sweep_mean=float(np.mean(batch_sizes)) if batch_sizes else None,
+        insert_run_ms_mean=float(np.mean(run_ms["insert"])) if run_ms["insert"] else None,
+        extract_run_ms_mean=float(np.mean(run_ms["extract"])) if run_ms["extract"] else None,
+        insert_phase_ms_mean={k.replace("_seconds", ""): float(np.mean(v)) * 1e3
+                              for k, v in phases.items() if v},
+        hbm_budget_bytes=budget, graph_device_bytes=dev_bytes)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +188,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--recall-queries", type=int, default=300)
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the incremental serving scenario (seed 500k + 64 insert batches "
+                         "+ Poisson extract queries)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the run, copy 1 GiB device-to-device three times (a plain "
                          "copy kernel of known size, the scale for HBM byte counters)")
@@ -224,6 +306,15 @@ def main():
 
     recall, ratio = g.sampled_recall(args.recall_queries, 42)
 
+    serving = None
+    if not args.no_serving:
+        g.close()
+        serving = serving_run(chip, datagen, k, seed_n=500_000, batches=64, insert_rate=16.0,
+                              extract_rate=64.0, budget=64 << 30)
+        print(f"[bench] serving: insert p50/p99 {serving['insert_p50_ms']:.1f}/"
+              f"{serving['insert_p99_ms']:.1f} ms, extract p50/p99 {serving['extract_p50_ms']:.1f}/"
+              f"{serving['extract_p99_ms']:.1f} ms", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -270,6 +361,7 @@ def main():
         "roofline": roofline,
         "rooflines": rooflines,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out))
     if args.copy_probe:

@@ -497,6 +497,16 @@ class Session:
     def release(self, rid: int) -> None:
         check(lib().coresg_hip_session_release(self.h, rid))
 
+    def graph_info(self) -> dict:
+        """Points and device bytes of the served graph (call once the queue
+        has drained)."""
+        gh = lib().coresg_hip_session_graph(self.h)
+        if not gh:
+            raise CoresgHipError(last_error())
+        gh = C.c_void_p(gh)
+        return dict(n=int(lib().coresg_hip_graph_size(gh)),
+                    device_bytes=int(lib().coresg_hip_graph_device_bytes(gh)))
+
     def close(self):
         if getattr(self, "h", None):
             lib().coresg_hip_session_destroy(self.h)

@@ -119,6 +119,7 @@ struct Graph {
     DBuf<float> out_w;
 
     DescentBuffers nnd;
+    DBuf<unsigned long long> ins_partial;  // insert: per-chunk best lists
 
     // Hubness / selection state (filled by the build).
     DBuf<std::int32_t> rdeg;

@@ -1225,6 +1225,12 @@ struct InsertArgs {
     unsigned long long* cand;  // proposal buffers of the existing points
     std::uint32_t* fill;
     std::uint32_t cap;
+    // column chunks: block b scans query block b / chunks against columns
+    // [c * chunk_cols, (c + 1) * chunk_cols) with c = b % chunks; with more
+    // than one chunk every block writes its KP best as keys to `partial`
+    // (per new point, per chunk) instead of the final list
+    std::uint32_t chunks, chunk_cols;
+    unsigned long long* partial;
 };
 
 // (d, id) lexicographic order on metric distances: the reference's brute
@@ -1267,7 +1273,11 @@ __global__ void __launch_bounds__(kInsThreads) insert_scan_kernel(InsertArgs a) 
     float* s_ld = lds;                                   // merge phase: 2 * kInsRows * KP
     std::uint32_t* s_li = reinterpret_cast<std::uint32_t*>(lds + 2 * kInsRows * KP);
     const std::uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-    for (std::uint32_t rb = a.n0 + blockIdx.x * kInsRows; rb < a.n1; rb += gridDim.x * kInsRows) {
+    const std::uint32_t chunk = blockIdx.x % a.chunks;
+    const std::uint32_t col0 = chunk * a.chunk_cols;
+    const std::uint32_t col1 = min(a.n1, col0 + a.chunk_cols);
+    const std::uint32_t qstep = (gridDim.x / a.chunks) * kInsRows;
+    for (std::uint32_t rb = a.n0 + (blockIdx.x / a.chunks) * kInsRows; rb < a.n1; rb += qstep) {
         const std::uint32_t p = rb + lane;
         const bool has = p < a.n1;
         float row[STRIDE];
@@ -1290,9 +1300,9 @@ __global__ void __launch_bounds__(kInsThreads) insert_scan_kernel(InsertArgs a) 
         // squared prefilter for td[KP-1], loosened by 2^-20 so that no pair
         // whose rounded sqrt ties or beats it is skipped
         float thr2 = __builtin_inff();
-        for (std::uint32_t c0 = 0; c0 < a.n1; c0 += kInsTile) {
+        for (std::uint32_t c0 = col0; c0 < col1; c0 += kInsTile) {
             __syncthreads();  // the previous tile is consumed
-            const std::uint32_t cend = min(kInsTile, a.n1 - c0);
+            const std::uint32_t cend = min(kInsTile, col1 - c0);
             for (std::uint32_t e = threadIdx.x; e < cend * (STRIDE / 4); e += kInsThreads) {
                 const std::uint32_t c = e / (STRIDE / 4), f = e - c * (STRIDE / 4);
                 reinterpret_cast<float4*>(s_tile)[e] =
@@ -1364,7 +1374,15 @@ __global__ void __launch_bounds__(kInsThreads) insert_scan_kernel(InsertArgs a) 
                 if (id == kNone || !key_less(d, id, td[KP - 1], ti[KP - 1])) break;
                 list_insert<KP>(td, ti, d, id);
             }
-            if (has) {
+            if (has && a.partial) {
+                unsigned long long* out =
+                    a.partial + (std::size_t(p - a.n0) * a.chunks + chunk) * KP;
+#pragma unroll
+                for (std::uint32_t i = 0; i < KP; ++i)
+                    out[i] = ti[i] == kNone
+                                 ? ~0ull
+                                 : (static_cast<unsigned long long>(fbits(td[i])) << 32) | ti[i];
+            } else if (has) {
                 std::uint32_t c = 0;
 #pragma unroll
                 for (std::uint32_t i = 0; i < KP; ++i) {
@@ -1379,6 +1397,38 @@ __global__ void __launch_bounds__(kInsThreads) insert_scan_kernel(InsertArgs a) 
                 a.count[p] = c;
             }
         }
+    }
+}
+
+// The k best of a new point's per-chunk lists: its chunks * KP keys
+// (distance bits << 32 | id, ~0 for empty) sorted in registers; keys order as
+// (distance, id), the brute force's order, and ids of different chunks are
+// distinct. One wavefront per new point.
+template <int R>
+__global__ void __launch_bounds__(64)
+    insert_merge_kernel(const unsigned long long* __restrict__ partial, std::uint32_t n0,
+                        std::uint32_t n1, std::uint32_t k, std::uint32_t keys,
+                        std::uint32_t* __restrict__ ids, float* __restrict__ dist,
+                        std::uint8_t* __restrict__ flags, std::uint32_t* __restrict__ count) {
+    __shared__ unsigned long long s_key[64 * R];
+    __shared__ unsigned long long s_top[kMaxK];
+    const std::uint32_t lane = threadIdx.x;
+    for (std::uint32_t p = n0 + blockIdx.x; p < n1; p += gridDim.x) {
+        const unsigned long long* in = partial + std::size_t(p - n0) * keys;
+        for (std::uint32_t i = lane; i < 64u * R; i += 64) s_key[i] = i < keys ? in[i] : ~0ull;
+        __syncthreads();
+        const std::uint32_t got = sort_unique_top<R>(s_key, 64u * R, k, s_top);
+        __syncthreads();
+        const std::uint32_t t = got < k ? got : k;
+        if (lane < t) {
+            const unsigned long long key = s_top[lane];
+            const std::size_t o = std::size_t(p) * k + lane;
+            ids[o] = static_cast<std::uint32_t>(key);
+            dist[o] = __uint_as_float(static_cast<std::uint32_t>(key >> 32));
+            flags[o] = 1;
+        }
+        if (lane == 0) count[p] = t;
+        __syncthreads();
     }
 }
 
@@ -1547,8 +1597,21 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
         b.partial.reserve(kMergeBlocks);
         b.updates.reserve(1);
         fill_u32(b.fill.p, n0 ? n0 : 1, 0u, s);
+        // Column chunks so that small batches still fill the chip: about 2048
+        // blocks, at most 16 chunks (16 * KP <= 512 keys per point to merge)
+        const std::uint32_t qblocks = (count + kInsRows - 1) / kInsRows;
+        std::uint32_t chunks = (2048u + qblocks - 1) / qblocks;
+        chunks = std::max(1u, std::min({chunks, 16u, 512u / kp, (n1 + kInsTile - 1) / kInsTile}));
+        const std::uint32_t chunk_cols =
+            ((n1 + chunks - 1) / chunks + kInsTile - 1) / kInsTile * kInsTile;
+        chunks = (n1 + chunk_cols - 1) / chunk_cols;
+        unsigned long long* partial = nullptr;
+        if (chunks > 1) {
+            g.ins_partial.reserve(std::size_t(count) * chunks * kp);
+            partial = g.ins_partial.p;
+        }
         InsertArgs ia{g.pts.p, n0, n1, k, g.ids.p, g.dist.p, g.flags.p, g.count.p,
-                      b.cand.p, b.fill.p, cap};
+                      b.cand.p, b.fill.p, cap, chunks, chunk_cols, partial};
         const std::size_t lds_tile = (std::size_t(kInsTile) * stride + kInsTile) * sizeof(float);
         const std::size_t lds_merge = std::size_t(4) * kInsRows * kp * sizeof(float);
         const std::size_t lds = std::max(lds_tile, lds_merge);
@@ -1556,12 +1619,29 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
             CSG_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                           hipFuncAttributeMaxDynamicSharedMemorySize,
                                           static_cast<int>(lds)));
-        const unsigned blocks =
-            static_cast<unsigned>(std::min<std::size_t>((count + kInsRows - 1) / kInsRows, 65535));
+        const unsigned blocks = static_cast<unsigned>(
+            std::min<std::size_t>(qblocks, 65535 / chunks) * chunks);
         {
             ScopedKernelTimer t(g, "insert_scan");
             hipLaunchKernelGGL(fn, dim3(blocks), dim3(kInsThreads), lds, s, ia);
             CSG_LAUNCH_CHECK();
+            if (partial) {
+                const std::uint32_t keys = chunks * kp;
+                const unsigned mblocks = static_cast<unsigned>(std::min<std::size_t>(count, 65536));
+                if (keys <= 64)
+                    insert_merge_kernel<1><<<mblocks, 64, 0, s>>>(partial, n0, n1, k, keys, g.ids.p,
+                                                                  g.dist.p, g.flags.p, g.count.p);
+                else if (keys <= 128)
+                    insert_merge_kernel<2><<<mblocks, 64, 0, s>>>(partial, n0, n1, k, keys, g.ids.p,
+                                                                  g.dist.p, g.flags.p, g.count.p);
+                else if (keys <= 256)
+                    insert_merge_kernel<4><<<mblocks, 64, 0, s>>>(partial, n0, n1, k, keys, g.ids.p,
+                                                                  g.dist.p, g.flags.p, g.count.p);
+                else
+                    insert_merge_kernel<8><<<mblocks, 64, 0, s>>>(partial, n0, n1, k, keys, g.ids.p,
+                                                                  g.dist.p, g.flags.p, g.count.p);
+                CSG_LAUNCH_CHECK();
+            }
         }
         if (n0) {
             const unsigned merge_blocks = static_cast<unsigned>(std::min<std::size_t>(n0, kMergeBlocks));
