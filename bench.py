@@ -309,8 +309,8 @@ def main():
     serving = None
     if not args.no_serving:
         g.close()
-        serving = serving_run(chip, datagen, k, seed_n=500_000, batches=64, insert_rate=16.0,
-                              extract_rate=64.0, budget=64 << 30)
+        serving = serving_run(chip, datagen, k, seed_n=500_000, batches=64, insert_rate=4.0,
+                              extract_rate=16.0, budget=64 << 30)
         print(f"[bench] serving: insert p50/p99 {serving['insert_p50_ms']:.1f}/"
               f"{serving['insert_p99_ms']:.1f} ms, extract p50/p99 {serving['extract_p50_ms']:.1f}/"
               f"{serving['extract_p99_ms']:.1f} ms", file=sys.stderr, flush=True)

@@ -308,7 +308,8 @@ int coresg_hip_graph_kernel_times(coresg_hip_graph* g, const char* kernel, doubl
  * "nnd_gram" (Gram-filter joins): {pairs visited, pairs computed exactly,
  * the largest relative error of a fast distance against FP64 (float bits),
  * the largest ratio of the bound's lower distance to the FP64 distance
- * (float bits; at most 1 when the bound holds)}. */
+ * (float bits; at most 1 when the bound holds)}; "insert_gram" (Gram-filter
+ * insert scans): {pairs visited, pairs computed exactly, 0, 0}. */
 int coresg_hip_graph_kernel_work(coresg_hip_graph* g, const char* kernel, uint64_t* work4);
 int coresg_hip_graph_reset_timers(coresg_hip_graph* g);
 /* While profiling, also audit the Gram-filter join: an FP64 distance for

@@ -1231,6 +1231,7 @@ struct InsertArgs {
     // (per new point, per chunk) instead of the final list
     std::uint32_t chunks, chunk_cols;
     unsigned long long* partial;
+    unsigned long long* tally;  // Gram-filter scan, profiling: {pairs, exact}
 };
 
 // (d, id) lexicographic order on metric distances: the reference's brute
@@ -1400,6 +1401,263 @@ __global__ void __launch_bounds__(kInsThreads) insert_scan_kernel(InsertArgs a) 
     }
 }
 
+// Write a new point's KP best (registers td/ti) as its final list or, with
+// column chunks, as its chunk's keys.
+template <std::uint32_t KP>
+__device__ __forceinline__ void insert_write_list(const InsertArgs& a, std::uint32_t p,
+                                                  std::uint32_t chunk, const float (&td)[KP],
+                                                  const std::uint32_t (&ti)[KP]) {
+    if (a.partial) {
+        unsigned long long* out = a.partial + (std::size_t(p - a.n0) * a.chunks + chunk) * KP;
+#pragma unroll
+        for (std::uint32_t i = 0; i < KP; ++i)
+            out[i] = ti[i] == kNone ? ~0ull
+                                    : (static_cast<unsigned long long>(fbits(td[i])) << 32) | ti[i];
+    } else {
+        std::uint32_t c = 0;
+#pragma unroll
+        for (std::uint32_t i = 0; i < KP; ++i) {
+            if (i < a.k && ti[i] != kNone) {
+                const std::size_t o = std::size_t(p) * a.k + i;
+                a.ids[o] = ti[i];
+                a.dist[o] = td[i];
+                a.flags[o] = 1;
+                c = i + 1;
+            }
+        }
+        a.count[p] = c;
+    }
+}
+
+// Gram-filter insert scan: the exact scan's results with the matrix cores
+// deciding which exact distances are needed. Per 64-column tile, the
+// 64 x 64 Gram of the (centred, FP16/BF16-rounded) new points and columns runs
+// on 16x16x16 MFMA (wavefront w: columns 16w..16w+15, all 64 rows), giving
+// fast distances D~ = N_q + N_c - 2G in LDS. A lane (one new point) then takes
+// the rigorous lower bound of the exact distance (the Gram-filter join's bound)
+// for each of its columns and computes the exact distance only when the bound
+// does not rule out both uses: entering the point's list (bound <= its
+// running threshold) and being offered to the column's list (bound below the
+// column's k-th). Rows are centred on the block's first new point.
+constexpr unsigned kGramTile = 64;
+
+template <std::uint32_t STRIDE, std::uint32_t KP, bool BF16>
+__global__ void __launch_bounds__(kInsThreads) insert_scan_gram_kernel(InsertArgs a) {
+    extern __shared__ float lds[];
+    constexpr std::uint32_t KP16 = (STRIDE + 15u) & ~15u, KS = KP16 / 16u;
+    constexpr std::uint32_t GP = kGramTile + 1;              // D~ tile pitch
+    float* s_tile = lds;                                      // kGramTile * STRIDE
+    float* s_thr = s_tile + kGramTile * STRIDE;               // kGramTile
+    float* s_N = s_thr + kGramTile;                           // kGramTile
+    float* s_nb = s_N + kGramTile;                            // kGramTile
+    float* s_q = s_nb + kGramTile;                            // kInsRows * STRIDE (queries)
+    float* s_Nq = s_q + kInsRows * STRIDE;                    // kInsRows
+    float* s_G = s_Nq + kInsRows;                             // kInsRows * GP
+    float* s_ld = lds;                                        // merge phase: 2 * kInsRows * KP
+    std::uint32_t* s_li = reinterpret_cast<std::uint32_t*>(lds + 2 * kInsRows * KP);
+    const std::uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const std::uint32_t li = lane & 15u, lk = 4u * (lane >> 4);
+    const float rho = GramConst<BF16>::rho;
+    const float abs_norm = GramConst<BF16>::absc * sqrtf(float(KP16)) * 1.01f;
+    const float cG = 2.0f * float(KP16 + 8) * 0x1p-24f;
+    const float eta = float(STRIDE + 24) * 0x1p-24f + 0x1p-18f;
+    const float nscale = 1.0f + float(KP16 + 4) * 0x1p-24f;
+    const std::uint32_t chunk = blockIdx.x % a.chunks;
+    const std::uint32_t col0 = chunk * a.chunk_cols;
+    const std::uint32_t col1 = min(a.n1, col0 + a.chunk_cols);
+    const std::uint32_t qstep = (gridDim.x / a.chunks) * kInsRows;
+    unsigned long long n_pairs = 0, n_exact = 0;
+    for (std::uint32_t rb = a.n0 + (blockIdx.x / a.chunks) * kInsRows; rb < a.n1; rb += qstep) {
+        const std::uint32_t p = rb + lane;
+        const bool has = p < a.n1;
+        __syncthreads();  // LDS of the previous query block is consumed
+        for (std::uint32_t e = threadIdx.x; e < kInsRows * (STRIDE / 4); e += kInsThreads) {
+            const std::uint32_t r = e / (STRIDE / 4), f = e - r * (STRIDE / 4);
+            reinterpret_cast<float4*>(s_q)[e] =
+                rb + r < a.n1 ? reinterpret_cast<const float4*>(a.pts + std::size_t(rb + r) * STRIDE)[f]
+                              : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        __syncthreads();
+        float row[STRIDE];
+#pragma unroll
+        for (std::uint32_t f = 0; f < STRIDE / 4; ++f) {
+            const float4 t = reinterpret_cast<const float4*>(s_q + lane * STRIDE)[f];
+            row[4 * f] = t.x;
+            row[4 * f + 1] = t.y;
+            row[4 * f + 2] = t.z;
+            row[4 * f + 3] = t.w;
+        }
+        // the query's rounded centred norm (centre: the block's first row)
+        float Nq = 0.0f;
+#pragma unroll
+        for (std::uint32_t f = 0; f < STRIDE; ++f) {
+            const float h = gram_round<BF16>(row[f] - s_q[f]);
+            Nq = __builtin_fmaf(h, h, Nq);
+        }
+        if (w == 0) s_Nq[lane] = Nq;
+        const float nbq = (sqrtf(Nq) * nscale + abs_norm) * (1.0f + 2.0f * rho);
+        float td[KP];
+        std::uint32_t ti[KP];
+#pragma unroll
+        for (std::uint32_t i = 0; i < KP; ++i) {
+            td[i] = __builtin_inff();
+            ti[i] = kNone;
+        }
+        float thr2 = __builtin_inff();
+        for (std::uint32_t c0 = col0; c0 < col1; c0 += kGramTile) {
+            __syncthreads();  // the previous tile is consumed
+            const std::uint32_t cend = min(kGramTile, col1 - c0);
+            for (std::uint32_t e = threadIdx.x; e < kGramTile * (STRIDE / 4); e += kInsThreads) {
+                const std::uint32_t c = e / (STRIDE / 4), f = e - c * (STRIDE / 4);
+                reinterpret_cast<float4*>(s_tile)[e] =
+                    c < cend ? reinterpret_cast<const float4*>(a.pts + std::size_t(c0 + c) * STRIDE)[f]
+                             : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            for (std::uint32_t c = threadIdx.x; c < cend; c += kInsThreads) {
+                const std::uint32_t j = c0 + c;
+                float t = -1.0f;  // new points take no proposals here
+                if (j < a.n0) {
+                    t = __builtin_inff();
+                    if (a.count[j] >= a.k) {
+                        const float dk = a.dist[std::size_t(j) * a.k + a.k - 1];
+                        t = dk * dk * (1.0f + 0x1p-20f);  // the merge makes the exact test
+                    }
+                }
+                s_thr[c] = t;
+            }
+            __syncthreads();
+            if (threadIdx.x < kGramTile) {
+                const std::uint32_t c = threadIdx.x;
+                float N = 0.0f;
+#pragma unroll
+                for (std::uint32_t f = 0; f < STRIDE; ++f) {
+                    const float h = gram_round<BF16>(s_tile[c * STRIDE + f] - s_q[f]);
+                    N = __builtin_fmaf(h, h, N);
+                }
+                s_N[c] = N;
+                s_nb[c] = (sqrtf(N) * nscale + abs_norm) * (1.0f + 2.0f * rho);
+            }
+            // Gram strip of wavefront w: columns 16w + li, all 64 rows
+            float bfrag[KS][4];
+#pragma unroll
+            for (std::uint32_t ks = 0; ks < KS; ++ks) {
+                const std::uint32_t k0 = 16u * ks + lk;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const std::uint32_t kk = k0 + static_cast<std::uint32_t>(q);
+                    bfrag[ks][q] = kk < STRIDE ? s_tile[(16u * w + li) * STRIDE + kk] - s_q[kk] : 0.0f;
+                }
+            }
+            __syncthreads();  // s_N complete
+#pragma unroll
+            for (std::uint32_t rt = 0; rt < kInsRows / 16u; ++rt) {
+                gram_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (std::uint32_t ks = 0; ks < KS; ++ks) {
+                    float afrag[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const std::uint32_t kk = 16u * ks + lk + static_cast<std::uint32_t>(q);
+                        afrag[q] = kk < STRIDE ? s_q[(16u * rt + li) * STRIDE + kk] - s_q[kk] : 0.0f;
+                    }
+                    acc = gram_mfma<BF16>(afrag, bfrag[ks], acc);
+                }
+                const std::uint32_t col = 16u * w + li;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const std::uint32_t r = 16u * rt + lk + static_cast<std::uint32_t>(q);
+                    s_G[r * GP + col] = (s_Nq[r] + s_N[col]) - 2.0f * acc[q];
+                }
+            }
+            __syncthreads();
+            for (std::uint32_t c = w; c < cend; c += 4) {
+                const std::uint32_t j = c0 + c;
+                if (!has || j == p) continue;
+                const float dt = s_G[lane * GP + c];
+                const float nbs = nbq + s_nb[c];
+                float tlo = 0.0f;
+                const float s2 = dt - cG * nbs * nbs;
+                if (s2 > 0.0f) {
+                    const float slo = sqrtf(s2) * (1.0f - 0x1p-20f);
+                    const float delta =
+                        (rho * nbs + 2.0f * abs_norm) * (1.0f + 0x1p-16f) + slo * 0x1p-21f;
+                    tlo = slo - delta;
+                }
+                const float thr_c = s_thr[c];
+                if (tlo > 0.0f) {
+                    const float rlo = tlo * tlo * (1.0f - eta);
+                    if (rlo > thr2 && !(rlo < thr_c)) {
+                        if (a.tally) ++n_pairs;
+                        continue;
+                    }
+                }
+                if (a.tally) {
+                    ++n_pairs;
+                    ++n_exact;
+                }
+                const float d2 = l2sqr_reg<STRIDE>(row, s_tile + c * STRIDE);
+                if (d2 <= thr2) {
+                    const float d = sqrt_exact(d2);
+                    if (key_less(d, j, td[KP - 1], ti[KP - 1])) {
+                        list_insert<KP>(td, ti, d, j);
+                        thr2 = td[KP - 1] * td[KP - 1] * (1.0f + 0x1p-20f);
+                    }
+                }
+                if (d2 < thr_c) {
+                    const std::uint32_t pos = atomicAdd(&a.fill[j], 1u);
+                    if (pos < a.cap)
+                        a.cand[std::size_t(j) * a.cap + pos] =
+                            (static_cast<unsigned long long>(fbits(sqrt_exact(d2))) << 32) | p;
+                }
+            }
+        }
+        // merge the four partial lists of each new point
+        __syncthreads();
+        if (w >= 2) {
+#pragma unroll
+            for (std::uint32_t i = 0; i < KP; ++i) {
+                s_ld[((w - 2) * kInsRows + lane) * KP + i] = td[i];
+                s_li[((w - 2) * kInsRows + lane) * KP + i] = ti[i];
+            }
+        }
+        __syncthreads();
+        if (w < 2) {
+            for (std::uint32_t i = 0; i < KP; ++i) {
+                const float d = s_ld[(w * kInsRows + lane) * KP + i];
+                const std::uint32_t id = s_li[(w * kInsRows + lane) * KP + i];
+                if (id == kNone || !key_less(d, id, td[KP - 1], ti[KP - 1])) break;
+                list_insert<KP>(td, ti, d, id);
+            }
+        }
+        __syncthreads();
+        if (w == 1) {
+#pragma unroll
+            for (std::uint32_t i = 0; i < KP; ++i) {
+                s_ld[lane * KP + i] = td[i];
+                s_li[lane * KP + i] = ti[i];
+            }
+        }
+        __syncthreads();
+        if (w == 0) {
+            for (std::uint32_t i = 0; i < KP; ++i) {
+                const float d = s_ld[lane * KP + i];
+                const std::uint32_t id = s_li[lane * KP + i];
+                if (id == kNone || !key_less(d, id, td[KP - 1], ti[KP - 1])) break;
+                list_insert<KP>(td, ti, d, id);
+            }
+            if (has) insert_write_list<KP>(a, p, chunk, td, ti);
+        }
+    }
+    if (a.tally) {
+        n_pairs = wave_sum(n_pairs);
+        n_exact = wave_sum(n_exact);
+        if (lane == 0) {
+            atomicAdd(&a.tally[0], n_pairs);
+            atomicAdd(&a.tally[1], n_exact);
+        }
+    }
+}
+
 // The k best of a new point's per-chunk lists: its chunks * KP keys
 // (distance bits << 32 | id, ~0 for empty) sorted in registers; keys order as
 // (distance, id), the brute force's order, and ids of different chunks are
@@ -1460,6 +1718,28 @@ InsertFn pick_insert(std::uint32_t stride, std::uint32_t k, std::uint32_t* kp) {
     }
     *kp = 64;
     return pick_insert_stride<64>(stride);
+}
+
+template <std::uint32_t KP, bool BF16>
+InsertFn pick_insert_gram_stride(std::uint32_t stride) {
+    switch (stride) {
+        case 8: return insert_scan_gram_kernel<8, KP, BF16>;
+        case 16: return insert_scan_gram_kernel<16, KP, BF16>;
+        case 24: return insert_scan_gram_kernel<24, KP, BF16>;
+        case 32: return insert_scan_gram_kernel<32, KP, BF16>;
+        case 40: return insert_scan_gram_kernel<40, KP, BF16>;
+        case 48: return insert_scan_gram_kernel<48, KP, BF16>;
+        case 56: return insert_scan_gram_kernel<56, KP, BF16>;
+        case 64: return insert_scan_gram_kernel<64, KP, BF16>;
+        default: return nullptr;
+    }
+}
+
+template <bool BF16>
+InsertFn pick_insert_gram(std::uint32_t stride, std::uint32_t k) {
+    if (k <= 16) return pick_insert_gram_stride<16, BF16>(stride);
+    if (k <= 32) return pick_insert_gram_stride<32, BF16>(stride);
+    return pick_insert_gram_stride<64, BF16>(stride);
 }
 
 void check_join_capacity(std::uint32_t cap) {
@@ -1550,7 +1830,17 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
     if (n1_64 >= 0x7FFFFFFFull) throw std::runtime_error("insert: point count exceeds 2^31");
     const std::uint32_t n1 = static_cast<std::uint32_t>(n1_64);
     std::uint32_t kp = 0;
-    const InsertFn fn = pick_insert(stride, k, &kp);
+    InsertFn fn = pick_insert(stride, k, &kp);
+    // the batch scan: exact (the library's choice), or the Gram filter: the
+    // same lists with exact distances for only 1-3 % of the pairs, but at the
+    // scan's occupancy (2 waves per SIMD: 32-entry register lists per point)
+    // the per-tile Gram staging and barriers cost more than the distances
+    // (MI355X, 20k points into 500k, d = 24: 92 ms exact, 117 ms FP16 filter)
+    const std::uint32_t gmode = p.gram == kGramAuto ? kGramExact : p.gram;
+    if (gmode > kGramBF16) throw std::runtime_error("insert: gram must be 0..3");
+    if (fn && gmode == kGramF16) fn = pick_insert_gram<false>(stride, k);
+    if (fn && gmode == kGramBF16) fn = pick_insert_gram<true>(stride, k);
+    const bool gram_scan = gmode != kGramExact;
     const bool brute = p.brute_force_limit && n1 <= p.brute_force_limit;
     if (!brute && !fn)
         throw std::runtime_error("insert: rows wider than 64 floats are not supported on device");
@@ -1610,9 +1900,19 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
             g.ins_partial.reserve(std::size_t(count) * chunks * kp);
             partial = g.ins_partial.p;
         }
+        unsigned long long* tally = nullptr;
+        if (gram_scan && g.profiling) {
+            b.gram_tally.reserve(2);
+            fill_u64(reinterpret_cast<std::uint64_t*>(b.gram_tally.p), 2, 0ull, s);
+            tally = b.gram_tally.p;
+        }
         InsertArgs ia{g.pts.p, n0, n1, k, g.ids.p, g.dist.p, g.flags.p, g.count.p,
-                      b.cand.p, b.fill.p, cap, chunks, chunk_cols, partial};
-        const std::size_t lds_tile = (std::size_t(kInsTile) * stride + kInsTile) * sizeof(float);
+                      b.cand.p, b.fill.p, cap, chunks, chunk_cols, partial, tally};
+        const std::size_t lds_tile =
+            gram_scan ? (std::size_t(kGramTile) * stride + 3 * kGramTile + std::size_t(kInsRows) * stride +
+                         kInsRows + std::size_t(kInsRows) * (kGramTile + 1)) *
+                            sizeof(float)
+                      : (std::size_t(kInsTile) * stride + kInsTile) * sizeof(float);
         const std::size_t lds_merge = std::size_t(4) * kInsRows * kp * sizeof(float);
         const std::size_t lds = std::max(lds_tile, lds_merge);
         if (lds > 64 * 1024)
@@ -1655,6 +1955,12 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
             CSG_CHECK(hipMemcpyAsync(&upd, b.updates.p, 8, hipMemcpyDeviceToHost, s));
             CSG_CHECK(hipStreamSynchronize(s));
             applied = upd;
+        }
+        if (tally) {
+            unsigned long long gt[2] = {0, 0};
+            CSG_CHECK(hipMemcpyAsync(gt, tally, sizeof(gt), hipMemcpyDeviceToHost, s));
+            CSG_CHECK(hipStreamSynchronize(s));
+            g.add_work("insert_gram", {gt[0], gt[1], 0, 0});
         }
     }
     CSG_CHECK(hipStreamSynchronize(s));

@@ -94,9 +94,12 @@ def check_downstream(g, x, k):
         assert np.float64(rr["weight"]).view(np.uint64) == np.float64(o["weight"]).view(np.uint64)
 
 
+@pytest.mark.parametrize("gram", [1, 2, 3])
 @pytest.mark.parametrize("name", ["mix_d16", "mix_d24", "ties"])
-def test_insert_lists_exact(name):
-    print(f"[gpu] insert exactness: {name}", flush=True)
+def test_insert_lists_exact(name, gram):
+    """gram 1: the exact scan; 2 / 3: the FP16 / BF16 Gram-filter scan, which
+    must give the same lists."""
+    print(f"[gpu] insert exactness: {name} gram={gram}", flush=True)
     gd = golden(name)
     x = gd["data"]
     k = int(gd["k"])
@@ -107,7 +110,7 @@ def test_insert_lists_exact(name):
     g.set_points(x[:n0])
     g.build(chip.build_params())
     ids0, dist0, flags0, count0 = g.get_knn()
-    st = g.insert(x[n0:])
+    st = g.insert(x[n0:], chip.build_params(gram=gram))
     assert g.n == n
     ids, dist, flags, count = g.get_knn()
     eids, edist, eflags, ecount = expected_lists(xp, n0, k, ids0, dist0, flags0, count0)
