@@ -178,6 +178,54 @@ def serving_run(chip, datagen, k, seed_n, batches, insert_rate, extract_rate, bu
         hbm_budget_bytes=budget, graph_device_bytes=dev_bytes)
 
 
+def mixed_precision_run(chip, datagen, n=2_000_000, d=128, k=32, mpts=(4, 16, 32)):
+    """The mixed-precision distance path: a 64-component anisotropic mixture
+    (eigenvalues log-uniform in [0.01, 4]), built with the exact FP32 join and
+    with the BF16 Gram filter (matrix cores, every retained edge recomputed in
+    exact FP32); the BF16 build's fast distances are audited against FP64 on
+    every visited pair, and mpts = 4, 16, 32 are extracted from each graph."""
+    import struct
+    f32 = lambda b: struct.unpack("f", struct.pack("I", b & 0xFFFFFFFF))[0]
+    t0 = time.perf_counter()
+    x = datagen.anisotropic_mixture(n, d, 64, 0.01, 4.0, seed=4)
+    gen_s = time.perf_counter() - t0
+    g = chip.Graph(n, d, k)
+    g.set_points(x)
+    del x
+    out = dict(workload=f"anisotropic-mixture-{n // 1_000_000}m-d{d}", n=n, d=d, k_max=k,
+               data_gen_s=gen_s)
+    for name, gram in (("exact_fp32", 1), ("bf16_gram_filter", 3)):
+        p = chip.build_params(sample=16, iters=10, gram=gram)
+        g.build(p)  # warm
+        t0 = time.perf_counter()
+        st = g.build(p)
+        build_s = time.perf_counter() - t0
+        g.prepare_edges()
+        t1 = time.perf_counter()
+        res = g.extract(list(mpts))
+        sweep_s = time.perf_counter() - t1
+        rec, _ = g.sampled_recall(300, 42)
+        out[name] = dict(build_ms=build_s * 1e3, pts_per_s=n / build_s, sweep_ms=sweep_s * 1e3,
+                         recall_sampled=rec, rounds=st["rounds"],
+                         mst_weights={int(m): r["weight"] for m, r in zip(mpts, res)})
+        print(f"[bench] mixed precision {name}: build {build_s * 1e3:.1f} ms, "
+              f"recall {rec:.4f}", file=sys.stderr, flush=True)
+    # audit of the BF16 fast distances (FP64 per visited pair; slow, untimed)
+    g.reset_timers()
+    g.set_profiling(True)
+    g.set_audit(True)
+    g.build(chip.build_params(sample=16, iters=10, gram=3))
+    g.set_profiling(False)
+    g.set_audit(False)
+    w = g.kernel_work("nnd_gram")
+    out["bf16_audit"] = dict(pairs=w[0], exact_recomputed=w[1],
+                             exact_fraction=w[1] / max(1, w[0]),
+                             max_rel_distance_error_vs_fp64=f32(w[2]),
+                             max_bound_ratio=f32(w[3]))
+    g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,6 +236,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--recall-queries", type=int, default=300)
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision measurement (n=2M, d=128, BF16 Gram filter)")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the incremental serving scenario (seed 500k + 64 insert batches "
                          "+ Poisson extract queries)")
@@ -315,6 +365,10 @@ def main():
               f"{serving['insert_p99_ms']:.1f} ms, extract p50/p99 {serving['extract_p50_ms']:.1f}/"
               f"{serving['extract_p99_ms']:.1f} ms", file=sys.stderr, flush=True)
 
+    mixed = None
+    if not args.no_mixed:
+        mixed = mixed_precision_run(chip, datagen)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -362,6 +416,7 @@ def main():
         "rooflines": rooflines,
         "cpu_baseline": cpu,
         "serving": serving,
+        "mixed_precision": mixed,
     }
     print(json.dumps(out))
     if args.copy_probe:
