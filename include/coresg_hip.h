@@ -292,6 +292,26 @@ coresg_hip_graph* coresg_hip_session_graph(coresg_hip_session* s);
 /* Finish the queued requests, stop the worker, free the graph. */
 void coresg_hip_session_destroy(coresg_hip_session* s);
 
+/* ---- 4. batched independent instances ------------------------------------ */
+
+/* Many independent point sets in one graph (the batched configuration of
+ * uneven instances): instance i holds points [offsets[i], offsets[i+1]),
+ * offsets[0] = 0 and offsets[count] = the graph's point count. Every CoreSG
+ * step then runs per instance, as if each were built alone (CoreSG::build,
+ * coresg.cpp:260-323, per instance): init_random draws inside the instance
+ * with the instance-local node index (bit-exact against a separate build),
+ * the descent's candidates never leave an instance, the anti-hub budget,
+ * selection, bridges and repair are per instance; the descent's early stop is
+ * taken over all instances together. One sweep over the shared edge list
+ * gives every instance's MSTs. Call before building; count = 0 clears. */
+int coresg_hip_graph_set_instances(coresg_hip_graph* g, const uint32_t* offsets, uint32_t count);
+/* CoreSG::mst per instance and mpts value (coresg.cpp:408-417): info[i * count
+ * + q] for instance i and mpts[q] (edges, components, exact weight, rounds).
+ * The union's edges and labels (coresg_hip_extract) hold every instance's
+ * MST edges in global ids, grouped by instance. */
+int coresg_hip_extract_instances(coresg_hip_graph* g, const uint32_t* mpts, uint32_t count,
+                                 coresg_hip_mst_info* info);
+
 /* Sampled k-NN recall on device (eval.cpp:24-97; queries chosen on the
  * host by the same mt19937_64 draw). */
 int coresg_hip_sampled_recall(coresg_hip_graph* g, size_t queries, uint64_t seed,

@@ -127,6 +127,8 @@ SIGNATURES = [
     ("coresg_hip_graph_kernel_work", C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_uint64)]),
     ("coresg_hip_graph_reset_timers", C.c_int, [_vp]),
     ("coresg_hip_graph_set_audit", C.c_int, [_vp, C.c_int]),
+    ("coresg_hip_graph_set_instances", C.c_int, [_vp, _vp, C.c_uint32]),
+    ("coresg_hip_extract_instances", C.c_int, [_vp, _vp, C.c_uint32, _vp]),
     ("coresg_hip_insert", C.c_int, [_vp, C.c_void_p, C.c_uint32, C.c_int, C.c_int,
                                     C.POINTER(BuildParams), C.POINTER(BuildStats)]),
     ("coresg_hip_graph_size", C.c_uint32, [_vp]),
@@ -375,6 +377,23 @@ class Graph:
         w = (C.c_uint64 * 4)()
         check(lib().coresg_hip_graph_kernel_work(self.h, name.encode(), w))
         return [int(v) for v in w]
+
+    def set_instances(self, offsets):
+        """Independent instances: instance i holds points [offsets[i], offsets[i+1])."""
+        off = _u32(offsets)
+        self.instance_offsets = off
+        check(lib().coresg_hip_graph_set_instances(self.h, ptr(off), max(off.size - 1, 0)))
+
+    def extract_instances(self, mpts) -> list:
+        """Per-instance MST summaries: out[i][q] for instance i, mpts[q]."""
+        mp = _u32(list(mpts))
+        ninst = self.instance_offsets.size - 1
+        info = (MstInfo * (ninst * mp.size))()
+        check(lib().coresg_hip_extract_instances(self.h, ptr(mp), mp.size, info))
+        return [[dict(mpts=info[i * mp.size + q].mpts, num_edges=info[i * mp.size + q].num_edges,
+                      num_components=info[i * mp.size + q].num_components,
+                      rounds=info[i * mp.size + q].rounds, weight=info[i * mp.size + q].weight)
+                 for q in range(mp.size)] for i in range(ninst)]
 
     def set_audit(self, on: bool):
         check(lib().coresg_hip_graph_set_audit(self.h, int(bool(on))))

@@ -357,6 +357,45 @@ int coresg_hip_graph_kernel_work(coresg_hip_graph* h, const char* kernel, uint64
     });
 }
 
+int coresg_hip_graph_set_instances(coresg_hip_graph* h, const uint32_t* offsets,
+                                   uint32_t count) {
+    return guarded([&] {
+        if (!h) throw std::runtime_error("set_instances: null graph");
+        Graph& g = h->g;
+        if (count == 0) {
+            g.inst_off.clear();
+        } else {
+            if (!offsets) throw std::runtime_error("set_instances: null offsets");
+            if (offsets[0] != 0 || offsets[count] != g.n)
+                throw std::runtime_error("set_instances: offsets must run from 0 to the point count");
+            for (uint32_t i = 0; i < count; ++i)
+                if (offsets[i + 1] <= offsets[i])
+                    throw std::runtime_error("set_instances: every instance needs a point");
+            g.inst_off.assign(offsets, offsets + count + 1);
+            std::vector<uint32_t> of(g.n);
+            for (uint32_t i = 0; i < count; ++i)
+                for (uint32_t v = offsets[i]; v < offsets[i + 1]; ++v) of[v] = i;
+            g.inst_of.reserve(g.n);
+            g.inst_off_d.reserve(std::size_t(count) + 1);
+            CSG_CHECK(hipMemcpyAsync(g.inst_of.p, of.data(), std::size_t(g.n) * 4,
+                                     hipMemcpyHostToDevice, g.stream));
+            CSG_CHECK(hipMemcpyAsync(g.inst_off_d.p, offsets, (std::size_t(count) + 1) * 4,
+                                     hipMemcpyHostToDevice, g.stream));
+            CSG_CHECK(hipStreamSynchronize(g.stream));
+        }
+        g.edges_ready = false;
+        g.hub_ready = false;
+    });
+}
+
+int coresg_hip_extract_instances(coresg_hip_graph* h, const uint32_t* mpts, uint32_t count,
+                                 coresg_hip_mst_info* info) {
+    return guarded([&] {
+        if (!h || !mpts || !info) throw std::runtime_error("extract_instances: null argument");
+        extract_instances(h->g, mpts, count, info);
+    });
+}
+
 int coresg_hip_graph_set_audit(coresg_hip_graph* h, int on) {
     return guarded([&] {
         if (!h) throw std::runtime_error("set_audit: null graph");

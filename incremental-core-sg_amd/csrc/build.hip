@@ -75,39 +75,27 @@ struct HostUF {
     }
 };
 
-void repair_connectivity(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_stats* st) {
-    const std::uint32_t n = g.n, ncomp0 = g.num_components;
-    std::uint32_t ncomp = ncomp0;
-    if (st) st->components_before_repair = st->components_after_repair = ncomp0;
-    if (ncomp0 <= 1 || n == 0) return;  // the bridges cannot reduce one component
-
-    std::vector<std::uint32_t> conn(n);
-    CSG_CHECK(hipMemcpyAsync(conn.data(), g.conn.p, std::size_t(n) * 4, hipMemcpyDeviceToHost,
-                             g.stream));
-    CSG_CHECK(hipStreamSynchronize(g.stream));
-    // labels_with(missing) (connectivity.cpp:352-374)
-    HostUF cuf(ncomp0);
-    for (std::size_t i = 0; i < g.bu.size(); ++i) {
-        if (g.bu[i] >= n || g.bv[i] >= n) continue;
-        cuf.unite(conn[g.bu[i]], conn[g.bv[i]]);
-    }
-    const std::vector<std::uint32_t> merged = cuf.labels();
-    ncomp = cuf.components;
-    std::vector<std::uint32_t> comp(n);
-    for (std::uint32_t v = 0; v < n; ++v) comp[v] = merged[conn[v]];
-    if (st) st->components_before_repair = st->components_after_repair = ncomp;
-    if (ncomp <= 1 || !p.ensure_connected) return;
-
-    std::vector<double> score(n);
-    CSG_CHECK(hipMemcpyAsync(score.data(), g.score.p, std::size_t(n) * 8,
-                             hipMemcpyDeviceToHost, g.stream));
-    CSG_CHECK(hipStreamSynchronize(g.stream));
+// Repair of the nodes [lo, hi) (one instance): reps of each of its components
+// after the bridges, an MST over the reps, the edges that join two
+// components. Returns the number of added edges; *after receives the range's
+// component count after the repair.
+std::size_t repair_range(Graph& g, const coresg_hip_build_params& p,
+                         const std::vector<std::uint32_t>& comp, std::uint32_t ncomp_all,
+                         const std::vector<double>& score, std::uint32_t lo, std::uint32_t hi,
+                         std::uint32_t* before, std::uint32_t* after) {
+    // the range's components, densely renumbered in first-appearance order
+    std::vector<std::uint32_t> local(ncomp_all, kNone);
+    std::uint32_t nloc = 0;
+    for (std::uint32_t v = lo; v < hi; ++v)
+        if (local[comp[v]] == kNone) local[comp[v]] = nloc++;
+    *before = *after = nloc;
+    if (nloc <= 1 || !p.ensure_connected) return 0;
     std::size_t per = p.reps_per_component ? p.reps_per_component : 1;
-    if (p.max_reps && std::size_t(ncomp) * per > p.max_reps)
-        per = std::max<std::size_t>(1, p.max_reps / std::size_t(ncomp));
-    std::vector<std::vector<std::uint32_t>> buckets(ncomp);
-    for (std::uint32_t v = 0; v < n; ++v) {
-        std::vector<std::uint32_t>& b = buckets[comp[v]];
+    if (p.max_reps && std::size_t(nloc) * per > p.max_reps)
+        per = std::max<std::size_t>(1, p.max_reps / std::size_t(nloc));
+    std::vector<std::vector<std::uint32_t>> buckets(nloc);
+    for (std::uint32_t v = lo; v < hi; ++v) {
+        std::vector<std::uint32_t>& b = buckets[local[comp[v]]];
         if (b.size() < per) {
             b.push_back(v);
         } else {
@@ -126,24 +114,66 @@ void repair_connectivity(Graph& g, const coresg_hip_build_params& p, coresg_hip_
                          [&](std::uint32_t a, std::uint32_t b) { return score[a] < score[b]; });
         reps.resize(p.max_reps);
     }
-    if (reps.size() < 2) return;
+    if (reps.size() < 2) return 0;
     std::vector<std::uint32_t> ru, rv;
     std::vector<float> rw;
     mst_over_subset(g, reps, &ru, &rv, &rw);
-    HostUF uf(ncomp);
+    HostUF uf(nloc);
     std::size_t added = 0;
     for (std::size_t i = 0; i < ru.size(); ++i) {
-        if (uf.unite(comp[ru[i]], comp[rv[i]])) {
+        if (uf.unite(local[comp[ru[i]]], local[comp[rv[i]]])) {
             g.bu.push_back(ru[i]);
             g.bv.push_back(rv[i]);
             g.bw.push_back(rw[i]);
             ++added;
         }
     }
+    *after = uf.components;
+    return added;
+}
+
+void repair_connectivity(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_stats* st) {
+    const std::uint32_t n = g.n, ncomp0 = g.num_components;
+    const std::uint32_t ninst = g.num_instances();
+    if (st) st->components_before_repair = st->components_after_repair = ncomp0;
+    if (ncomp0 <= ninst || n == 0) return;  // every instance is one component
+
+    std::vector<std::uint32_t> conn(n);
+    CSG_CHECK(hipMemcpyAsync(conn.data(), g.conn.p, std::size_t(n) * 4, hipMemcpyDeviceToHost,
+                             g.stream));
+    CSG_CHECK(hipStreamSynchronize(g.stream));
+    // labels_with(missing) (connectivity.cpp:352-374)
+    HostUF cuf(ncomp0);
+    for (std::size_t i = 0; i < g.bu.size(); ++i) {
+        if (g.bu[i] >= n || g.bv[i] >= n) continue;
+        cuf.unite(conn[g.bu[i]], conn[g.bv[i]]);
+    }
+    const std::vector<std::uint32_t> merged = cuf.labels();
+    const std::uint32_t ncomp = cuf.components;
+    std::vector<std::uint32_t> comp(n);
+    for (std::uint32_t v = 0; v < n; ++v) comp[v] = merged[conn[v]];
+    if (st) st->components_before_repair = st->components_after_repair = ncomp;
+    if (ncomp <= ninst || !p.ensure_connected) return;
+
+    std::vector<double> score(n);
+    CSG_CHECK(hipMemcpyAsync(score.data(), g.score.p, std::size_t(n) * 8,
+                             hipMemcpyDeviceToHost, g.stream));
+    CSG_CHECK(hipStreamSynchronize(g.stream));
+    std::size_t added = 0;
+    std::uint32_t before_all = 0, after_all = 0;
+    for (std::uint32_t i = 0; i < ninst; ++i) {
+        const std::uint32_t lo = g.inst_off.empty() ? 0 : g.inst_off[i];
+        const std::uint32_t hi = g.inst_off.empty() ? n : g.inst_off[i + 1];
+        std::uint32_t before = 0, after = 0;
+        added += repair_range(g, p, comp, ncomp, score, lo, hi, &before, &after);
+        before_all += before;
+        after_all += after;
+    }
     g.repair_count = added;
     if (st) {
         st->repair_edges = added;
-        st->components_after_repair = uf.components;
+        st->components_before_repair = before_all;
+        st->components_after_repair = after_all;
     }
 }
 
@@ -157,17 +187,48 @@ void build_bridges(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_
     const auto t0 = Clock::now();
     std::uint32_t tagged = 0;
     hubness_compute(g, &tagged);
-    const std::uint64_t knn_edges = count_knn_edges(g);
-    const std::size_t budget = antihub_budget(p, g.dim, knn_edges, g.n);
-    g.selected = select_antihubs(g, budget, p.min_per_component);
-    const double hub_s = since(t0);
-
-    const auto t1 = Clock::now();
     g.bu.clear();
     g.bv.clear();
     g.bw.clear();
     g.repair_count = 0;
-    mst_over_subset(g, g.selected, &g.bu, &g.bv, &g.bw);
+    std::uint64_t knn_edges = 0;
+    double bridge_s = 0.0;
+    if (g.inst_off.empty()) {
+        knn_edges = count_knn_edges(g);
+        const std::size_t budget = antihub_budget(p, g.dim, knn_edges, g.n);
+        g.selected = select_antihubs(g, budget, p.min_per_component);
+        const auto t1 = Clock::now();
+        mst_over_subset(g, g.selected, &g.bu, &g.bv, &g.bw);
+        bridge_s += since(t1);
+    } else {
+        // per instance, as if built alone: budget from the instance's own
+        // edge count, its lowest nodes, the quota winners of its components
+        // (components never cross instances, so one pass finds them all),
+        // the bridge MST over its selection
+        const std::vector<std::uint64_t> edges = knn_edges_per_instance(g);
+        const std::vector<std::uint32_t> quota = quota_winners(g, p.min_per_component);
+        std::vector<std::vector<std::uint32_t>> quota_of(g.num_instances());
+        for (std::uint32_t w : quota) {
+            if (w == kNone) continue;
+            const std::size_t i = static_cast<std::size_t>(
+                std::upper_bound(g.inst_off.begin(), g.inst_off.end(), w) - g.inst_off.begin() - 1);
+            quota_of[i].push_back(w);
+        }
+        g.selected.clear();
+        for (std::uint32_t i = 0; i < g.num_instances(); ++i) {
+            const std::uint32_t lo = g.inst_off[i], cnt = g.inst_off[i + 1] - lo;
+            knn_edges += edges[i];
+            const std::size_t budget = antihub_budget(p, g.dim, edges[i], cnt);
+            const std::vector<std::uint32_t> sel =
+                merge_selection(g, select_budget(g, lo, cnt, budget), quota_of[i]);
+            g.selected.insert(g.selected.end(), sel.begin(), sel.end());
+            const auto t1 = Clock::now();
+            mst_over_subset(g, sel, &g.bu, &g.bv, &g.bw);
+            bridge_s += since(t1);
+        }
+    }
+    const double hub_s = since(t0) - bridge_s;
+    const auto t1 = Clock::now();
     if (st) {
         st->knn_edges = knn_edges;
         st->antihubs_selected = g.selected.size();
@@ -177,7 +238,7 @@ void build_bridges(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_
     g.edges_ready = false;
     if (st) {
         st->hubness_seconds = hub_s;
-        st->bridge_seconds = since(t1);
+        st->bridge_seconds = bridge_s + since(t1);
         st->bridge_edges = g.bu.size();
     }
 }

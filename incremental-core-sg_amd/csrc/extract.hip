@@ -361,6 +361,50 @@ std::uint64_t count_knn_edges(Graph& g) {
     return total;
 }
 
+namespace {
+// One block per instance: the sum of its nodes' owned-edge counts.
+__global__ void __launch_bounds__(kThreads)
+    segment_sum_kernel(const std::uint32_t* __restrict__ deg,
+                       const std::uint32_t* __restrict__ off,
+                       unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long s_w[kThreads / 64];
+    const std::uint32_t lo = off[blockIdx.x], hi = off[blockIdx.x + 1];
+    unsigned long long acc = 0;
+    for (std::uint32_t v = lo + threadIdx.x; v < hi; v += blockDim.x) acc += deg[v];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63u) == 0) s_w[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (unsigned w = 0; w < blockDim.x / 64; ++w) t += s_w[w];
+        out[blockIdx.x] = t;
+    }
+}
+}  // namespace
+
+std::vector<std::uint64_t> knn_edges_per_instance(Graph& g) {
+    const std::uint32_t ninst = g.num_instances();
+    if (g.inst_off.empty()) return {count_knn_edges(g)};
+    const std::size_t slots = std::size_t(g.n) * g.k;
+    DBuf<std::uint32_t> deg;
+    DBuf<unsigned long long> out;
+    deg.reserve(g.n);
+    out.reserve(ninst);
+    fill_u32(deg.p, g.n, 0u, g.stream);
+    // every owned edge counted at its smaller endpoint, which lies in the
+    // same instance as the other one
+    emit_count_kernel<<<capped_blocks(slots, kThreads), kThreads, 0, g.stream>>>(
+        g.ids.p, g.dist.p, g.count.p, g.n, g.k, nullptr, deg.p, nullptr);
+    CSG_LAUNCH_CHECK();
+    segment_sum_kernel<<<ninst, kThreads, 0, g.stream>>>(deg.p, g.inst_off_d.p, out.p);
+    CSG_LAUNCH_CHECK();
+    std::vector<std::uint64_t> h(ninst);
+    CSG_CHECK(hipMemcpyAsync(h.data(), out.p, std::size_t(ninst) * 8, hipMemcpyDeviceToHost,
+                             g.stream));
+    CSG_CHECK(hipStreamSynchronize(g.stream));
+    return h;
+}
+
 void reweighted_edges(Graph& g, std::uint32_t mpts, std::uint32_t* u, std::uint32_t* v,
                       float* w) {
     check_mpts(g, &mpts);

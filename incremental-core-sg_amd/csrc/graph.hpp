@@ -129,6 +129,15 @@ struct Graph {
     std::vector<std::uint32_t> selected;
     bool hub_ready = false;
 
+    // Independent instances sharing the graph (batched uneven point sets):
+    // instance i holds points [inst_off[i], inst_off[i+1]); empty = one
+    // instance of all points. inst_of / inst_off_d are the device copies.
+    std::vector<std::uint32_t> inst_off;
+    DBuf<std::uint32_t> inst_of, inst_off_d;
+    std::uint32_t num_instances() const {
+        return inst_off.empty() ? 1u : static_cast<std::uint32_t>(inst_off.size() - 1);
+    }
+
     // Profiling (audit: FP64 check of the Gram-filter join's distances).
     bool profiling = false;
     bool audit = false;
@@ -164,5 +173,9 @@ void extract_msts(Graph& g, const std::uint32_t* mpts, std::uint32_t count, void
                   std::uint32_t* out_labels);
 void reweighted_edges(Graph& g, std::uint32_t mpts, std::uint32_t* u, std::uint32_t* v,
                       float* w);
+// Per-instance MST summaries of a batch of mpts values (info[i * count + q]).
+void extract_instances(Graph& g, const std::uint32_t* mpts, std::uint32_t count, void* info);
+// Undirected k-NN edge count of every instance.
+std::vector<std::uint64_t> knn_edges_per_instance(Graph& g);
 
 }  // namespace coresg_hip

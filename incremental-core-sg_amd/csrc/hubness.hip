@@ -353,77 +353,92 @@ void hubness_compute(Graph& g, std::uint32_t* tagged_out) {
     g.hub_ready = true;
 }
 
-std::vector<std::uint32_t> select_antihubs(Graph& g, std::size_t budget, std::size_t per) {
-    const std::uint32_t n = g.n;
+std::vector<std::uint32_t> select_budget(Graph& g, std::uint32_t lo, std::uint32_t cnt,
+                                         std::size_t budget) {
     hipStream_t s = g.stream;
     std::vector<std::uint32_t> sel;
-    if (budget > n) budget = n;
+    if (budget > cnt) budget = cnt;
+    if (budget == cnt) {
+        sel.resize(cnt);
+        for (std::uint32_t i = 0; i < cnt; ++i) sel[i] = lo + i;
+        return sel;
+    }
+    if (budget == 0) return sel;
+    // the range's arrays from `lo`: keys (score, rdeg, local index) order
+    // ties by id exactly as the global index does
+    const double* score = g.score.p + lo;
+    const std::int32_t* rdeg = g.rdeg.p + lo;
     DBuf<std::uint32_t> picked;
-    picked.reserve(budget ? budget : 1);
+    picked.reserve(budget);
     g.counters.reserve(16);
     std::uint32_t* d_cnt = g.counters.p + 14;
     fill_u32(d_cnt, 1, 0u, s);
-    if (budget > 0 && budget < n) {
-        DBuf<SelectState> st;
-        DBuf<std::uint32_t> hist;
-        st.reserve(1);
-        hist.reserve(256);
-        SelectState init{0, 0, 0, 0, static_cast<unsigned int>(budget - 1), 0};
-        CSG_CHECK(hipMemcpyAsync(st.p, &init, sizeof(init), hipMemcpyHostToDevice, s));
-        fill_u32(hist.p, 256, 0u, s);
-        const unsigned nb = capped_blocks(n, kThreads, 2048);
-        for (int d = 0; d < 16; ++d) {
-            radix_hist_kernel<<<nb, kThreads, 0, s>>>(n, g.score.p, g.rdeg.p, st.p, d, hist.p);
-            CSG_LAUNCH_CHECK();
-            radix_pick_kernel<<<1, 256, 0, s>>>(st.p, d, hist.p);
-            CSG_LAUNCH_CHECK();
-        }
-        collect_le_kernel<<<capped_blocks(n, kThreads), kThreads, 0, s>>>(
-            n, g.score.p, g.rdeg.p, st.p, static_cast<std::uint32_t>(budget), picked.p, d_cnt);
+    DBuf<SelectState> st;
+    DBuf<std::uint32_t> hist;
+    st.reserve(1);
+    hist.reserve(256);
+    SelectState init{0, 0, 0, 0, static_cast<unsigned int>(budget - 1), 0};
+    CSG_CHECK(hipMemcpyAsync(st.p, &init, sizeof(init), hipMemcpyHostToDevice, s));
+    fill_u32(hist.p, 256, 0u, s);
+    const unsigned nb = capped_blocks(cnt, kThreads, 2048);
+    for (int d = 0; d < 16; ++d) {
+        radix_hist_kernel<<<nb, kThreads, 0, s>>>(cnt, score, rdeg, st.p, d, hist.p);
         CSG_LAUNCH_CHECK();
-        std::uint32_t got = 0;
-        CSG_CHECK(hipMemcpyAsync(&got, d_cnt, 4, hipMemcpyDeviceToHost, s));
-        CSG_CHECK(hipStreamSynchronize(s));
-        if (got != budget) throw std::runtime_error("select: radix select miscounted");
-        sel.resize(budget);
-        CSG_CHECK(hipMemcpyAsync(sel.data(), picked.p, budget * 4, hipMemcpyDeviceToHost, s));
-    } else if (budget == n) {
-        sel.resize(n);
-        for (std::uint32_t i = 0; i < n; ++i) sel[i] = i;
+        radix_pick_kernel<<<1, 256, 0, s>>>(st.p, d, hist.p);
+        CSG_LAUNCH_CHECK();
     }
+    collect_le_kernel<<<capped_blocks(cnt, kThreads), kThreads, 0, s>>>(
+        cnt, score, rdeg, st.p, static_cast<std::uint32_t>(budget), picked.p, d_cnt);
+    CSG_LAUNCH_CHECK();
+    std::uint32_t got = 0;
+    CSG_CHECK(hipMemcpyAsync(&got, d_cnt, 4, hipMemcpyDeviceToHost, s));
+    CSG_CHECK(hipStreamSynchronize(s));
+    if (got != budget) throw std::runtime_error("select: radix select miscounted");
+    sel.resize(budget);
+    CSG_CHECK(hipMemcpyAsync(sel.data(), picked.p, budget * 4, hipMemcpyDeviceToHost, s));
+    CSG_CHECK(hipStreamSynchronize(s));
+    for (std::uint32_t& v : sel) v += lo;
+    return sel;
+}
 
+std::vector<std::uint32_t> quota_winners(Graph& g, std::size_t per) {
+    const std::uint32_t n = g.n;
+    hipStream_t s = g.stream;
     std::vector<std::uint32_t> quota;
     const std::uint32_t ncomp = g.num_components;
-    if (per > 0 && ncomp > 0) {
-        DBuf<unsigned long long> qs;
-        DBuf<std::uint32_t> qr, qi, winners;
-        DBuf<std::uint8_t> excl;
-        qs.reserve(ncomp);
-        qr.reserve(ncomp);
-        qi.reserve(ncomp);
-        winners.reserve(std::size_t(ncomp) * per);
-        excl.reserve(n);
-        fill_u64(reinterpret_cast<std::uint64_t*>(qs.p), ncomp, ~0ull, s);
-        fill_u32(qr.p, ncomp, kNone, s);
-        fill_u32(qi.p, ncomp, kNone, s);
-        fill_u8(excl.p, n, 0, s);
-        const unsigned nb = capped_blocks(n, kThreads);
-        for (std::size_t r = 0; r < per; ++r) {
-            for (int stage = 0; stage < 3; ++stage) {
-                quota_stage_kernel<<<nb, kThreads, 0, s>>>(n, g.score.p, g.rdeg.p, g.conn.p,
-                                                          ncomp, excl.p, stage, qs.p, qr.p, qi.p);
-                CSG_LAUNCH_CHECK();
-            }
-            quota_commit_kernel<<<capped_blocks(ncomp, kThreads), kThreads, 0, s>>>(
-                ncomp, static_cast<std::uint32_t>(r), qs.p, qr.p, qi.p, excl.p, winners.p);
+    if (per == 0 || ncomp == 0) return quota;
+    DBuf<unsigned long long> qs;
+    DBuf<std::uint32_t> qr, qi, winners;
+    DBuf<std::uint8_t> excl;
+    qs.reserve(ncomp);
+    qr.reserve(ncomp);
+    qi.reserve(ncomp);
+    winners.reserve(std::size_t(ncomp) * per);
+    excl.reserve(n);
+    fill_u64(reinterpret_cast<std::uint64_t*>(qs.p), ncomp, ~0ull, s);
+    fill_u32(qr.p, ncomp, kNone, s);
+    fill_u32(qi.p, ncomp, kNone, s);
+    fill_u8(excl.p, n, 0, s);
+    const unsigned nb = capped_blocks(n, kThreads);
+    for (std::size_t r = 0; r < per; ++r) {
+        for (int stage = 0; stage < 3; ++stage) {
+            quota_stage_kernel<<<nb, kThreads, 0, s>>>(n, g.score.p, g.rdeg.p, g.conn.p, ncomp,
+                                                      excl.p, stage, qs.p, qr.p, qi.p);
             CSG_LAUNCH_CHECK();
         }
-        quota.resize(std::size_t(ncomp) * per);
-        CSG_CHECK(hipMemcpyAsync(quota.data(), winners.p, quota.size() * 4,
-                                 hipMemcpyDeviceToHost, s));
+        quota_commit_kernel<<<capped_blocks(ncomp, kThreads), kThreads, 0, s>>>(
+            ncomp, static_cast<std::uint32_t>(r), qs.p, qr.p, qi.p, excl.p, winners.p);
+        CSG_LAUNCH_CHECK();
     }
+    quota.resize(std::size_t(ncomp) * per);
+    CSG_CHECK(hipMemcpyAsync(quota.data(), winners.p, quota.size() * 4, hipMemcpyDeviceToHost, s));
     CSG_CHECK(hipStreamSynchronize(s));
+    return quota;
+}
 
+std::vector<std::uint32_t> merge_selection(Graph& g, std::vector<std::uint32_t> sel,
+                                           const std::vector<std::uint32_t>& quota) {
+    hipStream_t s = g.stream;
     // Union of the budget set and the quota winners (distinct by
     // construction: one component per node, earlier winners excluded), then
     // the reference's final order (score, rdeg, id).
@@ -461,6 +476,10 @@ std::vector<std::uint32_t> select_antihubs(Graph& g, std::size_t budget, std::si
     std::vector<std::uint32_t> out(m);
     for (std::uint32_t i = 0; i < m; ++i) out[i] = all[order[i]];
     return out;
+}
+
+std::vector<std::uint32_t> select_antihubs(Graph& g, std::size_t budget, std::size_t per) {
+    return merge_selection(g, select_budget(g, 0, g.n, budget), quota_winners(g, per));
 }
 
 }  // namespace coresg_hip

@@ -15,6 +15,14 @@ void hubness_compute(Graph& g, std::uint32_t* tagged);
 // The `budget` lowest nodes by (score, rdeg, id) plus the `per` lowest of every
 // component, in that order (HubnessIndex::select).
 std::vector<std::uint32_t> select_antihubs(Graph& g, std::size_t budget, std::size_t per);
+// Its parts: the `budget` lowest of the nodes [lo, lo + cnt) (global ids, any
+// order); the `per` lowest of every component (ncomp * per, kNone padded);
+// their union in (score, rdeg, id) order.
+std::vector<std::uint32_t> select_budget(Graph& g, std::uint32_t lo, std::uint32_t cnt,
+                                         std::size_t budget);
+std::vector<std::uint32_t> quota_winners(Graph& g, std::size_t per);
+std::vector<std::uint32_t> merge_selection(Graph& g, std::vector<std::uint32_t> sel,
+                                           const std::vector<std::uint32_t>& quota);
 
 // MST over the subset `ids` in mutual-reachability space at k_max, edges in
 // Prim acceptance order, stored with the base distance in global ids

@@ -59,7 +59,8 @@ __global__ void __launch_bounds__(256)
     init_random_kernel(const float* __restrict__ pts, std::uint32_t n, std::uint32_t stride,
                        std::uint32_t k, std::uint64_t seed, std::uint32_t* __restrict__ ids,
                        float* __restrict__ dist, std::uint8_t* __restrict__ flags,
-                       std::uint32_t* __restrict__ count) {
+                       std::uint32_t* __restrict__ count, const std::uint32_t* __restrict__ inst_of,
+                       const std::uint32_t* __restrict__ inst_off) {
     __shared__ std::uint32_t s_stage_id[4][64];
     __shared__ float s_stage_d[4][64];
     const std::uint32_t lane = threadIdx.x & 63u, wib = threadIdx.x >> 6;
@@ -70,9 +71,17 @@ __global__ void __launch_bounds__(256)
         const std::uint32_t cnt0 = count[node];
         const std::uint32_t needed = k - cnt0;
         const std::uint32_t attempts = needed + needed / 2 + 4;
+        // With instances the node draws inside its own instance, with its
+        // instance-local index: exactly the init of the instance built alone.
+        std::uint32_t lo = 0, span = n;
+        if (inst_of) {
+            const std::uint32_t i = inst_of[node];
+            lo = inst_off[i];
+            span = inst_off[i + 1] - lo;
+        }
         // NodeRng(seed, node, 0): state s0, one discarded draw; draw t uses
         // state s0 + (t + 2) * phi.
-        const std::uint64_t s0 = seed ^ (0x9E3779B97F4A7C15ull * (std::uint64_t(node) + 1)) ^
+        const std::uint64_t s0 = seed ^ (0x9E3779B97F4A7C15ull * (std::uint64_t(node - lo) + 1)) ^
                                  (0xC2B2AE3D27D4EB4Full * 1ull);
         // existing entries
         const std::uint32_t my_id = lane < cnt0 ? ids[base + lane] : kNone;
@@ -89,7 +98,7 @@ __global__ void __launch_bounds__(256)
             if (live) {
                 const std::uint64_t z =
                     splitmix_mix(s0 + (std::uint64_t(t) + 2ull) * 0x9E3779B97F4A7C15ull);
-                cand = static_cast<std::uint32_t>(z >> 32) % n;
+                cand = static_cast<std::uint32_t>(z >> 32) % span + lo;
             }
             bool ok = live && cand != node;
             // not already in the list (existing or accepted in earlier chunks)
@@ -1086,8 +1095,10 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     if (random_fill && n >= 2) {
         std::unique_ptr<ScopedKernelTimer> t;
         if (gp) t.reset(new ScopedKernelTimer(*gp, "nnd_init"));
+        const bool inst = gp && !gp->inst_off.empty();
         init_random_kernel<<<wave_blocks, 256, 0, s>>>(pts, n, stride, k, seed, ids, dist, flags,
-                                                       count);
+                                                       count, inst ? gp->inst_of.p : nullptr,
+                                                       inst ? gp->inst_off_d.p : nullptr);
         CSG_LAUNCH_CHECK();
     }
     if (iters == 0) return res;
@@ -1760,6 +1771,8 @@ void build_knn(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_stat
     fill_u8(g.flags.p, nk, 0, s);
     fill_u32(g.count.p, n, 0u, s);
     DescentResult r;
+    if (!g.inst_off.empty() && p.brute_force_limit)
+        throw std::runtime_error("build_knn: brute_force_limit is not supported with instances");
     if (p.brute_force_limit && n <= p.brute_force_limit) {
         brute_force_kernel<<<capped_blocks(n, 64), 64, 0, s>>>(g.pts.p, n, g.stride, k, g.ids.p,
                                                                g.dist.p, g.flags.p, g.count.p);
@@ -1823,6 +1836,7 @@ void nndescent_host(int device, const coresg_hip_nnd_config& cfg, const float* v
 void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, int padded,
                    const coresg_hip_build_params& p, coresg_hip_build_stats* st) {
     if (!g.knn_set) throw std::runtime_error("insert: the CORE-SG is not built");
+    if (!g.inst_off.empty()) throw std::runtime_error("insert: not supported with instances");
     const auto t0 = std::chrono::steady_clock::now();
     if (count == 0) return;
     const std::uint32_t n0 = g.n, k = g.k, stride = g.stride;

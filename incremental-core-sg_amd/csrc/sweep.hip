@@ -470,6 +470,71 @@ __global__ void __launch_bounds__(kThreads)
         labels[v] = first_scan[minv[comp[std::size_t(v) * B + b]]];
 }
 
+// ---- per-instance summaries (independent instances sharing the graph) -------
+
+// One block per instance; a thread keeps one slot (threads beyond the largest
+// multiple of B idle) and folds the instance's per-(vertex, slot) hook keys.
+// Pass 0: MST edge count and (smallest exponent | any-inf) per slot; pass 1:
+// the exact integer sum against the instance's smallest exponent.
+template <int PASS>
+__global__ void __launch_bounds__(kThreads)
+    inst_fold_kernel(Sweep s, const std::uint32_t* __restrict__ off,
+                     const std::uint64_t* __restrict__ emin, std::uint64_t* __restrict__ out_a,
+                     std::uint64_t* __restrict__ out_b) {
+    __shared__ std::uint64_t s_a[kThreads], s_b[kThreads];
+    const std::uint32_t inst = blockIdx.x, B = s.B;
+    const std::uint32_t T = blockDim.x - blockDim.x % B;
+    const std::uint32_t lo = off[inst], hi = off[inst + 1];
+    const std::uint32_t b = threadIdx.x % B;
+    std::uint64_t a = 0, x = PASS == 0 ? 0xFFFFFFFFull : 0ull;
+    bool none = false;
+    int e_min = 0;
+    if (PASS == 1 && threadIdx.x < T) {
+        const std::uint64_t ex = emin[std::size_t(inst) * B + b];
+        none = (ex & 0xFFFFFFFFull) == 0xFFFFFFFFull || (ex & kExpInf);
+        e_min = int(ex & 0xFFFFFFFFull) - 1024;
+    }
+    if (threadIdx.x < T && !none) {
+        for (std::size_t i = std::size_t(lo) * B + threadIdx.x; i < std::size_t(hi) * B; i += T) {
+            const std::uint64_t key = s.mkey[i];
+            if (key == kNone64) continue;
+            const std::uint32_t bits = static_cast<std::uint32_t>(key >> 32);
+            if (PASS == 0) {
+                ++a;
+                if ((bits & 0x7FFFFFFFu) == 0) continue;
+                if ((bits & 0x7F800000u) == 0x7F800000u) {
+                    x |= kExpInf;
+                    continue;
+                }
+                std::uint32_t M;
+                int e;
+                odd_decompose(bits, &M, &e);
+                x = OpExp()(x, static_cast<std::uint32_t>(e + 1024));
+            } else {
+                if ((bits & 0x7FFFFFFFu) == 0) continue;
+                std::uint32_t M;
+                int e;
+                odd_decompose(bits, &M, &e);
+                const int sh = e - e_min;
+                const int len = 32 - __builtin_clz(M);
+                x = (sh + len > 53) ? kOver : checked_add(x, static_cast<std::uint64_t>(M) << sh);
+            }
+        }
+    }
+    s_a[threadIdx.x] = a;
+    s_b[threadIdx.x] = x;
+    __syncthreads();
+    if (threadIdx.x < B) {
+        std::uint64_t ra = 0, rx = PASS == 0 ? 0xFFFFFFFFull : 0ull;
+        for (std::uint32_t t = threadIdx.x; t < T; t += B) {
+            ra += s_a[t];
+            rx = PASS == 0 ? OpExp()(rx, s_b[t]) : OpAdd()(rx, s_b[t]);
+        }
+        if (PASS == 0) out_a[std::size_t(inst) * B + threadIdx.x] = ra;
+        out_b[std::size_t(inst) * B + threadIdx.x] = rx;
+    }
+}
+
 std::uint32_t pow2_at_least(std::uint32_t x) {
     std::uint32_t p = 1;
     while (p < x) p <<= 1;
@@ -495,13 +560,14 @@ void launch_round(const Sweep& sw, bool first, unsigned blocks, hipStream_t s) {
 // Exact double total of one slot's MST from the device statistics, or the
 // host's Kruskal-order sum when the integer sum saturated.
 double slot_weight(Graph& g, const Sweep& sw, std::uint32_t b, std::uint64_t ex,
-                   std::uint64_t sum) {
+                   std::uint64_t sum, std::uint32_t lo = 0, std::uint32_t hi = kNone) {
+    if (hi == kNone) hi = sw.n;
     if (ex & kExpInf) return HUGE_VAL;
     if ((ex & 0xFFFFFFFFull) == 0xFFFFFFFFull) return 0.0;
     if (!(sum & kOver)) return std::ldexp(static_cast<double>(sum), int(ex & 0xFFFFFFFFull) - 1024);
-    std::vector<std::uint64_t> keys(sw.n);
-    CSG_CHECK(hipMemcpy2DAsync(keys.data(), 8, sw.mkey + b, std::size_t(sw.B) * 8, 8, sw.n,
-                               hipMemcpyDeviceToHost, g.stream));
+    std::vector<std::uint64_t> keys(hi - lo);
+    CSG_CHECK(hipMemcpy2DAsync(keys.data(), 8, sw.mkey + std::size_t(lo) * sw.B + b,
+                               std::size_t(sw.B) * 8, 8, hi - lo, hipMemcpyDeviceToHost, g.stream));
     CSG_CHECK(hipStreamSynchronize(g.stream));
     keys.erase(std::remove(keys.begin(), keys.end(), kNone64), keys.end());
     std::sort(keys.begin(), keys.end());  // Kruskal order: (weight bits, rank)
@@ -517,7 +583,8 @@ double slot_weight(Graph& g, const Sweep& sw, std::uint32_t b, std::uint64_t ex,
 
 void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_mst_info* info,
                std::uint32_t* out_u, std::uint32_t* out_v, float* out_w,
-               std::uint32_t* out_labels) {
+               std::uint32_t* out_labels, coresg_hip_mst_info* inst_info = nullptr,
+               std::uint32_t inst_stride = 0) {
     const std::uint32_t n = g.n;
     const std::uint64_t m = g.num_edges;
     hipStream_t s = g.stream;
@@ -607,6 +674,38 @@ void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_
             if (total[b] >= n) throw std::runtime_error("extract: more MST edges than n-1");
             if (cnt[b] == 0 || total[b] + 1 >= n) sw.live &= ~(1u << b);
         }
+    }
+
+    if (inst_info) {
+        // per instance: edges, components, exact weight (rounds: the batch's)
+        const std::uint32_t ninst = g.num_instances();
+        const std::size_t m = std::size_t(ninst) * B;
+        DBuf<std::uint64_t> icnt, iexp, isum;
+        icnt.reserve(m);
+        iexp.reserve(m);
+        isum.reserve(m);
+        inst_fold_kernel<0><<<ninst, kThreads, 0, s>>>(sw, g.inst_off_d.p, nullptr, icnt.p, iexp.p);
+        CSG_LAUNCH_CHECK();
+        inst_fold_kernel<1><<<ninst, kThreads, 0, s>>>(sw, g.inst_off_d.p, iexp.p, nullptr, isum.p);
+        CSG_LAUNCH_CHECK();
+        std::vector<std::uint64_t> hc(m), he(m), hs(m);
+        CSG_CHECK(hipMemcpyAsync(hc.data(), icnt.p, m * 8, hipMemcpyDeviceToHost, s));
+        CSG_CHECK(hipMemcpyAsync(he.data(), iexp.p, m * 8, hipMemcpyDeviceToHost, s));
+        CSG_CHECK(hipMemcpyAsync(hs.data(), isum.p, m * 8, hipMemcpyDeviceToHost, s));
+        CSG_CHECK(hipStreamSynchronize(s));
+        for (std::uint32_t i = 0; i < ninst; ++i) {
+            const std::uint32_t lo = g.inst_off[i], hi = g.inst_off[i + 1];
+            for (std::uint32_t b = 0; b < B; ++b) {
+                coresg_hip_mst_info& r = inst_info[std::size_t(i) * inst_stride + b];
+                const std::size_t j = std::size_t(i) * B + b;
+                r.mpts = hm[b];
+                r.num_edges = static_cast<std::uint32_t>(hc[j]);
+                r.num_components = (hi - lo) - static_cast<std::uint32_t>(hc[j]);
+                r.rounds = rounds[b];
+                r.weight = slot_weight(g, sw, b, he[j], hs[j], lo, hi);
+            }
+        }
+        return;
     }
 
     // exact totals
@@ -706,6 +805,19 @@ void extract_msts(Graph& g, const std::uint32_t* mpts_list, std::uint32_t count,
         run_batch(g, mpts_list + q0, B, info + q0, out_u ? out_u + q0 * estride : nullptr,
                   out_v ? out_v + q0 * estride : nullptr, out_w ? out_w + q0 * estride : nullptr,
                   out_labels ? out_labels + q0 * n : nullptr);
+    }
+}
+
+void extract_instances(Graph& g, const std::uint32_t* mpts_list, std::uint32_t count,
+                       void* info_v) {
+    auto* info = static_cast<coresg_hip_mst_info*>(info_v);
+    if (g.inst_off.empty()) throw std::runtime_error("extract_instances: no instances set");
+    prepare_edges(g);
+    std::vector<coresg_hip_mst_info> union_info(kMaxSlots);
+    for (std::uint32_t q0 = 0; q0 < count; q0 += kMaxSlots) {
+        const std::uint32_t B = std::min<std::uint32_t>(kMaxSlots, count - q0);
+        run_batch(g, mpts_list + q0, B, union_info.data(), nullptr, nullptr, nullptr, nullptr,
+                  info + q0, count);
     }
 }
 
