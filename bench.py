@@ -226,6 +226,73 @@ def mixed_precision_run(chip, datagen, n=2_000_000, d=128, k=32, mpts=(4, 16, 32
     return out
 
 
+def batched_run(chip, datagen, n_sets=256, k=24, seed=5):
+    """Batched uneven instances: n_sets independent mixtures (n log-uniform in
+    [2k, 200k], d in {8, 16, 32, 64}, 4-40 components, 0-10 % noise), one graph
+    per d bucket with the instances as disjoint id ranges; each bucket built
+    and swept (mpts 2..k) in one pass. Beside it the plain form: every
+    instance as its own graph, built and swept one after another."""
+    rng = np.random.default_rng(seed)
+    dims = rng.choice([8, 16, 32, 64], size=n_sets)
+    sizes = np.exp(rng.uniform(np.log(2_000), np.log(200_000), size=n_sets)).astype(int)
+    comps = rng.integers(4, 41, size=n_sets)
+    noise = rng.uniform(0.0, 0.10, size=n_sets)
+    mpts = list(range(2, k + 1))
+    sets = [datagen.gaussian_mixture(int(sizes[i]), int(dims[i]), int(comps[i]), 0.2, 3.0,
+                                     float(noise[i]), seed=seed * 1000 + i) for i in range(n_sets)]
+    params = chip.build_params(sample=min(16, k // 2), iters=10)
+    out = dict(workload=f"{n_sets} mixtures, n log-uniform [2k,200k], d in {{8,16,32,64}}, "
+                        f"k_max={k}, mpts 2..{k}", instances=n_sets,
+               points=int(sizes.sum()), msts=n_sets * len(mpts), buckets={})
+    fused_build = fused_sweep = 0.0
+    weights = {}
+    for d in (8, 16, 32, 64):
+        idx = [i for i in range(n_sets) if dims[i] == d]
+        if not idx:
+            continue
+        x = np.concatenate([sets[i] for i in idx])
+        off = np.concatenate([[0], np.cumsum([sets[i].shape[0] for i in idx])]).astype(np.uint32)
+        g = chip.Graph(x.shape[0], d, k)
+        g.set_points(x)
+        g.set_instances(off)
+        g.build(params)  # warm (allocations)
+        t0 = time.perf_counter()
+        g.build(params)
+        t1 = time.perf_counter()
+        per = g.extract_instances(mpts)
+        t2 = time.perf_counter()
+        g.close()
+        fused_build += t1 - t0
+        fused_sweep += t2 - t1
+        for j, i in enumerate(idx):
+            weights[i] = [r["weight"] for r in per[j]]
+        out["buckets"][str(d)] = dict(instances=len(idx), points=int(off[-1]),
+                                      build_ms=(t1 - t0) * 1e3, sweep_ms=(t2 - t1) * 1e3)
+        print(f"[bench] batched d={d}: {len(idx)} instances, {int(off[-1])} points, build "
+              f"{(t1 - t0) * 1e3:.1f} ms, sweep {(t2 - t1) * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    # plain form: one graph per instance
+    plain_build = plain_sweep = 0.0
+    for i in range(n_sets):
+        g = chip.Graph(sets[i].shape[0], int(dims[i]), k)
+        g.set_points(sets[i])
+        t0 = time.perf_counter()
+        g.build(params)
+        t1 = time.perf_counter()
+        g.extract(mpts)
+        t2 = time.perf_counter()
+        g.close()
+        plain_build += t1 - t0
+        plain_sweep += t2 - t1
+    total = fused_build + fused_sweep
+    out.update(build_ms=fused_build * 1e3, sweep_ms=fused_sweep * 1e3,
+               pts_per_s=out["points"] / fused_build, msts_per_s=out["msts"] / fused_sweep,
+               plain_build_ms=plain_build * 1e3, plain_sweep_ms=plain_sweep * 1e3,
+               fused_over_plain=(plain_build + plain_sweep) / total)
+    print(f"[bench] batched: build {fused_build * 1e3:.1f} ms + sweep {fused_sweep * 1e3:.1f} ms; "
+          f"plain {plain_build * 1e3:.1f} + {plain_sweep * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,6 +303,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--recall-queries", type=int, default=300)
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched uneven instances (256 mixtures, d in {8,16,32,64})")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision measurement (n=2M, d=128, BF16 Gram filter)")
     ap.add_argument("--no-serving", action="store_true",
@@ -365,6 +434,10 @@ def main():
               f"{serving['insert_p99_ms']:.1f} ms, extract p50/p99 {serving['extract_p50_ms']:.1f}/"
               f"{serving['extract_p99_ms']:.1f} ms", file=sys.stderr, flush=True)
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_run(chip, datagen)
+
     mixed = None
     if not args.no_mixed:
         mixed = mixed_precision_run(chip, datagen)
@@ -417,6 +490,7 @@ def main():
         "cpu_baseline": cpu,
         "serving": serving,
         "mixed_precision": mixed,
+        "batched_instances": batched,
     }
     print(json.dumps(out))
     if args.copy_probe:
