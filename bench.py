@@ -257,7 +257,7 @@ def batched_run(chip, datagen, n_sets=256, k=24, seed=5):
         g.set_instances(off)
         g.build(params)  # warm (allocations)
         t0 = time.perf_counter()
-        g.build(params)
+        st = g.build(params)
         t1 = time.perf_counter()
         per = g.extract_instances(mpts)
         t2 = time.perf_counter()
@@ -267,7 +267,10 @@ def batched_run(chip, datagen, n_sets=256, k=24, seed=5):
         for j, i in enumerate(idx):
             weights[i] = [r["weight"] for r in per[j]]
         out["buckets"][str(d)] = dict(instances=len(idx), points=int(off[-1]),
-                                      build_ms=(t1 - t0) * 1e3, sweep_ms=(t2 - t1) * 1e3)
+                                      build_ms=(t1 - t0) * 1e3, sweep_ms=(t2 - t1) * 1e3,
+                                      knn_ms=st["knn_seconds"] * 1e3,
+                                      hubness_select_ms=st["hubness_seconds"] * 1e3,
+                                      bridges_ms=st["bridge_seconds"] * 1e3)
         print(f"[bench] batched d={d}: {len(idx)} instances, {int(off[-1])} points, build "
               f"{(t1 - t0) * 1e3:.1f} ms, sweep {(t2 - t1) * 1e3:.1f} ms", file=sys.stderr, flush=True)
     # plain form: one graph per instance

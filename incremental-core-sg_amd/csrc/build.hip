@@ -214,18 +214,18 @@ void build_bridges(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_
                 std::upper_bound(g.inst_off.begin(), g.inst_off.end(), w) - g.inst_off.begin() - 1);
             quota_of[i].push_back(w);
         }
-        g.selected.clear();
+        std::vector<std::size_t> budgets(g.num_instances());
         for (std::uint32_t i = 0; i < g.num_instances(); ++i) {
-            const std::uint32_t lo = g.inst_off[i], cnt = g.inst_off[i + 1] - lo;
             knn_edges += edges[i];
-            const std::size_t budget = antihub_budget(p, g.dim, edges[i], cnt);
-            const std::vector<std::uint32_t> sel =
-                merge_selection(g, select_budget(g, lo, cnt, budget), quota_of[i]);
-            g.selected.insert(g.selected.end(), sel.begin(), sel.end());
-            const auto t1 = Clock::now();
-            mst_over_subset(g, sel, &g.bu, &g.bv, &g.bw);
-            bridge_s += since(t1);
+            budgets[i] = antihub_budget(p, g.dim, edges[i], g.inst_off[i + 1] - g.inst_off[i]);
         }
+        const std::vector<std::vector<std::uint32_t>> sels =
+            merge_selections(g, select_budget_instances(g, budgets), quota_of);
+        g.selected.clear();
+        for (const auto& sel : sels) g.selected.insert(g.selected.end(), sel.begin(), sel.end());
+        const auto t1 = Clock::now();
+        mst_over_subsets(g, sels, &g.bu, &g.bv, &g.bw);
+        bridge_s += since(t1);
     }
     const double hub_s = since(t0) - bridge_s;
     const auto t1 = Clock::now();

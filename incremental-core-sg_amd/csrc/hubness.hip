@@ -222,6 +222,77 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
+// One 1024-thread block per instance: the `budgets[i]` lowest nodes of
+// instance i by (score, rdeg, id) -- the MSB radix select of radix_hist/pick/
+// collect_le with the instance-local index as the id, which orders ties
+// exactly as the global index does -- written to out[out_off[i] ...].
+constexpr unsigned kSegThreads = 1024;
+__global__ void __launch_bounds__(kSegThreads)
+    segmented_select_kernel(const double* __restrict__ score, const std::int32_t* __restrict__ rdeg,
+                            const std::uint32_t* __restrict__ inst_off,
+                            const std::uint32_t* __restrict__ budgets,
+                            const std::uint32_t* __restrict__ out_off,
+                            std::uint32_t* __restrict__ out) {
+    __shared__ std::uint32_t h[256];
+    __shared__ SelectState st;
+    __shared__ std::uint32_t filled;
+    const std::uint32_t lo = inst_off[blockIdx.x], cnt = inst_off[blockIdx.x + 1] - lo;
+    const std::uint32_t budget = budgets[blockIdx.x];
+    std::uint32_t* dst = out + out_off[blockIdx.x];
+    const double* sc = score + lo;
+    const std::int32_t* rd = rdeg + lo;
+    if (budget == 0) return;
+    if (budget >= cnt) {
+        for (std::uint32_t v = threadIdx.x; v < cnt; v += blockDim.x) dst[v] = lo + v;
+        return;
+    }
+    if (threadIdx.x == 0) {
+        st = SelectState{0, 0, 0, 0, budget - 1, 0};
+        filled = 0;
+    }
+    for (int digit = 0; digit < 16; ++digit) {
+        if (threadIdx.x < 256) h[threadIdx.x] = 0;
+        __syncthreads();
+        const SelectState s = st;
+        const int shift = (digit < 8) ? (56 - 8 * digit) : (56 - 8 * (digit - 8));
+        for (std::uint32_t v = threadIdx.x; v < cnt; v += blockDim.x) {
+            unsigned long long khi, klo;
+            node_key(sc, rd, v, &khi, &klo);
+            if ((khi & s.mask_hi) != s.pref_hi || (klo & s.mask_lo) != s.pref_lo) continue;
+            atomicAdd(&h[static_cast<unsigned>(((digit < 8) ? khi : klo) >> shift) & 0xFFu], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            SelectState t = st;
+            std::uint32_t cum = 0, b = 0;
+            for (b = 0; b < 256; ++b) {
+                if (t.rank < cum + h[b]) break;
+                cum += h[b];
+            }
+            if (b == 256) b = 255;  // unreachable when rank < |matching set|
+            t.rank -= cum;
+            if (digit < 8) {
+                t.pref_hi |= static_cast<unsigned long long>(b) << shift;
+                t.mask_hi |= 0xFFull << shift;
+            } else {
+                t.pref_lo |= static_cast<unsigned long long>(b) << shift;
+                t.mask_lo |= 0xFFull << shift;
+            }
+            st = t;
+        }
+        __syncthreads();
+    }
+    const SelectState s = st;
+    for (std::uint32_t v = threadIdx.x; v < cnt; v += blockDim.x) {
+        unsigned long long khi, klo;
+        node_key(sc, rd, v, &khi, &klo);
+        if (khi < s.pref_hi || (khi == s.pref_hi && klo <= s.pref_lo)) {
+            const std::uint32_t pos = atomicAdd(&filled, 1u);
+            if (pos < budget) dst[pos] = lo + v;
+        }
+    }
+}
+
 // ---- per-component quota: `per` rounds of a three-stage lexicographic min ---
 
 // Most nodes share a handful of components, so candidates are first reduced
@@ -401,6 +472,38 @@ std::vector<std::uint32_t> select_budget(Graph& g, std::uint32_t lo, std::uint32
     return sel;
 }
 
+std::vector<std::vector<std::uint32_t>> select_budget_instances(
+    Graph& g, const std::vector<std::size_t>& budgets) {
+    const std::uint32_t ninst = g.num_instances();
+    hipStream_t s = g.stream;
+    std::vector<std::uint32_t> b32(ninst), off(ninst + 1, 0);
+    for (std::uint32_t i = 0; i < ninst; ++i) {
+        const std::uint32_t cnt = g.inst_off[i + 1] - g.inst_off[i];
+        b32[i] = static_cast<std::uint32_t>(std::min<std::size_t>(budgets[i], cnt));
+        off[i + 1] = off[i] + b32[i];
+    }
+    std::vector<std::vector<std::uint32_t>> sel(ninst);
+    if (off[ninst] == 0) return sel;
+    DBuf<std::uint32_t> d_b, d_off, d_out;
+    d_b.reserve(ninst);
+    d_off.reserve(std::size_t(ninst) + 1);
+    d_out.reserve(off[ninst]);
+    CSG_CHECK(hipMemcpyAsync(d_b.p, b32.data(), std::size_t(ninst) * 4, hipMemcpyHostToDevice, s));
+    CSG_CHECK(hipMemcpyAsync(d_off.p, off.data(), (std::size_t(ninst) + 1) * 4,
+                             hipMemcpyHostToDevice, s));
+    {
+        ScopedKernelTimer t(g, "hub_select_instances");
+        segmented_select_kernel<<<ninst, kSegThreads, 0, s>>>(g.score.p, g.rdeg.p, g.inst_off_d.p,
+                                                              d_b.p, d_off.p, d_out.p);
+        CSG_LAUNCH_CHECK();
+    }
+    std::vector<std::uint32_t> all(off[ninst]);
+    CSG_CHECK(hipMemcpyAsync(all.data(), d_out.p, all.size() * 4, hipMemcpyDeviceToHost, s));
+    CSG_CHECK(hipStreamSynchronize(s));
+    for (std::uint32_t i = 0; i < ninst; ++i) sel[i].assign(all.begin() + off[i], all.begin() + off[i + 1]);
+    return sel;
+}
+
 std::vector<std::uint32_t> quota_winners(Graph& g, std::size_t per) {
     const std::uint32_t n = g.n;
     hipStream_t s = g.stream;
@@ -476,6 +579,59 @@ std::vector<std::uint32_t> merge_selection(Graph& g, std::vector<std::uint32_t> 
     std::vector<std::uint32_t> out(m);
     for (std::uint32_t i = 0; i < m; ++i) out[i] = all[order[i]];
     return out;
+}
+
+std::vector<std::vector<std::uint32_t>> merge_selections(
+    Graph& g, std::vector<std::vector<std::uint32_t>> sels,
+    const std::vector<std::vector<std::uint32_t>>& quotas) {
+    hipStream_t s = g.stream;
+    // every instance's union of budget set and quota winners, then one
+    // gather of all their scores and reverse degrees
+    std::vector<std::vector<std::uint32_t>> alls(sels.size());
+    std::vector<std::uint32_t> flat;
+    for (std::size_t i = 0; i < sels.size(); ++i) {
+        std::vector<std::uint32_t> all = sels[i];
+        std::sort(sels[i].begin(), sels[i].end());
+        for (std::uint32_t w : quotas[i]) {
+            if (w == kNone || std::binary_search(sels[i].begin(), sels[i].end(), w)) continue;
+            all.push_back(w);
+        }
+        flat.insert(flat.end(), all.begin(), all.end());
+        alls[i] = std::move(all);
+    }
+    const std::uint32_t m = static_cast<std::uint32_t>(flat.size());
+    if (m == 0) return alls;
+    DBuf<std::uint32_t> d_ids;
+    DBuf<double> d_sc;
+    DBuf<std::int32_t> d_rd;
+    d_ids.reserve(m);
+    d_sc.reserve(m);
+    d_rd.reserve(m);
+    CSG_CHECK(hipMemcpyAsync(d_ids.p, flat.data(), std::size_t(m) * 4, hipMemcpyHostToDevice, s));
+    gather_kernel<double><<<blocks_for(m, 256), 256, 0, s>>>(g.score.p, d_ids.p, m, d_sc.p);
+    gather_kernel<std::int32_t><<<blocks_for(m, 256), 256, 0, s>>>(g.rdeg.p, d_ids.p, m, d_rd.p);
+    CSG_LAUNCH_CHECK();
+    std::vector<double> sc(m);
+    std::vector<std::int32_t> rd(m);
+    CSG_CHECK(hipMemcpyAsync(sc.data(), d_sc.p, std::size_t(m) * 8, hipMemcpyDeviceToHost, s));
+    CSG_CHECK(hipMemcpyAsync(rd.data(), d_rd.p, std::size_t(m) * 4, hipMemcpyDeviceToHost, s));
+    CSG_CHECK(hipStreamSynchronize(s));
+    std::size_t base = 0;
+    for (auto& all : alls) {
+        const std::size_t c = all.size();
+        std::vector<std::uint32_t> order(c);
+        for (std::size_t i = 0; i < c; ++i) order[i] = static_cast<std::uint32_t>(i);
+        std::sort(order.begin(), order.end(), [&](std::uint32_t a, std::uint32_t b) {
+            if (sc[base + a] != sc[base + b]) return sc[base + a] < sc[base + b];
+            if (rd[base + a] != rd[base + b]) return rd[base + a] < rd[base + b];
+            return all[a] < all[b];
+        });
+        std::vector<std::uint32_t> out(c);
+        for (std::size_t i = 0; i < c; ++i) out[i] = all[order[i]];
+        all = std::move(out);
+        base += c;
+    }
+    return alls;
 }
 
 std::vector<std::uint32_t> select_antihubs(Graph& g, std::size_t budget, std::size_t per) {

@@ -21,6 +21,19 @@ std::vector<std::uint32_t> select_antihubs(Graph& g, std::size_t budget, std::si
 std::vector<std::uint32_t> select_budget(Graph& g, std::uint32_t lo, std::uint32_t cnt,
                                          std::size_t budget);
 std::vector<std::uint32_t> quota_winners(Graph& g, std::size_t per);
+// merge_selection for several instances at once (one gather).
+std::vector<std::vector<std::uint32_t>> merge_selections(
+    Graph& g, std::vector<std::vector<std::uint32_t>> sels,
+    const std::vector<std::vector<std::uint32_t>>& quotas);
+// select_budget for every instance at once (one block per instance).
+std::vector<std::vector<std::uint32_t>> select_budget_instances(
+    Graph& g, const std::vector<std::size_t>& budgets);
+// mst_over_subset for several subsets at once: every subset's matrix in one
+// launch, every subset's Prim in one launch (a block each); edges appended
+// subset after subset.
+void mst_over_subsets(Graph& g, const std::vector<std::vector<std::uint32_t>>& sets,
+                      std::vector<std::uint32_t>* u, std::vector<std::uint32_t>* v,
+                      std::vector<float>* w);
 std::vector<std::uint32_t> merge_selection(Graph& g, std::vector<std::uint32_t> sel,
                                            const std::vector<std::uint32_t>& quota);
 

@@ -120,11 +120,10 @@ __device__ __forceinline__ void block_argmin(float* v, std::uint32_t* i, float* 
 // Prim over the dense matrix, one 256-thread block. best/from/in_tree live in
 // global scratch of m entries each. Writes accepted edges (local ids,
 // canonical u < v, MRD weight) in acceptance order; out_count[0] = count.
-__global__ void __launch_bounds__(256)
-    prim_kernel(const float* __restrict__ mat, std::uint32_t m, float* __restrict__ best,
-                std::uint32_t* __restrict__ from, std::uint8_t* __restrict__ in_tree,
-                std::uint32_t* __restrict__ ou, std::uint32_t* __restrict__ ov,
-                float* __restrict__ ow, std::uint32_t* __restrict__ out_count) {
+__device__ void prim_run(const float* __restrict__ mat, std::uint32_t m, float* __restrict__ best,
+                         std::uint32_t* __restrict__ from, std::uint8_t* __restrict__ in_tree,
+                         std::uint32_t* __restrict__ ou, std::uint32_t* __restrict__ ov,
+                         float* __restrict__ ow, std::uint32_t* __restrict__ out_count) {
     __shared__ float sv[4];
     __shared__ std::uint32_t si[4];
     for (std::uint32_t j = threadIdx.x; j < m; j += 256) {
@@ -173,6 +172,98 @@ __global__ void __launch_bounds__(256)
         current = li;
     }
     if (threadIdx.x == 0) *out_count = t;
+}
+
+__global__ void __launch_bounds__(256)
+    prim_kernel(const float* __restrict__ mat, std::uint32_t m, float* __restrict__ best,
+                std::uint32_t* __restrict__ from, std::uint8_t* __restrict__ in_tree,
+                std::uint32_t* __restrict__ ou, std::uint32_t* __restrict__ ov,
+                float* __restrict__ ow, std::uint32_t* __restrict__ out_count) {
+    prim_run(mat, m, best, from, in_tree, ou, ov, ow, out_count);
+}
+
+// Several subsets at once: subset i's points are rows [so, so + m) of the
+// gathered arrays, its matrix sits at mat_off, its tiles at tile_off.
+struct SubsetMeta {
+    std::uint32_t so, m, tile_off, tiles_x;
+    std::uint64_t mat_off;
+};
+
+__device__ __forceinline__ std::uint32_t subset_of_tile(const SubsetMeta* meta, std::uint32_t nsets,
+                                                        std::uint32_t tile) {
+    std::uint32_t lo = 0, hi = nsets;  // largest i with meta[i].tile_off <= tile
+    while (hi - lo > 1) {
+        const std::uint32_t mid = (lo + hi) / 2;
+        if (meta[mid].tile_off <= tile) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256)
+    mrd_tiles_batched_kernel(const float* __restrict__ sub, std::uint32_t stride,
+                             const float* __restrict__ core, const SubsetMeta* __restrict__ meta,
+                             std::uint32_t nsets, float* __restrict__ mat) {
+    extern __shared__ float lds[];
+    float* A = lds;
+    float* B = lds + kTile * stride;
+    const SubsetMeta M = meta[subset_of_tile(meta, nsets, blockIdx.x)];
+    const std::uint32_t t = blockIdx.x - M.tile_off;
+    const std::uint32_t row_base = (t / M.tiles_x) * kTile, col_base = (t % M.tiles_x) * kTile;
+    const float* pts = sub + std::size_t(M.so) * stride;
+    const float* cr = core + M.so;
+    const int tx = threadIdx.x & (kTile - 1), ty = threadIdx.x / kTile;
+    for (std::uint32_t e = threadIdx.x; e < kTile * stride; e += 256) {
+        const std::uint32_t rr = e / stride, dd = e - rr * stride;
+        const std::uint32_t gi = row_base + rr, gj = col_base + rr;
+        A[e] = gi < M.m ? pts[std::size_t(gi) * stride + dd] : 0.0f;
+        B[e] = gj < M.m ? pts[std::size_t(gj) * stride + dd] : 0.0f;
+    }
+    __syncthreads();
+    const std::uint32_t gr = row_base + ty, gc = col_base + tx;
+    if (gr >= M.m || gc >= M.m) return;
+    float value = 0.0f;
+    if (gr != gc) {
+        const float d = sqrt_exact(l2sqr_exact(A + ty * stride, B + tx * stride, stride));
+        value = fmax_ref(d, fmax_ref(cr[gr], cr[gc]));
+    }
+    mat[M.mat_off + std::size_t(gr) * M.m + gc] = value;
+}
+
+__global__ void __launch_bounds__(256)
+    prim_batched_kernel(const float* __restrict__ mat, const SubsetMeta* __restrict__ meta,
+                        float* __restrict__ best, std::uint32_t* __restrict__ from,
+                        std::uint8_t* __restrict__ in_tree, std::uint32_t* __restrict__ ou,
+                        std::uint32_t* __restrict__ ov, float* __restrict__ ow,
+                        std::uint32_t* __restrict__ counts) {
+    const SubsetMeta M = meta[blockIdx.x];
+    if (M.m < 2) {
+        if (threadIdx.x == 0) counts[blockIdx.x] = 0;
+        return;
+    }
+    prim_run(mat + M.mat_off, M.m, best + M.so, from + M.so, in_tree + M.so, ou + M.so,
+             ov + M.so, ow + M.so, counts + blockIdx.x);
+}
+
+// Base weights and global ids of every subset's accepted edges (a block each).
+__global__ void __launch_bounds__(256)
+    base_weight_batched_kernel(const float* __restrict__ sub, std::uint32_t stride,
+                               const std::uint32_t* __restrict__ ids,
+                               const SubsetMeta* __restrict__ meta,
+                               const std::uint32_t* __restrict__ counts,
+                               std::uint32_t* __restrict__ ou, std::uint32_t* __restrict__ ov,
+                               float* __restrict__ ow) {
+    const SubsetMeta M = meta[blockIdx.x];
+    const std::uint32_t c = counts[blockIdx.x];
+    for (std::uint32_t t = threadIdx.x; t < c; t += blockDim.x) {
+        const std::uint32_t e = M.so + t;
+        const std::uint32_t a = M.so + ou[e], b = M.so + ov[e];
+        ow[e] = sqrt_exact(l2sqr_exact(sub + std::size_t(a) * stride,
+                                       sub + std::size_t(b) * stride, stride));
+        const std::uint32_t ga = ids[a], gb = ids[b];
+        ou[e] = ga < gb ? ga : gb;
+        ov[e] = ga < gb ? gb : ga;
+    }
 }
 
 __global__ void subset_gather_kernel(const float* __restrict__ pts, std::uint32_t stride,
@@ -347,6 +438,85 @@ void mst_over_subset(Graph& g, const std::vector<std::uint32_t>& ids,
         CSG_CHECK(hipMemcpyAsync(w->data() + base, ow.p, std::size_t(t) * 4,
                                  hipMemcpyDeviceToHost, s));
         CSG_CHECK(hipStreamSynchronize(s));
+    }
+}
+
+void mst_over_subsets(Graph& g, const std::vector<std::vector<std::uint32_t>>& sets,
+                      std::vector<std::uint32_t>* u, std::vector<std::uint32_t>* v,
+                      std::vector<float>* w) {
+    const std::uint32_t stride = g.stride;
+    if (stride > kMaxLdsStride) {  // rows too long for the LDS tiles: one by one
+        for (const auto& ids : sets) mst_over_subset(g, ids, u, v, w);
+        return;
+    }
+    hipStream_t s = g.stream;
+    const std::uint32_t nsets = static_cast<std::uint32_t>(sets.size());
+    if (nsets == 0) return;
+    std::vector<SubsetMeta> meta(nsets);
+    std::vector<std::uint32_t> all;
+    std::uint64_t mat_total = 0;
+    std::uint32_t tiles = 0;
+    for (std::uint32_t i = 0; i < nsets; ++i) {
+        const std::uint32_t m = static_cast<std::uint32_t>(sets[i].size());
+        const std::uint32_t tx = (m + kTile - 1) / kTile;
+        meta[i] = SubsetMeta{static_cast<std::uint32_t>(all.size()), m, tiles, tx, mat_total};
+        all.insert(all.end(), sets[i].begin(), sets[i].end());
+        mat_total += std::uint64_t(m) * m;
+        tiles += m >= 2 ? tx * tx : 0;
+        if (m < 2) meta[i].tiles_x = 0;
+    }
+    const std::uint32_t M = static_cast<std::uint32_t>(all.size());
+    if (M == 0) return;
+    DBuf<std::uint32_t> d_ids, from, ou, ov, cnt;
+    DBuf<float> sub, core, mat, best, ow;
+    DBuf<std::uint8_t> in_tree;
+    DBuf<SubsetMeta> d_meta;
+    d_ids.reserve(M);
+    sub.reserve(std::size_t(M) * stride);
+    core.reserve(M);
+    mat.reserve(mat_total ? mat_total : 1);
+    best.reserve(M);
+    from.reserve(M);
+    in_tree.reserve(M);
+    ou.reserve(M);
+    ov.reserve(M);
+    ow.reserve(M);
+    cnt.reserve(nsets);
+    d_meta.reserve(nsets);
+    CSG_CHECK(hipMemcpyAsync(d_ids.p, all.data(), std::size_t(M) * 4, hipMemcpyHostToDevice, s));
+    CSG_CHECK(hipMemcpyAsync(d_meta.p, meta.data(), sizeof(SubsetMeta) * nsets,
+                             hipMemcpyHostToDevice, s));
+    subset_gather_kernel<<<capped_blocks(std::size_t(M) * stride, 256), 256, 0, s>>>(
+        g.pts.p, stride, d_ids.p, M, g.dist.p, g.count.p, g.k, sub.p, core.p);
+    CSG_LAUNCH_CHECK();
+    if (tiles) {
+        ScopedKernelTimer t(g, "bridge_mrd_matrix");
+        const std::size_t lds = std::size_t(2) * kTile * stride * sizeof(float);
+        mrd_tiles_batched_kernel<<<tiles, 256, lds, s>>>(sub.p, stride, core.p, d_meta.p, nsets,
+                                                         mat.p);
+        CSG_LAUNCH_CHECK();
+    }
+    {
+        ScopedKernelTimer t(g, "bridge_prim");
+        prim_batched_kernel<<<nsets, 256, 0, s>>>(mat.p, d_meta.p, best.p, from.p, in_tree.p,
+                                                  ou.p, ov.p, ow.p, cnt.p);
+        CSG_LAUNCH_CHECK();
+    }
+    base_weight_batched_kernel<<<nsets, 256, 0, s>>>(sub.p, stride, d_ids.p, d_meta.p, cnt.p,
+                                                     ou.p, ov.p, ow.p);
+    CSG_LAUNCH_CHECK();
+    std::vector<std::uint32_t> hc(nsets), hu(M), hv(M);
+    std::vector<float> hw(M);
+    CSG_CHECK(hipMemcpyAsync(hc.data(), cnt.p, std::size_t(nsets) * 4, hipMemcpyDeviceToHost, s));
+    CSG_CHECK(hipMemcpyAsync(hu.data(), ou.p, std::size_t(M) * 4, hipMemcpyDeviceToHost, s));
+    CSG_CHECK(hipMemcpyAsync(hv.data(), ov.p, std::size_t(M) * 4, hipMemcpyDeviceToHost, s));
+    CSG_CHECK(hipMemcpyAsync(hw.data(), ow.p, std::size_t(M) * 4, hipMemcpyDeviceToHost, s));
+    CSG_CHECK(hipStreamSynchronize(s));
+    for (std::uint32_t i = 0; i < nsets; ++i) {
+        const std::uint32_t so = meta[i].so;
+        u->insert(u->end(), hu.begin() + so, hu.begin() + so + hc[i]);
+        v->insert(v->end(), hv.begin() + so, hv.begin() + so + hc[i]);
+        w->insert(w->end(), hw.begin() + so, hw.begin() + so + hc[i]);
     }
 }
 
