@@ -477,34 +477,78 @@ __device__ __forceinline__ void join_propose(const JoinArgs& a, const std::uint3
     }
 }
 
+// Proposals of one node from its pair distances (the reference's try_pair
+// test, nndescent.cpp:41-44, both ends): pair p = (i, j) with distance d
+// proposes j to i when d < thr_i and i to j when d < thr_j. The distance
+// pass has counted, per member, the proposals it receives (LDS atomics into
+// s_cnt); one global atomic per receiving member reserves its range of the
+// round's buffer, then the pairs are visited again and each key is written at
+// the member's base plus an LDS cursor. Keys past a member's capacity in this
+// round are dropped.
+__device__ __forceinline__ void join_emit(const JoinArgs& a, const std::uint32_t* s_id,
+                                          const float* s_thr, const float* s_Dp,
+                                          std::uint32_t* s_cnt, std::uint32_t* s_base,
+                                          const PairMap& pm, std::uint32_t npair,
+                                          std::uint32_t ns) {
+    const std::uint32_t tid = threadIdx.x;
+    for (std::uint32_t x = tid; x < ns; x += kJoinThreads) {
+        const std::uint32_t c = s_cnt[x];
+        s_base[x] = c ? atomicAdd(&a.fill[s_id[x]], c) : a.cap;
+        s_cnt[x] = 0;  // now the write cursor
+    }
+    __syncthreads();
+    for (std::uint32_t p = tid; p < npair; p += kJoinThreads) {
+        std::uint32_t i, j;
+        if (!pm(p, &i, &j)) continue;
+        const float d = s_Dp[p];
+        const bool to_i = d < s_thr[i], to_j = d < s_thr[j];
+        if (!(to_i || to_j)) continue;
+        const unsigned long long hb = static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32;
+        if (to_i) {
+            const std::uint32_t pos = s_base[i] + atomicAdd(&s_cnt[i], 1u);
+            if (pos < a.cap) a.cand[std::size_t(s_id[i]) * a.cap + pos] = hb | s_id[j];
+        }
+        if (to_j) {
+            const std::uint32_t pos = s_base[j] + atomicAdd(&s_cnt[j], 1u);
+            if (pos < a.cap) a.cand[std::size_t(s_id[j]) * a.cap + pos] = hb | s_id[i];
+        }
+    }
+}
+
 // Exact local join: every (new i, member j) pair's distance is computed in
 // the reference's exact summation order (both rows from LDS), the pairs
-// spread densely over the threads.
+// spread densely over the threads; each pair's distance is kept by pair index
+// and its proposals counted on the spot. LDS: ids[S], thr[S], cnt[S],
+// base[S], rows[S*stride], distances[A*S].
 template <std::uint32_t STRIDE>
 __global__ void __launch_bounds__(kJoinThreads) join_kernel(JoinArgs a) {
     extern __shared__ float lds[];
-    const std::uint32_t S = 6u * a.smp, P = S + 1;
+    const std::uint32_t S = 6u * a.smp;
     const std::uint32_t stride = STRIDE ? STRIDE : a.stride;
     std::uint32_t* s_id = reinterpret_cast<std::uint32_t*>(lds);
     float* s_thr = lds + S;
-    float* s_rows = lds + 2 * S;
-    float* s_D = s_rows + std::size_t(S) * stride;
+    std::uint32_t* s_cnt = reinterpret_cast<std::uint32_t*>(lds + 2 * S);
+    std::uint32_t* s_base = s_cnt + S;
+    float* s_rows = lds + 4 * S;
+    float* s_Dp = s_rows + std::size_t(S) * stride;
     const std::uint32_t tid = threadIdx.x;
     for (std::uint32_t v = blockIdx.x; v < a.n; v += gridDim.x) {
         std::uint32_t na, ns;
+        for (std::uint32_t x = tid; x < S; x += kJoinThreads) s_cnt[x] = 0;
         if (!join_stage(a, v, S, stride, s_id, s_thr, s_rows, &na, &ns)) continue;
         const PairMap pm(ns, na);
-        for (std::uint32_t p = tid; p < na * ns; p += kJoinThreads) {
+        const std::uint32_t npair = na * ns;
+        for (std::uint32_t p = tid; p < npair; p += kJoinThreads) {
             std::uint32_t i, j;
             if (!pm(p, &i, &j)) continue;
             const float d = l2sqr_exact(s_rows + std::size_t(i) * stride,
                                         s_rows + std::size_t(j) * stride, stride);
-            s_D[i * P + j] = d;
-            if (j < na) s_D[j * P + i] = d;
+            s_Dp[p] = d;
+            if (d < s_thr[i]) atomicAdd(&s_cnt[i], 1u);
+            if (d < s_thr[j]) atomicAdd(&s_cnt[j], 1u);
         }
-        for (std::uint32_t i = tid; i < na; i += kJoinThreads) s_D[i * P + i] = __builtin_inff();
         __syncthreads();
-        join_propose(a, s_id, s_thr, s_D, P, na, ns);
+        join_emit(a, s_id, s_thr, s_Dp, s_cnt, s_base, pm, npair, ns);
         __syncthreads();
     }
 }
@@ -1128,9 +1172,11 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
                            : gram == kGramF16 ? pick_join_gram_t<false>(stride)
                                               : pick_join_gram_t<true>(stride);
     const std::size_t lds =
-        (2 * std::size_t(S) + std::size_t(S) * stride + std::size_t(A) * (S + 1) +
-         (gram == kGramExact ? 0 : 2 * std::size_t(S))) *
-        sizeof(float);
+        gram == kGramExact
+            ? (4 * std::size_t(S) + std::size_t(S) * stride + std::size_t(A) * S) * sizeof(float)
+            : (2 * std::size_t(S) + std::size_t(S) * stride + std::size_t(A) * (S + 1) +
+               2 * std::size_t(S)) *
+                  sizeof(float);
     const unsigned join_threads = kJoinThreads;
     const unsigned join_grid = static_cast<unsigned>(std::min<std::size_t>(n, 65536));
     if (lds > 150 * 1024) throw std::runtime_error("nndescent: candidate tile exceeds LDS");
