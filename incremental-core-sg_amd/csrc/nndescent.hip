@@ -398,85 +398,6 @@ struct PairMap {
     }
 };
 
-// Proposals from the completed tile (the reference's try_pair test,
-// nndescent.cpp:41-44): new member x gets every member y with D(x,y) < thr_x
-// (row targets), old member x gets every new member y with D(y,x) < thr_x
-// (column targets). Wavefront w takes the rows x = w (mod 2) -- one ballot
-// per row chunk -- and one chunk of 64 columns, a lane per column. All counts
-// are taken first, one batch of atomics reserves the targets' buffer ranges,
-// then the tile is read again to write the keys (distance bits << 32 | id).
-// Proposals at or beyond a target's capacity in this round are dropped.
-__device__ __forceinline__ void join_propose(const JoinArgs& a, const std::uint32_t* s_id,
-                                             const float* s_thr, const float* s_D,
-                                             std::uint32_t P, std::uint32_t na,
-                                             std::uint32_t ns) {
-    const std::uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-    const std::uint64_t below = (1ull << lane) - 1ull;
-    std::uint32_t rcount = 0;
-    for (std::uint32_t x = w; x < na; x += 2) {
-        const float thr = s_thr[x];
-        std::uint32_t c = 0;
-        for (std::uint32_t y0 = 0; y0 < ns; y0 += 64) {
-            const std::uint32_t y = y0 + lane;
-            c += __popcll(__ballot(y < ns && s_D[x * P + y] < thr));
-        }
-        if (lane == (x >> 1)) rcount = c;
-    }
-    const std::uint32_t xc = na + 64u * w + lane;
-    const bool has_col = xc < ns;
-    const float thr_c = has_col ? s_thr[xc] : -1.0f;
-    std::uint32_t ccount = 0;
-    if (has_col)
-        for (std::uint32_t y = 0; y < na; ++y) ccount += s_D[y * P + xc] < thr_c ? 1u : 0u;
-    // reserve
-    const std::uint32_t xr = 2u * lane + w;
-    std::uint32_t rbase = a.cap, cbase = a.cap;
-    if (rcount) rbase = atomicAdd(&a.fill[s_id[xr]], rcount);
-    if (ccount) cbase = atomicAdd(&a.fill[s_id[xc]], ccount);
-    // write row targets
-    for (std::uint32_t x = w; x < na; x += 2) {
-        const float thr = s_thr[x];
-        std::uint32_t pos0 = __shfl(rbase, static_cast<int>(x >> 1), 64);
-        unsigned long long* tgt = a.cand + std::size_t(s_id[x]) * a.cap;
-        for (std::uint32_t y0 = 0; y0 < ns; y0 += 64) {
-            const std::uint32_t y = y0 + lane;
-            const float d = y < ns ? s_D[x * P + y] : __builtin_inff();
-            const bool pass = d < thr;
-            const std::uint64_t m = __ballot(pass);
-            const std::uint32_t pos = pos0 + __popcll(m & below);
-            if (pass && pos < a.cap)
-                tgt[pos] = (static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32) | s_id[y];
-            pos0 += __popcll(m);
-        }
-    }
-    // write column targets
-    if (ccount) {
-        unsigned long long* tgt = a.cand + std::size_t(s_id[xc]) * a.cap;
-        std::uint32_t pos = cbase;
-        for (std::uint32_t y = 0; y < na && pos < a.cap; ++y) {
-            const float d = s_D[y * P + xc];
-            if (!(d < thr_c)) continue;
-            tgt[pos++] = (static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32) | s_id[y];
-        }
-    }
-    // columns beyond the first 128 (sample > 21 only)
-    for (std::uint32_t xb = na + 128u + 64u * w; xb < ns; xb += 128u) {
-        const std::uint32_t x = xb + lane;
-        if (x >= ns) continue;
-        const float thr = s_thr[x];
-        std::uint32_t c = 0;
-        for (std::uint32_t y = 0; y < na; ++y) c += s_D[y * P + x] < thr ? 1u : 0u;
-        if (c == 0) continue;
-        std::uint32_t pos = atomicAdd(&a.fill[s_id[x]], c);
-        unsigned long long* tgt = a.cand + std::size_t(s_id[x]) * a.cap;
-        for (std::uint32_t y = 0; y < na && pos < a.cap; ++y) {
-            const float d = s_D[y * P + x];
-            if (!(d < thr)) continue;
-            tgt[pos++] = (static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32) | s_id[y];
-        }
-    }
-}
-
 // Proposals of one node from its pair distances (the reference's try_pair
 // test, nndescent.cpp:41-44, both ends): pair p = (i, j) with distance d
 // proposes j to i when d < thr_i and i to j when d < thr_j. The distance
@@ -485,11 +406,12 @@ __device__ __forceinline__ void join_propose(const JoinArgs& a, const std::uint3
 // round's buffer, then the pairs are visited again and each key is written at
 // the member's base plus an LDS cursor. Keys past a member's capacity in this
 // round are dropped.
+// Distances are read as s_D[p] (pitch 0) or from a tile, s_D[i * pitch + j].
 __device__ __forceinline__ void join_emit(const JoinArgs& a, const std::uint32_t* s_id,
-                                          const float* s_thr, const float* s_Dp,
-                                          std::uint32_t* s_cnt, std::uint32_t* s_base,
-                                          const PairMap& pm, std::uint32_t npair,
-                                          std::uint32_t ns) {
+                                          const float* s_thr, const float* s_D,
+                                          std::uint32_t pitch, std::uint32_t* s_cnt,
+                                          std::uint32_t* s_base, const PairMap& pm,
+                                          std::uint32_t npair, std::uint32_t ns) {
     const std::uint32_t tid = threadIdx.x;
     for (std::uint32_t x = tid; x < ns; x += kJoinThreads) {
         const std::uint32_t c = s_cnt[x];
@@ -500,7 +422,7 @@ __device__ __forceinline__ void join_emit(const JoinArgs& a, const std::uint32_t
     for (std::uint32_t p = tid; p < npair; p += kJoinThreads) {
         std::uint32_t i, j;
         if (!pm(p, &i, &j)) continue;
-        const float d = s_Dp[p];
+        const float d = s_D[pitch ? i * pitch + j : p];
         const bool to_i = d < s_thr[i], to_j = d < s_thr[j];
         if (!(to_i || to_j)) continue;
         const unsigned long long hb = static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32;
@@ -548,7 +470,7 @@ __global__ void __launch_bounds__(kJoinThreads) join_kernel(JoinArgs a) {
             if (d < s_thr[j]) atomicAdd(&s_cnt[j], 1u);
         }
         __syncthreads();
-        join_emit(a, s_id, s_thr, s_Dp, s_cnt, s_base, pm, npair, ns);
+        join_emit(a, s_id, s_thr, s_Dp, 0, s_cnt, s_base, pm, npair, ns);
         __syncthreads();
     }
 }
@@ -648,7 +570,8 @@ __device__ __forceinline__ unsigned int wave_max_u32(unsigned int v) {
 }
 
 // LDS: ids[S], thr[S], rows[S*stride], D[A][S+1], N[S] (squared norms of the
-// rounded centred rows), nb[S] (outward bound on both norms of a row).
+// rounded centred rows), nb[S] (outward bound on both norms of a row), cnt[S],
+// base[S].
 template <std::uint32_t STRIDE, bool BF16>
 __global__ void __launch_bounds__(kJoinThreads) join_gram_kernel(JoinArgs a) {
     extern __shared__ float lds[];
@@ -661,6 +584,8 @@ __global__ void __launch_bounds__(kJoinThreads) join_gram_kernel(JoinArgs a) {
     float* s_D = s_rows + std::size_t(S) * stride;
     float* s_N = s_D + std::size_t(A) * P;
     float* s_nb = s_N + S;
+    std::uint32_t* s_cnt = reinterpret_cast<std::uint32_t*>(s_nb + S);
+    std::uint32_t* s_base = s_cnt + S;
     const std::uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
     const float rho = GramConst<BF16>::rho;
     const float abs_norm = GramConst<BF16>::absc * sqrtf(float(KP)) * 1.01f;
@@ -671,6 +596,7 @@ __global__ void __launch_bounds__(kJoinThreads) join_gram_kernel(JoinArgs a) {
     unsigned int worst_rel = 0, worst_ratio = 0;
     for (std::uint32_t v = blockIdx.x; v < a.n; v += gridDim.x) {
         std::uint32_t na, ns;
+        for (std::uint32_t x = tid; x < S; x += kJoinThreads) s_cnt[x] = 0;
         if (!join_stage(a, v, S, stride, s_id, s_thr, s_rows, &na, &ns)) continue;
         // 1. squared norms of the rounded centred rows and their outward bound
         for (std::uint32_t j = tid; j < ns; j += kJoinThreads) {
@@ -734,7 +660,8 @@ __global__ void __launch_bounds__(kJoinThreads) join_gram_kernel(JoinArgs a) {
             const float* rj = s_rows + std::size_t(j) * stride;
             const float d = reject ? __builtin_inff() : l2sqr_exact(ri, rj, stride);
             s_D[i * P + j] = d;
-            if (j < na) s_D[j * P + i] = d;
+            if (d < s_thr[i]) atomicAdd(&s_cnt[i], 1u);
+            if (d < s_thr[j]) atomicAdd(&s_cnt[j], 1u);
             if (a.tally) {
                 ++n_pairs;
                 n_exact += reject ? 0u : 1u;
@@ -755,9 +682,8 @@ __global__ void __launch_bounds__(kJoinThreads) join_gram_kernel(JoinArgs a) {
                 }
             }
         }
-        for (std::uint32_t i = tid; i < na; i += kJoinThreads) s_D[i * P + i] = __builtin_inff();
         __syncthreads();
-        join_propose(a, s_id, s_thr, s_D, P, na, ns);
+        join_emit(a, s_id, s_thr, s_D, P, s_cnt, s_base, pm, na * ns, ns);
         __syncthreads();
     }
     if (a.tally) {
@@ -1175,7 +1101,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         gram == kGramExact
             ? (4 * std::size_t(S) + std::size_t(S) * stride + std::size_t(A) * S) * sizeof(float)
             : (2 * std::size_t(S) + std::size_t(S) * stride + std::size_t(A) * (S + 1) +
-               2 * std::size_t(S)) *
+               4 * std::size_t(S)) *
                   sizeof(float);
     const unsigned join_threads = kJoinThreads;
     const unsigned join_grid = static_cast<unsigned>(std::min<std::size_t>(n, 65536));
