@@ -203,15 +203,21 @@ __global__ void __launch_bounds__(256)
         const bool take_new = isnew && rn < max_new && id < n;
         const bool take_old = isold && ro < max_old && id < n;
         if (isnew && rn < max_new) flags[base + lane] = 0;
+        // bounded reverse lists: a full list takes no more (the reference
+        // checks the count before its fetch_add, nndescent.cpp ReverseCache)
         if (take_new) {
             new_list[std::size_t(v) * max_new + rn] = id;
-            const std::uint32_t pos = atomicAdd(&rev_new_cnt[id], 1u);
-            if (pos < smp) rev_new[std::size_t(id) * smp + pos] = v;
+            if (__atomic_load_n(&rev_new_cnt[id], __ATOMIC_RELAXED) < smp) {
+                const std::uint32_t pos = atomicAdd(&rev_new_cnt[id], 1u);
+                if (pos < smp) rev_new[std::size_t(id) * smp + pos] = v;
+            }
         }
         if (take_old) {
             old_list[std::size_t(v) * max_old + ro] = id;
-            const std::uint32_t pos = atomicAdd(&rev_old_cnt[id], 1u);
-            if (pos < smp) rev_old[std::size_t(id) * smp + pos] = v;
+            if (__atomic_load_n(&rev_old_cnt[id], __ATOMIC_RELAXED) < smp) {
+                const std::uint32_t pos = atomicAdd(&rev_old_cnt[id], 1u);
+                if (pos < smp) rev_old[std::size_t(id) * smp + pos] = v;
+            }
         }
         if (lane == 0) {
             const std::uint32_t tn = __popcll(mnew), to = __popcll(mold);
