@@ -206,30 +206,39 @@ __global__ void core_init_kernel(const float* __restrict__ dist,
 // ---- symmetric adjacency (built once per CORE-SG) --------------------------
 
 __global__ void __launch_bounds__(kThreads)
-    adj_degree_kernel(const std::uint32_t* __restrict__ eu, const std::uint32_t* __restrict__ ev,
-                      std::uint64_t m, std::uint32_t* __restrict__ deg) {
+    adj_degree_kernel(const std::uint32_t* __restrict__ ev, std::uint64_t m,
+                      std::uint32_t* __restrict__ deg) {
     for (std::uint64_t e = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
-         e += std::uint64_t(gridDim.x) * blockDim.x) {
-        atomicAdd(&deg[eu[e]], 1u);
+         e += std::uint64_t(gridDim.x) * blockDim.x)
         atomicAdd(&deg[ev[e]], 1u);
-    }
+}
+
+// Degree of every vertex as first endpoint: its segment of the sorted list.
+__global__ void __launch_bounds__(kThreads)
+    first_degree_kernel(const std::uint32_t* __restrict__ eoff, std::uint32_t n,
+                        std::uint32_t* __restrict__ deg) {
+    for (std::uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < n;
+         x += gridDim.x * blockDim.x)
+        deg[x] = eoff[x + 1] - eoff[x];
 }
 
 __global__ void __launch_bounds__(kThreads)
     adj_fill_kernel(const std::uint32_t* __restrict__ eu, const std::uint32_t* __restrict__ ev,
                     const float* __restrict__ ew, std::uint64_t m,
-                    const std::uint32_t* __restrict__ off, std::uint32_t* __restrict__ cursor,
-                    std::uint32_t* __restrict__ nbr, std::uint32_t* __restrict__ rank,
-                    float* __restrict__ w) {
+                    const std::uint32_t* __restrict__ eoff, const std::uint32_t* __restrict__ off,
+                    std::uint32_t* __restrict__ cursor, std::uint32_t* __restrict__ nbr,
+                    std::uint32_t* __restrict__ rank, float* __restrict__ w) {
     for (std::uint64_t e = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
          e += std::uint64_t(gridDim.x) * blockDim.x) {
         const std::uint32_t u = eu[e], v = ev[e];
         const float we = ew[e];
-        const std::uint32_t pu = off[u] + atomicAdd(&cursor[u], 1u);
+        // first-endpoint entries keep the sorted list's order (no atomic);
+        // second-endpoint entries follow them, placed by a cursor
+        const std::uint32_t pu = off[u] + static_cast<std::uint32_t>(e - eoff[u]);
         nbr[pu] = v;
         rank[pu] = static_cast<std::uint32_t>(e);
         w[pu] = we;
-        const std::uint32_t pv = off[v] + atomicAdd(&cursor[v], 1u);
+        const std::uint32_t pv = off[v] + (eoff[v + 1] - eoff[v]) + atomicAdd(&cursor[v], 1u);
         nbr[pv] = u | kSecondEnd;
         rank[pv] = static_cast<std::uint32_t>(e);
         w[pv] = we;
@@ -320,9 +329,10 @@ void prepare_edges(Graph& g) {
     // Symmetric adjacency for the vertex-centric Boruvka rounds.
     g.adj_off.reserve(std::size_t(n) + 1);
     fill_u32(g.adj_off.p, std::size_t(n) + 1, 0u, s);
+    first_degree_kernel<<<capped_blocks(n, kThreads), kThreads, 0, s>>>(g.eoff.p, n, g.adj_off.p);
+    CSG_LAUNCH_CHECK();
     if (m) {
-        adj_degree_kernel<<<capped_blocks(m, kThreads), kThreads, 0, s>>>(g.eu.p, g.ev.p, m,
-                                                                          g.adj_off.p);
+        adj_degree_kernel<<<capped_blocks(m, kThreads), kThreads, 0, s>>>(g.ev.p, m, g.adj_off.p);
         CSG_LAUNCH_CHECK();
     }
     exclusive_scan_u32(g.adj_off.p, std::size_t(n) + 1, nullptr, g.scan_tmp.p, s);
@@ -332,8 +342,8 @@ void prepare_edges(Graph& g) {
     if (m) {
         fill_u32(cursor.p, n, 0u, s);
         adj_fill_kernel<<<capped_blocks(m, kThreads), kThreads, 0, s>>>(
-            g.eu.p, g.ev.p, g.ew.p, m, g.adj_off.p, cursor.p, g.adj_nbr.p, g.adj_rank.p,
-            g.adj_w.p);
+            g.eu.p, g.ev.p, g.ew.p, m, g.eoff.p, g.adj_off.p, cursor.p, g.adj_nbr.p,
+            g.adj_rank.p, g.adj_w.p);
         CSG_LAUNCH_CHECK();
     }
     CSG_CHECK(hipStreamSynchronize(s));
