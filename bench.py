@@ -382,8 +382,8 @@ def main():
     plain_s = float(np.mean(plain_s))
 
     kernels = {}
-    for name in ("nnd_join", "nnd_gather", "nnd_sample", "nnd_merge", "boruvka_round1", "boruvka_round2",
-                 "boruvka_round_live", "hub_slot_pass", "hub_score", "bridge_mrd_matrix",
+    for name in ("nnd_init", "nnd_join", "nnd_gather", "nnd_sample", "nnd_merge", "boruvka_round1",
+                 "boruvka_rounds_graph", "hub_slot_pass", "hub_score", "bridge_mrd_matrix",
                  "bridge_prim"):
         ms, cnt = g.kernel_time(name)
         if cnt:

@@ -21,6 +21,7 @@ const char* get_last_error() { return t_last_error.c_str(); }
 
 Graph::~Graph() {
     resolve_timers();
+    if (sweep_exec) (void)hipGraphExecDestroy(sweep_exec);
     if (stream) {
         (void)hipSetDevice(device);
         (void)hipStreamDestroy(stream);

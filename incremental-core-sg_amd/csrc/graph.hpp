@@ -115,7 +115,11 @@ struct Graph {
     // mpts sweep: per-block per-slot partials, per-slot results, slot mpts,
     // and one slot's outputs
     DBuf<std::uint64_t> slot_part, slot_out;
-    DBuf<std::uint32_t> slot_mpts, label_buf, out_u, out_v;
+    DBuf<std::uint32_t> slot_mpts, label_buf, out_u, out_v, slot_rounds;
+    // the captured Boruvka rounds (replayed while slots are live) and the
+    // launch parameters it was captured with
+    hipGraphExec_t sweep_exec = nullptr;
+    std::vector<unsigned char> sweep_key;
     DBuf<float> out_w;
 
     DescentBuffers nnd;

@@ -44,14 +44,16 @@ constexpr unsigned kThreads = 256;
 constexpr std::uint32_t kMaxSlots = 32;
 constexpr unsigned kReduceBlocks = 1024;  // blocks of the per-slot reducing kernels
 
-constexpr std::uint32_t kErrEdgeIndex = 1u, kErrHookCycle = 2u;
+constexpr std::uint32_t kErrEdgeIndex = 1u, kErrHookCycle = 2u, kErrHookCount = 4u,
+                        kErrTooManyEdges = 8u;
+constexpr std::uint32_t kGraphRounds = 4;  // Boruvka rounds per replay of the captured graph
 constexpr std::uint64_t kOver = 1ull << 63;  // saturated checked sum
 constexpr std::uint64_t kExpInf = 1ull << 32;
 
 struct Sweep {
     std::uint32_t n, k, B, Bp;  // points, list length, slots, slots rounded up to a power of 2
     std::uint64_t m;            // CORE-SG edges
-    std::uint32_t live;         // bit b: slot b still running
+    const std::uint32_t* live;  // device word, bit b: slot b still running
     const std::uint32_t* off;   // adjacency
     const std::uint32_t* nbr;
     const std::uint32_t* rank;
@@ -191,9 +193,11 @@ __global__ void __launch_bounds__(kThreads) sweep_round_kernel(Sweep s) {
     const std::uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const std::uint32_t waves = (gridDim.x * blockDim.x) / kWave;
     const std::uint64_t gmask = (W == 64) ? ~0ull : (((1ull << W) - 1ull) << (grp * W));
+    const std::uint32_t live = *s.live;
+    if (live == 0) return;
     for (std::uint32_t vb = wave * per_wave; vb < s.n; vb += waves * per_wave) {
         const std::uint32_t v = vb + grp;
-        const bool slot = v < s.n && b < s.B && ((s.live >> b) & 1u);
+        const bool slot = v < s.n && b < s.B && ((live >> b) & 1u);
         const std::size_t i = std::size_t(v) * s.B + b;
         const bool act = slot && (FIRST || s.active[i] != 0);
         std::uint64_t key_min = kNone64;
@@ -265,6 +269,8 @@ __global__ void __launch_bounds__(kThreads) sweep_round_kernel(Sweep s) {
 // every hook adds its edge to the tree exactly once. Hooks per slot are
 // counted into per-block partials.
 __global__ void __launch_bounds__(kThreads) sweep_hook_kernel(Sweep s) {
+    const std::uint32_t live = *s.live;
+    if (live == 0) return;  // block-uniform; the partials are read for live slots only
     const SlotStride st = slot_stride(s.B);
     std::uint64_t hooks = 0;
     const std::size_t total = std::size_t(s.n) * s.B;
@@ -272,7 +278,7 @@ __global__ void __launch_bounds__(kThreads) sweep_hook_kernel(Sweep s) {
         for (std::size_t i = st.first; i < total; i += st.stride) {
             const std::uint32_t v = static_cast<std::uint32_t>(i / s.B);
             const std::uint32_t b = static_cast<std::uint32_t>(i - std::size_t(v) * s.B);
-            if (!((s.live >> b) & 1u) || s.comp[i] != v) continue;
+            if (!((live >> b) & 1u) || s.comp[i] != v) continue;
             const std::uint64_t key = s.best[i];
             if (key == kNone64) continue;
             const std::uint32_t e = static_cast<std::uint32_t>(key);
@@ -294,12 +300,14 @@ __global__ void __launch_bounds__(kThreads) sweep_hook_kernel(Sweep s) {
 
 // Final root of every hooked root (chains end at a root that did not hook).
 __global__ void __launch_bounds__(kThreads) sweep_jump_kernel(Sweep s) {
+    const std::uint32_t live = *s.live;
+    if (live == 0) return;
     const std::size_t total = std::size_t(s.n) * s.B;
     for (std::size_t i = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
          i += std::size_t(gridDim.x) * blockDim.x) {
         const std::uint32_t v = static_cast<std::uint32_t>(i / s.B);
         const std::uint32_t b = static_cast<std::uint32_t>(i - std::size_t(v) * s.B);
-        if (!((s.live >> b) & 1u) || s.comp[i] != v) continue;
+        if (!((live >> b) & 1u) || s.comp[i] != v) continue;
         // The bound only guards against a corrupted parent array.
         std::uint32_t r = v, it = 0;
         for (; it < s.n; ++it) {
@@ -313,15 +321,47 @@ __global__ void __launch_bounds__(kThreads) sweep_jump_kernel(Sweep s) {
 }
 
 __global__ void __launch_bounds__(kThreads) sweep_relabel_kernel(Sweep s) {
+    const std::uint32_t live = *s.live;
+    if (live == 0) return;
     const std::size_t total = std::size_t(s.n) * s.B;
     for (std::size_t i = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
          i += std::size_t(gridDim.x) * blockDim.x) {
         const std::uint32_t v = static_cast<std::uint32_t>(i / s.B);
         const std::uint32_t b = static_cast<std::uint32_t>(i - std::size_t(v) * s.B);
-        if (!((s.live >> b) & 1u)) continue;
+        if (!((live >> b) & 1u)) continue;
         s.comp[i] = s.newroot[std::size_t(s.comp[i]) * s.B + b];
         s.parent[i] = v;
         s.best[i] = kNone64;
+    }
+}
+
+// End of a round, one wavefront: per live slot add its hooks to the total,
+// record the round, and retire the slot when it hooked nothing or holds n - 1
+// edges (a spanning tree). Keeping this on the device lets several rounds run
+// as one captured graph without a host read in between.
+__global__ void __launch_bounds__(64)
+    sweep_live_kernel(const std::uint64_t* __restrict__ cnt, std::uint32_t B, std::uint32_t n,
+                      std::uint32_t* __restrict__ live, std::uint64_t* __restrict__ total,
+                      std::uint32_t* __restrict__ rounds, std::uint32_t* __restrict__ round_no,
+                      std::uint32_t* __restrict__ err) {
+    const std::uint32_t b = threadIdx.x;
+    const std::uint32_t mask = *live;
+    if (mask == 0) return;  // wave-uniform
+    const std::uint32_t r = *round_no + 1;
+    bool keep = false;
+    if (b < B && ((mask >> b) & 1u)) {
+        const std::uint64_t c = cnt[b];
+        if (c & kOver) atomicOr(err, kErrHookCount);
+        const std::uint64_t t = total[b] + c;
+        total[b] = t;
+        rounds[b] = r;
+        if (t >= n) atomicOr(err, kErrTooManyEdges);
+        keep = !(c == 0 || t + 1 >= n);
+    }
+    const std::uint64_t m = __ballot(keep);
+    if (b == 0) {
+        *live = static_cast<std::uint32_t>(m);
+        *round_no = r;
     }
 }
 
@@ -597,7 +637,7 @@ void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_
     g.mst_keys.reserve(nb);
     g.active.reserve(nb);
     g.slot_part.reserve(std::size_t(kReduceBlocks) * kMaxSlots);
-    g.slot_out.reserve(4 * kMaxSlots);
+    g.slot_out.reserve(4 * kMaxSlots);  // counts, exponents, sums, totals
     g.slot_mpts.reserve(kMaxSlots);
     g.counters.reserve(16);
 
@@ -616,7 +656,18 @@ void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_
     sw.B = B;
     sw.Bp = pow2_at_least(B);
     sw.m = m;
-    sw.live = (B == 32) ? 0xFFFFFFFFu : ((1u << B) - 1u);
+    // device state of the round loop: live mask, round number, per-slot
+    // totals and rounds (slot_out: [0,32) hook counts, [32,64) exponents,
+    // [64,96) sums, [96,128) totals; counters: 0 error bits, 1 live, 2 round)
+    std::uint32_t* d_live = g.counters.p + 1;
+    std::uint32_t* d_round = g.counters.p + 2;
+    std::uint64_t* d_total = g.slot_out.p + 3 * kMaxSlots;
+    DBuf<std::uint32_t>& d_rounds = g.slot_rounds;
+    d_rounds.reserve(kMaxSlots);
+    fill_u32(d_live, 1, (B == 32) ? 0xFFFFFFFFu : ((1u << B) - 1u), s);
+    fill_u64(d_total, B, 0ull, s);
+    fill_u32(d_rounds.p, B, 0u, s);
+    sw.live = d_live;
     sw.off = g.adj_off.p;
     sw.nbr = g.adj_nbr.p;
     sw.rank = g.adj_rank.p;
@@ -639,17 +690,10 @@ void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_
     sweep_init_kernel<<<eblk, kThreads, 0, s>>>(sw, g.dist.p, g.count.p, g.slot_mpts.p);
     CSG_LAUNCH_CHECK();
 
-    std::uint64_t total[kMaxSlots] = {};
-    std::uint32_t rounds[kMaxSlots] = {};
     std::uint64_t* d_cnt = g.slot_out.p;
-    for (std::uint32_t round = 1; sw.live; ++round) {
-        {
-            if (g.profiling && round == 1) g.add_work("boruvka_round1", {2 * m, n, B});
-            ScopedKernelTimer t(g, round == 1   ? "boruvka_round1"
-                                   : round == 2 ? "boruvka_round2"
-                                                : "boruvka_round_live");
-            launch_round(sw, round == 1, vblk, s);
-        }
+    // the rest of a round after its scan: hooks, their per-slot count, pointer
+    // jumping, relabelling, liveness
+    auto round_tail = [&]() {
         sweep_hook_kernel<<<kReduceBlocks, kThreads, 0, s>>>(sw);
         CSG_LAUNCH_CHECK();
         slot_final_kernel<OpAdd><<<B, kThreads, 0, s>>>(sw.part, kReduceBlocks, B, 0ull, d_cnt);
@@ -658,23 +702,61 @@ void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_
         CSG_LAUNCH_CHECK();
         sweep_relabel_kernel<<<eblk, kThreads, 0, s>>>(sw);
         CSG_LAUNCH_CHECK();
-        std::uint64_t cnt[kMaxSlots];
-        std::uint32_t err = 0;
-        CSG_CHECK(hipMemcpyAsync(cnt, d_cnt, B * 8, hipMemcpyDeviceToHost, s));
-        CSG_CHECK(hipMemcpyAsync(&err, g.counters.p, 4, hipMemcpyDeviceToHost, s));
-        CSG_CHECK(hipStreamSynchronize(s));
-        if (err)
-            throw std::runtime_error("extract: Boruvka state corrupted (error bits " +
-                                     std::to_string(err) + ")");
-        for (std::uint32_t b = 0; b < B; ++b) {
-            if (!((sw.live >> b) & 1u)) continue;
-            if (cnt[b] & kOver) throw std::runtime_error("extract: hook count overflow");
-            total[b] += cnt[b];
-            rounds[b] = round;
-            if (total[b] >= n) throw std::runtime_error("extract: more MST edges than n-1");
-            if (cnt[b] == 0 || total[b] + 1 >= n) sw.live &= ~(1u << b);
+        sweep_live_kernel<<<1, 64, 0, s>>>(d_cnt, B, n, d_live, d_total, d_rounds.p, d_round,
+                                           g.counters.p);
+        CSG_LAUNCH_CHECK();
+    };
+    // round 1 (each vertex its own component) directly
+    {
+        if (g.profiling) g.add_work("boruvka_round1", {2 * m, n, B});
+        ScopedKernelTimer t(g, "boruvka_round1");
+        launch_round(sw, true, vblk, s);
+    }
+    round_tail();
+    // later rounds: kGraphRounds of them per replay of a captured graph (the
+    // same launches every round; kernels of retired slots return at once), one
+    // host read of the live mask per replay
+    {
+        std::vector<unsigned char> key(sizeof(Sweep) + 4 * sizeof(unsigned));
+        const unsigned dims[4] = {vblk, eblk, B, kGraphRounds};
+        std::memcpy(key.data(), &sw, sizeof(Sweep));
+        std::memcpy(key.data() + sizeof(Sweep), dims, sizeof(dims));
+        if (!g.sweep_exec || g.sweep_key != key) {
+            if (g.sweep_exec) CSG_CHECK(hipGraphExecDestroy(g.sweep_exec));
+            g.sweep_exec = nullptr;
+            hipGraph_t graph = nullptr;
+            CSG_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (std::uint32_t r = 0; r < kGraphRounds; ++r) {
+                launch_round(sw, false, vblk, s);
+                round_tail();
+            }
+            CSG_CHECK(hipStreamEndCapture(s, &graph));
+            const hipError_t e = hipGraphInstantiate(&g.sweep_exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            CSG_CHECK(e);
+            g.sweep_key = key;
         }
     }
+    for (std::uint32_t replay = 0;; ++replay) {
+        std::uint32_t st[2] = {0, 0};  // error bits, live mask
+        CSG_CHECK(hipMemcpyAsync(st, g.counters.p, 8, hipMemcpyDeviceToHost, s));
+        CSG_CHECK(hipStreamSynchronize(s));
+        if (st[0])
+            throw std::runtime_error("extract: Boruvka state corrupted (error bits " +
+                                     std::to_string(st[0]) + ")");
+        if (st[1] == 0) break;
+        if (replay * kGraphRounds > 64)
+            throw std::runtime_error("extract: Boruvka did not converge");
+        {
+            ScopedKernelTimer t(g, "boruvka_rounds_graph");
+            CSG_CHECK(hipGraphLaunch(g.sweep_exec, s));
+        }
+    }
+    std::uint64_t total[kMaxSlots] = {};
+    std::uint32_t rounds[kMaxSlots] = {};
+    CSG_CHECK(hipMemcpyAsync(total, d_total, B * 8, hipMemcpyDeviceToHost, s));
+    CSG_CHECK(hipMemcpyAsync(rounds, d_rounds.p, B * 4, hipMemcpyDeviceToHost, s));
+    CSG_CHECK(hipStreamSynchronize(s));
 
     if (inst_info) {
         // per instance: edges, components, exact weight (rounds: the batch's)
