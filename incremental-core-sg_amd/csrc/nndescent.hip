@@ -42,6 +42,7 @@ namespace {
 constexpr std::uint32_t kMaxK = 64;        // list length handled by one wavefront
 constexpr std::uint32_t kMaxSample = 32;   // reference cap (cuda_common.cuh:32)
 constexpr std::uint32_t kJoinThreads = 128;
+constexpr std::uint32_t kJoinRegStride = 64;  // widest row a join thread keeps in registers
 
 // ---------------------------------------------------------------------------
 // init_random
@@ -389,18 +390,30 @@ __device__ __forceinline__ bool join_stage(const JoinArgs& a, std::uint32_t v, s
     return true;
 }
 
-// Pair p -> (i, j) = (p / ns, p % ns) over the (new i, member j) pairs; the
-// rounded-up reciprocal gives the exact quotient for p < 2^14 and ns <= 192.
-// Unordered new x new pairs are visited once (i < j).
-struct PairMap {
-    std::uint32_t ns, na;
-    std::uint64_t recip;
-    __device__ PairMap(std::uint32_t ns_, std::uint32_t na_)
-        : ns(ns_), na(na_), recip(((1ull << 24) + ns_ - 1) / ns_) {}
-    __device__ bool operator()(std::uint32_t p, std::uint32_t* i, std::uint32_t* j) const {
-        *i = static_cast<std::uint32_t>((std::uint64_t(p) * recip) >> 24);
-        *j = p - *i * ns;
-        return !(*j < na && *j <= *i);
+// A thread's walk over the (new i, member j) pairs p = i * ns + j, p = p0,
+// p0 + step, ...: (i, j) starts at (p0 / ns, p0 % ns) and advances by
+// (step / ns, step % ns) with one carry, so no pair costs a division. The
+// quotients come from a rounded-up reciprocal, exact for arguments below 2^14
+// and ns <= 192. Unordered new x new pairs are visited once (i < j).
+struct PairWalk {
+    std::uint32_t i, j, di, dj, ns, na;
+    __device__ PairWalk(std::uint32_t p0, std::uint32_t step, std::uint32_t ns_,
+                        std::uint32_t na_)
+        : ns(ns_), na(na_) {
+        const std::uint64_t recip = ((1ull << 24) + ns_ - 1) / ns_;
+        i = static_cast<std::uint32_t>((std::uint64_t(p0) * recip) >> 24);
+        j = p0 - i * ns_;
+        di = static_cast<std::uint32_t>((std::uint64_t(step) * recip) >> 24);
+        dj = step - di * ns_;
+    }
+    __device__ bool live() const { return !(j < na && j <= i); }
+    __device__ void next() {
+        i += di;
+        j += dj;
+        if (j >= ns) {
+            j -= ns;
+            ++i;
+        }
     }
 };
 
@@ -416,8 +429,8 @@ struct PairMap {
 __device__ __forceinline__ void join_emit(const JoinArgs& a, const std::uint32_t* s_id,
                                           const float* s_thr, const float* s_D,
                                           std::uint32_t pitch, std::uint32_t* s_cnt,
-                                          std::uint32_t* s_base, const PairMap& pm,
-                                          std::uint32_t npair, std::uint32_t ns) {
+                                          std::uint32_t* s_base, std::uint32_t na,
+                                          std::uint32_t ns) {
     const std::uint32_t tid = threadIdx.x;
     for (std::uint32_t x = tid; x < ns; x += kJoinThreads) {
         const std::uint32_t c = s_cnt[x];
@@ -425,10 +438,11 @@ __device__ __forceinline__ void join_emit(const JoinArgs& a, const std::uint32_t
         s_cnt[x] = 0;  // now the write cursor
     }
     __syncthreads();
-    for (std::uint32_t p = tid; p < npair; p += kJoinThreads) {
-        std::uint32_t i, j;
-        if (!pm(p, &i, &j)) continue;
-        const float d = s_D[pitch ? i * pitch + j : p];
+    PairWalk w(tid, kJoinThreads, ns, na);
+    for (std::uint32_t p = tid; p < na * ns; p += kJoinThreads, w.next()) {
+        if (!w.live()) continue;
+        const std::uint32_t i = w.i, j = w.j;
+        const float d = s_D[pitch ? __umul24(i, pitch) + j : p];
         const bool to_i = d < s_thr[i], to_j = d < s_thr[j];
         if (!(to_i || to_j)) continue;
         const unsigned long long hb = static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32;
@@ -464,19 +478,108 @@ __global__ void __launch_bounds__(kJoinThreads) join_kernel(JoinArgs a) {
         std::uint32_t na, ns;
         for (std::uint32_t x = tid; x < S; x += kJoinThreads) s_cnt[x] = 0;
         if (!join_stage(a, v, S, stride, s_id, s_thr, s_rows, &na, &ns)) continue;
-        const PairMap pm(ns, na);
-        const std::uint32_t npair = na * ns;
-        for (std::uint32_t p = tid; p < npair; p += kJoinThreads) {
-            std::uint32_t i, j;
-            if (!pm(p, &i, &j)) continue;
-            const float d = l2sqr_exact(s_rows + std::size_t(i) * stride,
-                                        s_rows + std::size_t(j) * stride, stride);
+        PairWalk w(tid, kJoinThreads, ns, na);
+        for (std::uint32_t p = tid; p < na * ns; p += kJoinThreads, w.next()) {
+            if (!w.live()) continue;
+            const std::uint32_t i = w.i, j = w.j;
+            const float d = l2sqr_exact(s_rows + __umul24(i, stride),
+                                        s_rows + __umul24(j, stride), stride);
             s_Dp[p] = d;
             if (d < s_thr[i]) atomicAdd(&s_cnt[i], 1u);
             if (d < s_thr[j]) atomicAdd(&s_cnt[j], 1u);
         }
         __syncthreads();
-        join_emit(a, s_id, s_thr, s_Dp, 0, s_cnt, s_base, pm, npair, ns);
+        join_emit(a, s_id, s_thr, s_Dp, 0, s_cnt, s_base, na, ns);
+        __syncthreads();
+    }
+}
+
+// Exact local join, member-stationary, for rows of at most kJoinRegStride
+// floats and at most kJoinThreads members: thread t keeps member j = t % ns
+// (its row, threshold and proposal count) in registers and visits the new
+// rows i = t / ns, + G, ... (G = kJoinThreads / ns), so a pair reads one row
+// from LDS. The distance pass keeps, per thread, one bit per visited pair
+// for each proposal direction instead of the distance; after the per-member
+// reservation the emit pass recomputes the distances of the pairs with a set
+// bit (the same function on the same rows: the same bits). With no distance
+// tile a node needs LDS for ids[S], thr[S], cnt[S], base[S] and rows[S*STRIDE]
+// only, which doubles the blocks a CU holds.
+template <std::uint32_t STRIDE>
+__global__ void __launch_bounds__(kJoinThreads) join_reg_kernel(JoinArgs a) {
+    static_assert(STRIDE % 8 == 0 && STRIDE <= kJoinRegStride, "row kept in registers");
+    extern __shared__ float lds[];
+    const std::uint32_t S = 6u * a.smp;  // <= kJoinThreads (host-checked)
+    std::uint32_t* s_id = reinterpret_cast<std::uint32_t*>(lds);
+    float* s_thr = lds + S;
+    std::uint32_t* s_cnt = reinterpret_cast<std::uint32_t*>(lds + 2 * S);
+    std::uint32_t* s_base = s_cnt + S;
+    float* s_rows = lds + 4 * S;
+    const std::uint32_t tid = threadIdx.x;
+    for (std::uint32_t v = blockIdx.x; v < a.n; v += gridDim.x) {
+        std::uint32_t na, ns;
+        for (std::uint32_t x = tid; x < S; x += kJoinThreads) s_cnt[x] = 0;
+        if (!join_stage(a, v, S, STRIDE, s_id, s_thr, s_rows, &na, &ns)) continue;
+        const std::uint32_t G = kJoinThreads / ns;
+        // t / ns by a rounded-up 16-bit reciprocal: exact for t < 128, ns < 500
+        const std::uint32_t i0 = (tid * ((65536u + ns - 1) / ns)) >> 16;
+        const std::uint32_t j = tid - i0 * ns;
+        const bool active = i0 < G;
+        const std::uint32_t iend = j < na ? j : na;  // new x new pairs once (i < j)
+        float rj[STRIDE];
+        std::uint64_t to_i = 0, to_j = 0;  // bit r: pair (i0 + r G, j) proposes
+        if (active) {
+            const float4* src = reinterpret_cast<const float4*>(s_rows + j * STRIDE);
+#pragma unroll
+            for (std::uint32_t q = 0; q < STRIDE / 4; ++q) {
+                const float4 f = src[q];
+                rj[4 * q] = f.x;
+                rj[4 * q + 1] = f.y;
+                rj[4 * q + 2] = f.z;
+                rj[4 * q + 3] = f.w;
+            }
+            const float thr_j = s_thr[j];
+            std::uint64_t bit = 1;
+            for (std::uint32_t i = i0; i < iend; i += G, bit <<= 1) {
+                const float d = l2sqr_reg<STRIDE>(rj, s_rows + i * STRIDE);
+                if (d < s_thr[i]) {
+                    to_i |= bit;
+                    atomicAdd(&s_cnt[i], 1u);
+                }
+                if (d < thr_j) to_j |= bit;
+            }
+            if (to_j) atomicAdd(&s_cnt[j], static_cast<std::uint32_t>(__popcll(to_j)));
+        }
+        __syncthreads();
+        // one reservation per receiving member, then write at LDS cursors
+        for (std::uint32_t x = tid; x < ns; x += kJoinThreads) {
+            const std::uint32_t c = s_cnt[x];
+            s_base[x] = c ? atomicAdd(&a.fill[s_id[x]], c) : a.cap;
+            s_cnt[x] = 0;
+        }
+        __syncthreads();
+        if (active && (to_i | to_j)) {
+            const std::uint32_t id_j = s_id[j];
+            std::uint32_t pj = to_j ? s_base[j] + atomicAdd(&s_cnt[j],
+                                                            static_cast<std::uint32_t>(__popcll(to_j)))
+                                    : 0u;
+            std::uint64_t m = to_i | to_j;
+            while (m) {
+                const std::uint32_t r = static_cast<std::uint32_t>(__ffsll(static_cast<unsigned long long>(m)) - 1);
+                m &= m - 1;
+                const std::uint32_t i = i0 + r * G;
+                const float d = l2sqr_reg<STRIDE>(rj, s_rows + i * STRIDE);
+                const unsigned long long hb =
+                    static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32;
+                if ((to_i >> r) & 1ull) {
+                    const std::uint32_t pos = s_base[i] + atomicAdd(&s_cnt[i], 1u);
+                    if (pos < a.cap) a.cand[std::size_t(s_id[i]) * a.cap + pos] = hb | id_j;
+                }
+                if ((to_j >> r) & 1ull) {
+                    if (pj < a.cap) a.cand[std::size_t(id_j) * a.cap + pj] = hb | s_id[i];
+                    ++pj;
+                }
+            }
+        }
         __syncthreads();
     }
 }
@@ -644,11 +747,11 @@ __global__ void __launch_bounds__(kJoinThreads) join_gram_kernel(JoinArgs a) {
         __syncthreads();
         // 3. pair pass: +inf where the bound rules out both tests, else the
         // exact distance
-        const PairMap pm(ns, na);
-        for (std::uint32_t p = tid; p < na * ns; p += kJoinThreads) {
-            std::uint32_t i, j;
-            if (!pm(p, &i, &j)) continue;
-            const float dt = s_D[i * P + j];
+        PairWalk w(tid, kJoinThreads, ns, na);
+        for (std::uint32_t p = tid; p < na * ns; p += kJoinThreads, w.next()) {
+            if (!w.live()) continue;
+            const std::uint32_t i = w.i, j = w.j;
+            const float dt = s_D[__umul24(i, P) + j];
             const float nbs = s_nb[i] + s_nb[j];
             float tlo = 0.0f;
             const float s2 = dt - cG * nbs * nbs;
@@ -662,10 +765,10 @@ __global__ void __launch_bounds__(kJoinThreads) join_gram_kernel(JoinArgs a) {
                 const float rlo = tlo * tlo * (1.0f - eta);
                 reject = rlo >= s_thr[i] && rlo >= s_thr[j];
             }
-            const float* ri = s_rows + std::size_t(i) * stride;
-            const float* rj = s_rows + std::size_t(j) * stride;
+            const float* ri = s_rows + __umul24(i, stride);
+            const float* rj = s_rows + __umul24(j, stride);
             const float d = reject ? __builtin_inff() : l2sqr_exact(ri, rj, stride);
-            s_D[i * P + j] = d;
+            s_D[__umul24(i, P) + j] = d;
             if (d < s_thr[i]) atomicAdd(&s_cnt[i], 1u);
             if (d < s_thr[j]) atomicAdd(&s_cnt[j], 1u);
             if (a.tally) {
@@ -689,7 +792,7 @@ __global__ void __launch_bounds__(kJoinThreads) join_gram_kernel(JoinArgs a) {
             }
         }
         __syncthreads();
-        join_emit(a, s_id, s_thr, s_D, P, s_cnt, s_base, pm, na * ns, ns);
+        join_emit(a, s_id, s_thr, s_D, P, s_cnt, s_base, na, ns);
         __syncthreads();
     }
     if (a.tally) {
@@ -1023,6 +1126,19 @@ JoinFn pick_join(std::uint32_t stride) {
     }
 }
 
+// Member-stationary exact join for the strides it is built for, else nullptr.
+JoinFn pick_join_reg(std::uint32_t stride) {
+    switch (stride) {
+        case 8: return join_reg_kernel<8>;
+        case 16: return join_reg_kernel<16>;
+        case 24: return join_reg_kernel<24>;
+        case 32: return join_reg_kernel<32>;
+        case 48: return join_reg_kernel<48>;
+        case 64: return join_reg_kernel<64>;
+        default: return nullptr;
+    }
+}
+
 template <bool BF16>
 JoinFn pick_join_gram_t(std::uint32_t stride) {
     switch (stride) {
@@ -1100,11 +1216,14 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     b.gthr.reserve(std::size_t(n) * S);
     b.gcnt.reserve(n);
     gram = resolve_gram(gram, stride);
-    const JoinFn join_fn = gram == kGramExact ? pick_join(stride)
+    const bool join_reg = gram == kGramExact && S <= kJoinThreads && pick_join_reg(stride);
+    const JoinFn join_fn = join_reg ? pick_join_reg(stride)
+                           : gram == kGramExact ? pick_join(stride)
                            : gram == kGramF16 ? pick_join_gram_t<false>(stride)
                                               : pick_join_gram_t<true>(stride);
     const std::size_t lds =
-        gram == kGramExact
+        join_reg    ? (4 * std::size_t(S) + std::size_t(S) * stride) * sizeof(float)
+        : gram == kGramExact
             ? (4 * std::size_t(S) + std::size_t(S) * stride + std::size_t(A) * S) * sizeof(float)
             : (2 * std::size_t(S) + std::size_t(S) * stride + std::size_t(A) * (S + 1) +
                4 * std::size_t(S)) *
