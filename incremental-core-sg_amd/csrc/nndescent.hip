@@ -179,6 +179,11 @@ __global__ void __launch_bounds__(256)
 // sampling
 // ---------------------------------------------------------------------------
 
+// One segment of SEG lanes per node (SEG >= k, a power of two), 64 / SEG
+// nodes per wavefront: the nodes' dependent loads and atomics are in flight
+// together. Lane `sub` of a segment holds list entry `sub`; ballots are masked
+// to the segment.
+template <std::uint32_t SEG>
 __global__ void __launch_bounds__(256)
     sample_kernel(std::uint32_t n, std::uint32_t k, const std::uint32_t* __restrict__ ids,
                   std::uint8_t* __restrict__ flags, const std::uint32_t* __restrict__ count,
@@ -187,23 +192,28 @@ __global__ void __launch_bounds__(256)
                   std::uint32_t* __restrict__ old_cnt, std::uint32_t* __restrict__ rev_new,
                   std::uint32_t* __restrict__ rev_new_cnt, std::uint32_t* __restrict__ rev_old,
                   std::uint32_t* __restrict__ rev_old_cnt) {
+    constexpr std::uint32_t P = 64 / SEG;  // nodes per wavefront
     const std::uint32_t lane = threadIdx.x & 63u;
+    const std::uint32_t sub = lane % SEG, seg = lane / SEG;
+    const std::uint64_t segmask = (SEG == 64 ? ~0ull : ((1ull << SEG) - 1ull)) << (seg * SEG);
+    const std::uint64_t lt = ((1ull << lane) - 1ull) & segmask;
     const std::uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const std::uint32_t waves = (gridDim.x * blockDim.x) >> 6;
     const std::uint32_t max_new = smp, max_old = 3u * smp;
-    for (std::uint32_t v = wave; v < n; v += waves) {
+    for (std::uint32_t v0 = wave * P; v0 < n; v0 += waves * P) {
+        const std::uint32_t v = v0 + seg;
+        const bool live = v < n;
         const std::size_t base = std::size_t(v) * k;
-        const std::uint32_t c = count[v];
-        const bool has = lane < c;
-        const std::uint32_t id = has ? ids[base + lane] : kNone;
-        const bool isnew = has && flags[base + lane] != 0;
+        const std::uint32_t c = live ? count[v] : 0u;
+        const bool has = sub < c;
+        const std::uint32_t id = has ? ids[base + sub] : kNone;
+        const bool isnew = has && flags[base + sub] != 0;
         const bool isold = has && !isnew;
-        const std::uint64_t lt = (1ull << lane) - 1ull;
         const std::uint64_t mnew = __ballot(isnew), mold = __ballot(isold);
         const std::uint32_t rn = __popcll(mnew & lt), ro = __popcll(mold & lt);
         const bool take_new = isnew && rn < max_new && id < n;
         const bool take_old = isold && ro < max_old && id < n;
-        if (isnew && rn < max_new) flags[base + lane] = 0;
+        if (isnew && rn < max_new) flags[base + sub] = 0;
         // bounded reverse lists: a full list takes no more (the reference
         // checks the count before its fetch_add, nndescent.cpp ReverseCache)
         if (take_new) {
@@ -220,8 +230,8 @@ __global__ void __launch_bounds__(256)
                 if (pos < smp) rev_old[std::size_t(id) * smp + pos] = v;
             }
         }
-        if (lane == 0) {
-            const std::uint32_t tn = __popcll(mnew), to = __popcll(mold);
+        if (sub == 0 && live) {
+            const std::uint32_t tn = __popcll(mnew & segmask), to = __popcll(mold & segmask);
             new_cnt[v] = tn < max_new ? tn : max_new;
             old_cnt[v] = to < max_old ? to : max_old;
         }
@@ -232,10 +242,13 @@ __global__ void __launch_bounds__(256)
 // local join
 // ---------------------------------------------------------------------------
 
-// Gather: one wavefront per node builds its deduplicated candidate list --
-// A = sampled new u reverse new (first), then old u reverse old -- keeping the
-// first copy of every id, plus each member's threshold D[k-1]^2 (to_rank of
-// its current k-th distance, nndescent.cpp:41-42). cnt = na | ns << 16.
+// Gather: each node builds its deduplicated candidate list -- A = sampled new
+// u reverse new (first), then old u reverse old -- keeping the first copy of
+// every id, plus each member's threshold D[k-1]^2 (to_rank of its current
+// k-th distance, nndescent.cpp:41-42). cnt = na | ns << 16. One segment of
+// SEG lanes per node, 64 / SEG nodes per wavefront (their dependent loads in
+// flight together); ballots are masked to the segment.
+template <std::uint32_t SEG>
 __global__ void __launch_bounds__(256)
     gather_kernel(std::uint32_t n, std::uint32_t k, std::uint32_t smp,
                   const float* __restrict__ dist, const std::uint32_t* __restrict__ new_list,
@@ -248,19 +261,25 @@ __global__ void __launch_bounds__(256)
                   const std::uint32_t* __restrict__ rev_old_cnt,
                   std::uint32_t* __restrict__ out_ids, float* __restrict__ out_thr,
                   std::uint32_t* __restrict__ out_cnt) {
-    __shared__ std::uint32_t s_raw[4][6 * kMaxSample];
+    constexpr std::uint32_t P = 64 / SEG;  // nodes per wavefront
+    __shared__ std::uint32_t s_raw[4][P][6 * kMaxSample];
     const std::uint32_t lane = threadIdx.x & 63u, wib = threadIdx.x >> 6;
+    const std::uint32_t sub = lane % SEG, seg = lane / SEG;
+    const std::uint64_t segmask = (SEG == 64 ? ~0ull : ((1ull << SEG) - 1ull)) << (seg * SEG);
+    const std::uint64_t lt = ((1ull << lane) - 1ull) & segmask;
     const std::uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const std::uint32_t waves = (gridDim.x * blockDim.x) >> 6;
     const std::uint32_t S = 6u * smp;
-    std::uint32_t* raw = s_raw[wib];
-    for (std::uint32_t v = wave; v < n; v += waves) {
-        const std::uint32_t nn = new_cnt[v];
-        const std::uint32_t rn = min(rev_new_cnt[v], smp);
-        const std::uint32_t no = old_cnt[v];
-        const std::uint32_t ro = min(rev_old_cnt[v], smp);
+    std::uint32_t* raw = s_raw[wib][seg];
+    for (std::uint32_t v0 = wave * P; v0 < n; v0 += waves * P) {
+        const std::uint32_t v = v0 + seg;
+        const bool live = v < n;
+        const std::uint32_t nn = live ? new_cnt[v] : 0u;
+        const std::uint32_t rn = live ? min(rev_new_cnt[v], smp) : 0u;
+        const std::uint32_t no = live ? old_cnt[v] : 0u;
+        const std::uint32_t ro = live ? min(rev_old_cnt[v], smp) : 0u;
         const std::uint32_t na0 = nn + rn, ns0 = na0 + no + ro;
-        for (std::uint32_t i = lane; i < ns0; i += 64) {
+        for (std::uint32_t i = sub; i < ns0; i += SEG) {
             std::uint32_t id;
             if (i < nn) id = new_list[std::size_t(v) * smp + i];
             else if (i < na0) id = rev_new[std::size_t(v) * smp + (i - nn)];
@@ -271,9 +290,15 @@ __global__ void __launch_bounds__(256)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // segments differ in length: every lane runs the longest one's chunks
+        std::uint32_t ns_max = ns0;
+#pragma unroll
+        for (std::uint32_t off = SEG; off < 64; off <<= 1)
+            ns_max = max(ns_max, static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(ns_max),
+                                                                       static_cast<int>(off), 64)));
         std::uint32_t na = 0, ns = 0;
-        for (std::uint32_t i0 = 0; i0 < ns0; i0 += 64) {
-            const std::uint32_t i = i0 + lane;
+        for (std::uint32_t i0 = 0; i0 < ns_max; i0 += SEG) {
+            const std::uint32_t i = i0 + sub;
             std::uint32_t id = kNone;
             bool keep = false;
             if (i < ns0) {
@@ -281,18 +306,18 @@ __global__ void __launch_bounds__(256)
                 keep = id != kNone;
                 for (std::uint32_t j = 0; j < i && keep; ++j) keep = raw[j] != id;
             }
-            const std::uint64_t m = __ballot(keep);
-            const std::uint32_t pos = ns + __popcll(m & ((1ull << lane) - 1ull));
+            const std::uint64_t m = __ballot(keep) & segmask;
+            const std::uint32_t pos = ns + __popcll(m & lt);
             if (keep) {
                 out_ids[std::size_t(v) * S + pos] = id;
                 const float t = dist[std::size_t(id) * k + k - 1];
                 out_thr[std::size_t(v) * S + pos] = t * t;
             }
             // members taken from the A part (positions < na0)
-            na += __popcll(__ballot(keep && i < na0));
+            na += __popcll(__ballot(keep && i < na0) & segmask);
             ns += __popcll(m);
         }
-        if (lane == 0) out_cnt[v] = na | (ns << 16);
+        if (sub == 0 && live) out_cnt[v] = na | (ns << 16);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1253,7 +1278,9 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         {
             std::unique_ptr<ScopedKernelTimer> t;
             if (gp) t.reset(new ScopedKernelTimer(*gp, "nnd_sample"));
-            sample_kernel<<<wave_blocks, 256, 0, s>>>(
+            const auto sample_fn = k <= 16 ? sample_kernel<16>
+                                   : k <= 32 ? sample_kernel<32> : sample_kernel<64>;
+            sample_fn<<<wave_blocks, 256, 0, s>>>(
                 n, k, ids, flags, count, smp, b.new_list.p, b.new_cnt.p, b.old_list.p,
                 b.old_cnt.p, b.rev_new.p, b.rev_new_cnt.p, b.rev_old.p, b.rev_old_cnt.p);
             CSG_LAUNCH_CHECK();
@@ -1261,7 +1288,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         {
             std::unique_ptr<ScopedKernelTimer> t;
             if (gp) t.reset(new ScopedKernelTimer(*gp, "nnd_gather"));
-            gather_kernel<<<wave_blocks, 256, 0, s>>>(
+            gather_kernel<32><<<wave_blocks, 256, 0, s>>>(
                 n, k, smp, dist, b.new_list.p, b.new_cnt.p, b.old_list.p, b.old_cnt.p,
                 b.rev_new.p, b.rev_new_cnt.p, b.rev_old.p, b.rev_old_cnt.p, b.gids.p, b.gthr.p,
                 b.gcnt.p);
