@@ -921,6 +921,83 @@ __device__ __forceinline__ std::uint32_t sort_unique_top(const unsigned long lon
     return static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
 }
 
+// The same result as sort_unique_top<R> for 64 R / 2 < nv <= 64 R survivors
+// (nv <= kMaxCap), without sorting them all: the keys go to registers; a
+// sorted sample of 64 (spread over s_key) gives a threshold T with about 72
+// keys below it; only those are sorted. When at least k distinct keys lie
+// below T they hold the best k; otherwise (or when more than 128 lie below)
+// all keys are sorted. The distinct count comes from an open-addressing set
+// of the ids (same id implies same key) built over s_key once the best k are
+// stored.
+template <int R>
+__device__ __forceinline__ std::uint32_t select_top_large(unsigned long long* s_key, std::uint32_t nv,
+                                          std::uint32_t k, unsigned long long* s_top) {
+    static_assert(64 * R <= kMaxCap, "s_key holds the survivors");
+    const std::uint32_t lane = __lane_id();
+    const std::uint64_t lt = (1ull << lane) - 1ull;
+    unsigned long long key[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const std::uint32_t e = static_cast<std::uint32_t>(r) * 64u + lane;
+        key[r] = e < nv ? s_key[e] : ~0ull;
+    }
+    unsigned long long smpl[1] = {s_key[(lane * nv) >> 6]};
+    wave_sort_regs<1>(smpl);
+    std::uint32_t q = (72u * 65u) / nv;
+    q = q > 64u ? 63u : (q > 0u ? q - 1u : 0u);
+    const unsigned long long T = __shfl(smpl[0], static_cast<int>(q), 64);
+    __syncthreads();  // every read of s_key done
+    std::uint32_t P = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const bool pass = key[r] < T;
+        const std::uint64_t pm = __ballot(pass);
+        if (pass) s_key[P + __popcll(pm & lt)] = key[r];
+        P += __popcll(pm);
+    }
+    __syncthreads();
+    std::uint32_t dp = 0;
+    if (P <= 64) dp = sort_unique_top<1>(s_key, P, k, s_top);
+    else if (P <= 128) dp = sort_unique_top<2>(s_key, P, k, s_top);
+    __syncthreads();
+    if (dp < k) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const std::uint32_t e = static_cast<std::uint32_t>(r) * 64u + lane;
+            if (e < nv) s_key[e] = key[r];
+        }
+        __syncthreads();
+        const std::uint32_t all = sort_unique_top<R>(s_key, nv, k, s_top);
+        __syncthreads();
+        return all;
+    }
+    // distinct count: ids into a set of H = 2 pow2ceil(nv) <= 2 kMaxCap slots
+    std::uint32_t* tab = reinterpret_cast<std::uint32_t*>(s_key);
+    const std::uint32_t lg = 33u - static_cast<std::uint32_t>(__clz(nv - 1));
+    const std::uint32_t H = 1u << lg;
+    for (std::uint32_t h = lane; h < H; h += 64) tab[h] = kNone;
+    __syncthreads();
+    std::uint32_t distinct = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        bool fresh = false;
+        if (key[r] != ~0ull) {
+            const std::uint32_t id = static_cast<std::uint32_t>(key[r]);
+            for (std::uint32_t h = (id * 0x9E3779B1u) >> (32u - lg);; h = (h + 1) & (H - 1)) {
+                const std::uint32_t prev = atomicCAS(&tab[h], kNone, id);
+                if (prev == kNone) {
+                    fresh = true;
+                    break;
+                }
+                if (prev == id) break;
+            }
+        }
+        distinct += __popcll(__ballot(fresh));
+    }
+    __syncthreads();
+    return distinct;
+}
+
 // Number of entries of the ascending a[0, len) (len <= 64) below x (STRICT) or
 // at most x.
 template <bool STRICT>
@@ -942,7 +1019,8 @@ __device__ __forceinline__ std::uint32_t count_before(const float* a, std::uint3
 // per-candidate tests (knn_graph.cpp:37-55): not the node itself, not already
 // listed, and (for a full list) closer than the current k-th. The survivors
 // are sorted by (distance, id) in registers and the first copy of each key is
-// kept. The best k are merged with the list: an entry's new position is its
+// kept (above 128 survivors only those below a sampled threshold are sorted,
+// select_top_large). The best k are merged with the list: an entry's new position is its
 // rank plus the number of entries of the other sorted list before it --
 // existing entries first among equal distances (the reference's sorted
 // insertion) -- found by binary search.
@@ -995,8 +1073,8 @@ __global__ void __launch_bounds__(64)
             if (nv > 0) {
                 if (nv <= 64) nvalid = sort_unique_top<1>(s_key, nv, k, s_top);
                 else if (nv <= 128) nvalid = sort_unique_top<2>(s_key, nv, k, s_top);
-                else if (nv <= 256) nvalid = sort_unique_top<4>(s_key, nv, k, s_top);
-                else nvalid = sort_unique_top<8>(s_key, nv, k, s_top);
+                else if (nv <= 256) nvalid = select_top_large<4>(s_key, nv, k, s_top);
+                else nvalid = select_top_large<8>(s_key, nv, k, s_top);
             }
             t = nvalid < k ? nvalid : k;
             __syncthreads();  // s_top complete
