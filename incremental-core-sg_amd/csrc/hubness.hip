@@ -49,27 +49,81 @@ __device__ __forceinline__ void uf_unite(std::uint32_t* parent, std::uint32_t a,
     }
 }
 
+// Root of x read through plain (cached) loads. A stale pointer still names an
+// ancestor that x had at some point (pointers only move to smaller ancestors),
+// so equal results for two nodes prove them connected; different results
+// prove nothing and go to uf_unite.
+__device__ __forceinline__ std::uint32_t uf_find_cached(const std::uint32_t* parent,
+                                                        std::uint32_t x) {
+    while (true) {
+        const std::uint32_t p = parent[x];
+        if (p == x) return x;
+        x = p;
+    }
+}
+
 __global__ void iota_kernel(std::uint32_t* __restrict__ p, std::uint32_t n) {
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += gridDim.x * blockDim.x)
         p[i] = i;
 }
 
-// One pass over every stored slot: reverse-degree count and union.
+// Components, phase 1 (a heuristic in the spirit of Afforest, Sutton et al.
+// 2018): a few rounds of Shiloach-Vishkin hooking along each node's first
+// kLinkFirst listed neighbours -- the larger of two roots is hooked under the
+// smaller by atomicMin -- each followed by compressing every path to its
+// root. Hooks follow real edges and pointers only decrease, so the forest
+// stays a valid union-find state; a hook lost to a concurrent smaller one is
+// simply redone by the full pass below, which visits every slot (the lists
+// are directed; no edge is skipped) and is exact. The point is depth: the
+// full pass then mostly finds both roots in one or two cached loads.
+constexpr std::uint32_t kLinkFirst = 2;
+constexpr int kHookRounds = 3;
+
+__global__ void __launch_bounds__(kThreads)
+    hook_kernel(const std::uint32_t* __restrict__ ids, const std::uint32_t* __restrict__ count,
+                std::uint32_t n, std::uint32_t k, std::uint32_t* parent) {
+    for (std::uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += gridDim.x * blockDim.x) {
+        const std::uint32_t c = count[p] < kLinkFirst ? count[p] : kLinkFirst;
+        for (std::uint32_t j = 0; j < c; ++j) {
+            const std::uint32_t q = ids[std::size_t(p) * k + j];
+            if (q >= n) continue;
+            const std::uint32_t rp = uf_find_cached(parent, p), rq = uf_find_cached(parent, q);
+            if (rp != rq) atomicMin(&parent[rp > rq ? rp : rq], rp < rq ? rp : rq);
+        }
+    }
+}
+
+// Every node to its root (no concurrent hooks: plain loads see valid
+// ancestors).
+__global__ void __launch_bounds__(kThreads)
+    compress_kernel(std::uint32_t n, std::uint32_t* parent) {
+    for (std::uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += gridDim.x * blockDim.x)
+        parent[v] = uf_find_cached(parent, v);
+}
+
+// One pass over every stored slot: reverse-degree count and union. A
+// segment of SEG >= k lanes serves one node (lane j its slot j).
+template <std::uint32_t SEG>
 __global__ void __launch_bounds__(kThreads)
     slot_pass_kernel(const std::uint32_t* __restrict__ ids,
                      const std::uint32_t* __restrict__ count, std::uint32_t n, std::uint32_t k,
-                     std::int32_t* __restrict__ rdeg, std::uint32_t* __restrict__ parent) {
-    const std::size_t total = std::size_t(n) * k;
-    for (std::size_t gid = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; gid < total;
-         gid += std::size_t(gridDim.x) * blockDim.x) {
-        const std::uint32_t p = static_cast<std::uint32_t>(gid / k);
-        const std::uint32_t j = static_cast<std::uint32_t>(gid - std::size_t(p) * k);
-        if (j >= count[p]) continue;
-        const std::uint32_t q = ids[gid];
+                     std::int32_t* __restrict__ rdeg, std::uint32_t* parent) {
+    constexpr std::uint32_t P = 64 / SEG;
+    const std::uint32_t lane = threadIdx.x & 63u, sub = lane % SEG, seg = lane / SEG;
+    const std::uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const std::uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+    for (std::uint32_t v0 = wave * P; v0 < n; v0 += waves * P) {
+        const std::uint32_t p = v0 + seg;
+        if (p >= n || sub >= count[p]) continue;
+        const std::uint32_t q = ids[std::size_t(p) * k + sub];
         if (q >= n) continue;
         atomicAdd(&rdeg[q], 1);
-        if (q != p) uf_unite(parent, p, q);
+        if (q == p) continue;
+        if (uf_find_cached(parent, p) == uf_find_cached(parent, q)) continue;
+        uf_unite(parent, p, q);
     }
 }
 
@@ -390,10 +444,18 @@ void hubness_compute(Graph& g, std::uint32_t* tagged_out) {
     fill_u32(reinterpret_cast<std::uint32_t*>(g.rdeg.p), n, 0u, s);
     iota_kernel<<<capped_blocks(n, kThreads), kThreads, 0, s>>>(parent.p, n);
     CSG_LAUNCH_CHECK();
-    const std::size_t slots = std::size_t(n) * k;
     {
         ScopedKernelTimer t(g, "hub_slot_pass");
-        slot_pass_kernel<<<capped_blocks(slots, kThreads), kThreads, 0, s>>>(
+        for (int r = 0; r < kHookRounds; ++r) {
+            hook_kernel<<<capped_blocks(n, kThreads), kThreads, 0, s>>>(g.ids.p, g.count.p, n, k,
+                                                                       parent.p);
+            CSG_LAUNCH_CHECK();
+            compress_kernel<<<capped_blocks(n, kThreads), kThreads, 0, s>>>(n, parent.p);
+            CSG_LAUNCH_CHECK();
+        }
+        const auto pass = k <= 16 ? slot_pass_kernel<16>
+                          : k <= 32 ? slot_pass_kernel<32> : slot_pass_kernel<64>;
+        pass<<<capped_blocks(std::size_t(n) * 64, kThreads, 16384), kThreads, 0, s>>>(
             g.ids.p, g.count.p, n, k, g.rdeg.p, parent.p);
         CSG_LAUNCH_CHECK();
     }
