@@ -62,12 +62,15 @@ __global__ void __launch_bounds__(kThreads)
                     const std::uint32_t cq = count[q];
                     const std::size_t qb = std::size_t(q) * k;
                     if (cq > 0 && !(dist[gid] > dist[qb + cq - 1])) {
-                        for (std::uint32_t t = 0; t < cq; ++t) {
-                            if (ids[qb + t] == p) {
-                                mutual = true;
-                                break;
+                        std::uint32_t t = 0;
+                        if ((k & 3u) == 0) {  // rows 16-byte aligned: four ids per load
+                            const uint4* row = reinterpret_cast<const uint4*>(ids + qb);
+                            for (; t + 4 <= cq && !mutual; t += 4) {
+                                const uint4 x = row[t >> 2];
+                                mutual = x.x == p || x.y == p || x.z == p || x.w == p;
                             }
                         }
+                        for (; t < cq && !mutual; ++t) mutual = ids[qb + t] == p;
                     }
                 }
                 if (!mutual) {

@@ -1129,16 +1129,24 @@ MergeFn pick_merge(std::uint32_t k) {
 
 constexpr std::uint32_t kMergeBlocks = 65536;
 
-__global__ void __launch_bounds__(256)
+// One block of kSumThreads adds the per-block partials (loads unrolled so
+// that many are in flight per thread).
+constexpr unsigned kSumThreads = 1024;
+__global__ void __launch_bounds__(kSumThreads)
     sum_partials_kernel(const unsigned long long* __restrict__ partial, std::uint32_t m,
                         unsigned long long* __restrict__ out) {
-    __shared__ unsigned long long s_w[4];
+    __shared__ unsigned long long s_w[kSumThreads / 64];
     unsigned long long v = 0;
-    for (std::uint32_t i = threadIdx.x; i < m; i += 256) v += partial[i];
+#pragma unroll 8
+    for (std::uint32_t i = threadIdx.x; i < m; i += kSumThreads) v += partial[i];
     v = wave_sum(v);
     if ((threadIdx.x & 63u) == 0) s_w[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0) out[0] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (unsigned w = 0; w < kSumThreads / 64; ++w) t += s_w[w];
+        out[0] = t;
+    }
 }
 
 // Work tallies of one join launch (profiling only): members staged (rows
@@ -1394,7 +1402,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
                                                      b.fill.p, cap, b.partial.p);
             CSG_LAUNCH_CHECK();
         }
-        sum_partials_kernel<<<1, 256, 0, s>>>(b.partial.p, merge_blocks, b.updates.p);
+        sum_partials_kernel<<<1, kSumThreads, 0, s>>>(b.partial.p, merge_blocks, b.updates.p);
         CSG_LAUNCH_CHECK();
         unsigned long long upd = 0, tl[3] = {0, 0, 0};
         CSG_CHECK(hipMemcpyAsync(&upd, b.updates.p, 8, hipMemcpyDeviceToHost, s));
@@ -2165,7 +2173,7 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
                                                      g.count.p, b.cand.p, b.fill.p, cap,
                                                      b.partial.p);
             CSG_LAUNCH_CHECK();
-            sum_partials_kernel<<<1, 256, 0, s>>>(b.partial.p, merge_blocks, b.updates.p);
+            sum_partials_kernel<<<1, kSumThreads, 0, s>>>(b.partial.p, merge_blocks, b.updates.p);
             CSG_LAUNCH_CHECK();
             unsigned long long upd = 0;
             CSG_CHECK(hipMemcpyAsync(&upd, b.updates.p, 8, hipMemcpyDeviceToHost, s));
