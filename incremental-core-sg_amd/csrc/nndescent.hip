@@ -247,7 +247,9 @@ __global__ void __launch_bounds__(256)
 // every id, plus each member's threshold D[k-1]^2 (to_rank of its current
 // k-th distance, nndescent.cpp:41-42). cnt = na | ns << 16. One segment of
 // SEG lanes per node, 64 / SEG nodes per wavefront (their dependent loads in
-// flight together); ballots are masked to the segment.
+// flight together); ballots are masked to the segment. First copies are
+// found in a per-node LDS table (id -> smallest position) instead of
+// comparing every entry with all earlier ones.
 template <std::uint32_t SEG>
 __global__ void __launch_bounds__(256)
     gather_kernel(std::uint32_t n, std::uint32_t k, std::uint32_t smp,
@@ -262,7 +264,11 @@ __global__ void __launch_bounds__(256)
                   std::uint32_t* __restrict__ out_ids, float* __restrict__ out_thr,
                   std::uint32_t* __restrict__ out_cnt) {
     constexpr std::uint32_t P = 64 / SEG;  // nodes per wavefront
-    __shared__ std::uint32_t s_raw[4][P][6 * kMaxSample];
+    constexpr std::uint32_t R = (6 * kMaxSample + SEG - 1) / SEG;  // entries per lane
+    // per node: open-addressing table id -> smallest position (load <= 3/4)
+    constexpr std::uint32_t H = 256;
+    static_assert(4 * 6 * kMaxSample <= 3 * H, "table load");
+    __shared__ std::uint32_t s_hkey[4][P][H], s_hidx[4][P][H];
     const std::uint32_t lane = threadIdx.x & 63u, wib = threadIdx.x >> 6;
     const std::uint32_t sub = lane % SEG, seg = lane / SEG;
     const std::uint64_t segmask = (SEG == 64 ? ~0ull : ((1ull << SEG) - 1ull)) << (seg * SEG);
@@ -270,7 +276,8 @@ __global__ void __launch_bounds__(256)
     const std::uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const std::uint32_t waves = (gridDim.x * blockDim.x) >> 6;
     const std::uint32_t S = 6u * smp;
-    std::uint32_t* raw = s_raw[wib][seg];
+    std::uint32_t* hkey = s_hkey[wib][seg];
+    std::uint32_t* hidx = s_hidx[wib][seg];
     for (std::uint32_t v0 = wave * P; v0 < n; v0 += waves * P) {
         const std::uint32_t v = v0 + seg;
         const bool live = v < n;
@@ -279,42 +286,53 @@ __global__ void __launch_bounds__(256)
         const std::uint32_t no = live ? old_cnt[v] : 0u;
         const std::uint32_t ro = live ? min(rev_old_cnt[v], smp) : 0u;
         const std::uint32_t na0 = nn + rn, ns0 = na0 + no + ro;
-        for (std::uint32_t i = sub; i < ns0; i += SEG) {
-            std::uint32_t id;
-            if (i < nn) id = new_list[std::size_t(v) * smp + i];
-            else if (i < na0) id = rev_new[std::size_t(v) * smp + (i - nn)];
-            else if (i < na0 + no) id = old_list[std::size_t(v) * 3 * smp + (i - na0)];
-            else id = rev_old[std::size_t(v) * smp + (i - na0 - no)];
-            raw[i] = id < n ? id : kNone;
+        for (std::uint32_t h = sub; h < H; h += SEG) {
+            hkey[h] = kNone;
+            hidx[h] = kNone;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // segments differ in length: every lane runs the longest one's chunks
-        std::uint32_t ns_max = ns0;
+        // entry e = sub + r SEG: its id, and its slot in the table
+        std::uint32_t id[R], slot[R];
 #pragma unroll
-        for (std::uint32_t off = SEG; off < 64; off <<= 1)
-            ns_max = max(ns_max, static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(ns_max),
-                                                                       static_cast<int>(off), 64)));
-        std::uint32_t na = 0, ns = 0;
-        for (std::uint32_t i0 = 0; i0 < ns_max; i0 += SEG) {
-            const std::uint32_t i = i0 + sub;
-            std::uint32_t id = kNone;
-            bool keep = false;
-            if (i < ns0) {
-                id = raw[i];
-                keep = id != kNone;
-                for (std::uint32_t j = 0; j < i && keep; ++j) keep = raw[j] != id;
+        for (std::uint32_t r = 0; r < R; ++r) {
+            const std::uint32_t e = sub + r * SEG;
+            std::uint32_t x = kNone;
+            if (e < nn) x = new_list[std::size_t(v) * smp + e];
+            else if (e < na0) x = rev_new[std::size_t(v) * smp + (e - nn)];
+            else if (e < na0 + no) x = old_list[std::size_t(v) * 3 * smp + (e - na0)];
+            else if (e < ns0) x = rev_old[std::size_t(v) * smp + (e - na0 - no)];
+            id[r] = x < n ? x : kNone;
+            slot[r] = 0;
+            if (id[r] != kNone) {
+                std::uint32_t h = (id[r] * 0x9E3779B1u) >> 24;
+                for (;;) {
+                    const std::uint32_t prev = atomicCAS(&hkey[h], kNone, id[r]);
+                    if (prev == kNone || prev == id[r]) break;
+                    h = (h + 1) & (H - 1);
+                }
+                atomicMin(&hidx[h], e);
+                slot[r] = h;
             }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // first copies, in order; members from the A part (positions < na0)
+        std::uint32_t na = 0, ns = 0;
+#pragma unroll
+        for (std::uint32_t r = 0; r < R; ++r) {
+            const std::uint32_t e = sub + r * SEG;
+            const bool keep = id[r] != kNone && hidx[slot[r]] == e;
             const std::uint64_t m = __ballot(keep) & segmask;
             const std::uint32_t pos = ns + __popcll(m & lt);
             if (keep) {
-                out_ids[std::size_t(v) * S + pos] = id;
-                const float t = dist[std::size_t(id) * k + k - 1];
+                out_ids[std::size_t(v) * S + pos] = id[r];
+                const float t = dist[std::size_t(id[r]) * k + k - 1];
                 out_thr[std::size_t(v) * S + pos] = t * t;
             }
-            // members taken from the A part (positions < na0)
-            na += __popcll(__ballot(keep && i < na0) & segmask);
+            na += __popcll(__ballot(keep && e < na0) & segmask);
             ns += __popcll(m);
         }
         if (sub == 0 && live) out_cnt[v] = na | (ns << 16);
@@ -323,6 +341,7 @@ __global__ void __launch_bounds__(256)
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
+
 
 struct JoinArgs {
     const float* pts;
