@@ -1032,18 +1032,25 @@ __device__ __forceinline__ std::uint32_t count_before(const float* a, std::uint3
     return pos;
 }
 
+// Set of a list's ids for the merge's listed test (at most kMaxK of kListSet
+// slots used).
+constexpr std::uint32_t kListSet = 2 * kMaxK;
+__device__ __forceinline__ std::uint32_t list_slot(std::uint32_t id) {
+    return (id * 0x9E3779B1u) >> 25;  // 7 bits: kListSet = 128
+}
+
 // One 64-thread group per node (nodes [0, n), candidate ids below id_limit),
-// lists of at most KC entries. The round's proposals
+// lists of at most kMaxK entries. The round's proposals
 // cand[v*cap .. v*cap + min(fill, cap)) are first filtered by try_insert's
 // per-candidate tests (knn_graph.cpp:37-55): not the node itself, not already
-// listed, and (for a full list) closer than the current k-th. The survivors
-// are sorted by (distance, id) in registers and the first copy of each key is
-// kept (above 128 survivors only those below a sampled threshold are sorted,
-// select_top_large). The best k are merged with the list: an entry's new position is its
-// rank plus the number of entries of the other sorted list before it --
-// existing entries first among equal distances (the reference's sorted
-// insertion) -- found by binary search.
-template <std::uint32_t KC>
+// listed (an LDS set of the list's ids), and (for a full list) closer than
+// the current k-th. The survivors are sorted by (distance, id) in registers
+// and the first copy of each key is kept (above 128 survivors only those
+// below a sampled threshold are sorted, select_top_large). The best k are
+// merged with the list: an entry's new position is its rank plus the number
+// of entries of the other sorted list before it -- existing entries first
+// among equal distances (the reference's sorted insertion) -- found by binary
+// search.
 __global__ void __launch_bounds__(64)
     merge_kernel(std::uint32_t n, std::uint32_t id_limit, std::uint32_t k,
                  std::uint32_t* __restrict__ ids,
@@ -1054,6 +1061,7 @@ __global__ void __launch_bounds__(64)
     __shared__ unsigned long long s_key[kMaxCap];
     __shared__ unsigned long long s_top[kMaxK];
     __shared__ float s_ld[kMaxK], s_td[kMaxK];
+    __shared__ std::uint32_t s_list[kListSet];
     const std::uint32_t lane = threadIdx.x;
     const std::uint64_t lt = (1ull << lane) - 1ull;
     unsigned long long local_upd = 0;
@@ -1068,6 +1076,15 @@ __global__ void __launch_bounds__(64)
         m = m < cap ? m : cap;
         std::uint32_t t = 0;
         if (m > 0) {
+            // the listed ids in an open-addressing set (kListSet slots, load
+            // <= 1/2): about one probe per candidate
+            for (std::uint32_t h = lane; h < kListSet; h += 64) s_list[h] = kNone;
+            __syncthreads();
+            if (lane < c) {
+                std::uint32_t h = list_slot(my_id);
+                while (atomicCAS(&s_list[h], kNone, my_id) != kNone) h = (h + 1) & (kListSet - 1);
+            }
+            __syncthreads();
             // 1. per-candidate tests, survivors compacted into s_key
             std::uint32_t nv = 0;
             for (std::uint32_t i0 = 0; i0 < m; i0 += 64) {
@@ -1076,11 +1093,15 @@ __global__ void __launch_bounds__(64)
                 const std::uint32_t cid = static_cast<std::uint32_t>(key);
                 const float cd = __uint_as_float(static_cast<std::uint32_t>(key >> 32));
                 bool ok = key != ~0ull && cid != v && cid < id_limit && cd < last;
-#pragma unroll
-                for (std::uint32_t j = 0; j < KC; ++j) {
-                    const std::uint32_t lj = static_cast<std::uint32_t>(
-                        __builtin_amdgcn_readlane(static_cast<int>(my_id), static_cast<int>(j)));
-                    ok = ok && lj != cid;  // unlisted slots hold kNone
+                if (ok) {
+                    for (std::uint32_t h = list_slot(cid);; h = (h + 1) & (kListSet - 1)) {
+                        const std::uint32_t x = s_list[h];
+                        if (x == cid) {
+                            ok = false;
+                            break;
+                        }
+                        if (x == kNone) break;
+                    }
                 }
                 const std::uint64_t vm = __ballot(ok);
                 if (ok) s_key[nv + __popcll(vm & lt)] = key;
@@ -1140,11 +1161,7 @@ using MergeFn = void (*)(std::uint32_t, std::uint32_t, std::uint32_t, std::uint3
                          std::uint8_t*, std::uint32_t*, const unsigned long long*, std::uint32_t*,
                          std::uint32_t, unsigned long long*);
 
-MergeFn pick_merge(std::uint32_t k) {
-    if (k <= 16) return merge_kernel<16>;
-    if (k <= 32) return merge_kernel<32>;
-    return merge_kernel<64>;
-}
+MergeFn pick_merge(std::uint32_t) { return merge_kernel; }
 
 constexpr std::uint32_t kMergeBlocks = 65536;
 
