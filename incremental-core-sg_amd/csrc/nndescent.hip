@@ -42,6 +42,14 @@ namespace {
 constexpr std::uint32_t kMaxK = 64;        // list length handled by one wavefront
 constexpr std::uint32_t kMaxSample = 32;   // reference cap (cuda_common.cuh:32)
 constexpr std::uint32_t kJoinThreads = 128;
+// reverse lists by target buckets: kRevTargets targets per bucket, at most
+// kRevMaxBuckets buckets (n <= 2^24; larger graphs use one device atomic per
+// entry), sampling and scatter on a grid of kRevBlocks blocks
+constexpr std::uint32_t kRevShift = 11;
+constexpr std::uint32_t kRevTargets = 1u << kRevShift;
+constexpr std::uint32_t kRevMaxBuckets = 8192;
+constexpr unsigned kRevBlocks = 4096;
+constexpr unsigned kRevThreads = 1024;
 constexpr std::uint32_t kJoinRegStride = 64;  // widest row a join thread keeps in registers
 
 // ---------------------------------------------------------------------------
@@ -182,8 +190,11 @@ __global__ void __launch_bounds__(256)
 // One segment of SEG lanes per node (SEG >= k, a power of two), 64 / SEG
 // nodes per wavefront: the nodes' dependent loads and atomics are in flight
 // together. Lane `sub` of a segment holds list entry `sub`; ballots are masked
-// to the segment.
-template <std::uint32_t SEG>
+// to the segment. With BUCKETS the reverse lists are left to the bucket
+// passes below and the block only counts its sampled entries per target
+// bucket (hist, bucket-major: hist[b * gridDim.x + block]); otherwise every
+// entry takes its reverse-list slot with a device atomic.
+template <std::uint32_t SEG, bool BUCKETS>
 __global__ void __launch_bounds__(256)
     sample_kernel(std::uint32_t n, std::uint32_t k, const std::uint32_t* __restrict__ ids,
                   std::uint8_t* __restrict__ flags, const std::uint32_t* __restrict__ count,
@@ -191,7 +202,9 @@ __global__ void __launch_bounds__(256)
                   std::uint32_t* __restrict__ new_cnt, std::uint32_t* __restrict__ old_list,
                   std::uint32_t* __restrict__ old_cnt, std::uint32_t* __restrict__ rev_new,
                   std::uint32_t* __restrict__ rev_new_cnt, std::uint32_t* __restrict__ rev_old,
-                  std::uint32_t* __restrict__ rev_old_cnt) {
+                  std::uint32_t* __restrict__ rev_old_cnt, std::uint32_t* __restrict__ hist,
+                  std::uint32_t nb) {
+    extern __shared__ std::uint32_t s_hist[];  // nb buckets (BUCKETS only)
     constexpr std::uint32_t P = 64 / SEG;  // nodes per wavefront
     const std::uint32_t lane = threadIdx.x & 63u;
     const std::uint32_t sub = lane % SEG, seg = lane / SEG;
@@ -200,6 +213,10 @@ __global__ void __launch_bounds__(256)
     const std::uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const std::uint32_t waves = (gridDim.x * blockDim.x) >> 6;
     const std::uint32_t max_new = smp, max_old = 3u * smp;
+    if (BUCKETS) {
+        for (std::uint32_t x = threadIdx.x; x < nb; x += blockDim.x) s_hist[x] = 0;
+        __syncthreads();
+    }
     for (std::uint32_t v0 = wave * P; v0 < n; v0 += waves * P) {
         const std::uint32_t v = v0 + seg;
         const bool live = v < n;
@@ -213,19 +230,21 @@ __global__ void __launch_bounds__(256)
         const std::uint32_t rn = __popcll(mnew & lt), ro = __popcll(mold & lt);
         const bool take_new = isnew && rn < max_new && id < n;
         const bool take_old = isold && ro < max_old && id < n;
-        if (isnew && rn < max_new) flags[base + sub] = 0;
-        // bounded reverse lists: a full list takes no more (the reference
-        // checks the count before its fetch_add, nndescent.cpp ReverseCache)
-        if (take_new) {
-            new_list[std::size_t(v) * max_new + rn] = id;
-            if (__atomic_load_n(&rev_new_cnt[id], __ATOMIC_RELAXED) < smp) {
+        if (isnew && rn < max_new) {
+            flags[base + sub] = 0;
+            new_list[std::size_t(v) * max_new + rn] = id < n ? id : kNone;
+        }
+        if (isold && ro < max_old) old_list[std::size_t(v) * max_old + ro] = id < n ? id : kNone;
+        if (BUCKETS) {
+            if (take_new || take_old) atomicAdd(&s_hist[id >> kRevShift], 1u);
+        } else {
+            // bounded reverse lists: a full list takes no more (the reference
+            // checks the count before its fetch_add, nndescent.cpp ReverseCache)
+            if (take_new && __atomic_load_n(&rev_new_cnt[id], __ATOMIC_RELAXED) < smp) {
                 const std::uint32_t pos = atomicAdd(&rev_new_cnt[id], 1u);
                 if (pos < smp) rev_new[std::size_t(id) * smp + pos] = v;
             }
-        }
-        if (take_old) {
-            old_list[std::size_t(v) * max_old + ro] = id;
-            if (__atomic_load_n(&rev_old_cnt[id], __ATOMIC_RELAXED) < smp) {
+            if (take_old && __atomic_load_n(&rev_old_cnt[id], __ATOMIC_RELAXED) < smp) {
                 const std::uint32_t pos = atomicAdd(&rev_old_cnt[id], 1u);
                 if (pos < smp) rev_old[std::size_t(id) * smp + pos] = v;
             }
@@ -235,6 +254,89 @@ __global__ void __launch_bounds__(256)
             new_cnt[v] = tn < max_new ? tn : max_new;
             old_cnt[v] = to < max_old ? to : max_old;
         }
+    }
+    if (BUCKETS) {
+        __syncthreads();
+        for (std::uint32_t x = threadIdx.x; x < nb; x += blockDim.x)
+            hist[std::size_t(x) * gridDim.x + blockIdx.x] = s_hist[x];
+    }
+}
+
+// Reverse lists by target buckets (kRevTargets consecutive targets each), in
+// place of one device atomic per sampled entry. After sample_kernel<.., true>
+// and an exclusive scan of its histograms, the scatter pass walks the same
+// nodes in the same blocks and writes each sampled entry (target, source,
+// old bit) into its bucket's range at the block's offset plus an LDS cursor;
+// then one block per bucket hands out the reverse-list slots with LDS
+// counters and writes every target's counts. Which sources a full reverse
+// list keeps is arrival order, as with the reference's fetch_add.
+template <std::uint32_t SEG>
+__global__ void __launch_bounds__(256)
+    rev_scatter_kernel(std::uint32_t n, std::uint32_t smp,
+                       const std::uint32_t* __restrict__ new_list,
+                       const std::uint32_t* __restrict__ new_cnt,
+                       const std::uint32_t* __restrict__ old_list,
+                       const std::uint32_t* __restrict__ old_cnt,
+                       const std::uint32_t* __restrict__ off, std::uint32_t nb,
+                       unsigned long long* __restrict__ pairs) {
+    extern __shared__ std::uint32_t s_cur[];  // nb bucket cursors
+    constexpr std::uint32_t P = 64 / SEG;
+    const std::uint32_t lane = threadIdx.x & 63u;
+    const std::uint32_t sub = lane % SEG, seg = lane / SEG;
+    const std::uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const std::uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+    for (std::uint32_t x = threadIdx.x; x < nb; x += blockDim.x)
+        s_cur[x] = off[std::size_t(x) * gridDim.x + blockIdx.x];
+    __syncthreads();
+    for (std::uint32_t v0 = wave * P; v0 < n; v0 += waves * P) {
+        const std::uint32_t v = v0 + seg;
+        if (v >= n) continue;
+        const std::uint32_t nc = new_cnt[v], oc = old_cnt[v];
+        for (std::uint32_t e = sub; e < nc; e += SEG) {
+            const std::uint32_t id = new_list[std::size_t(v) * smp + e];
+            if (id < n) {
+                const std::uint32_t pos = atomicAdd(&s_cur[id >> kRevShift], 1u);
+                pairs[pos] = (static_cast<unsigned long long>(id) << 32) | v;
+            }
+        }
+        for (std::uint32_t e = sub; e < oc; e += SEG) {
+            const std::uint32_t id = old_list[std::size_t(v) * 3 * smp + e];
+            if (id < n) {
+                const std::uint32_t pos = atomicAdd(&s_cur[id >> kRevShift], 1u);
+                pairs[pos] = (static_cast<unsigned long long>(id) << 32) | v | 0x80000000u;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kRevThreads)
+    rev_bucket_kernel(std::uint32_t n, std::uint32_t smp,
+                      const unsigned long long* __restrict__ pairs,
+                      const std::uint32_t* __restrict__ off, std::uint32_t nblk,
+                      const std::uint32_t* __restrict__ total, std::uint32_t* __restrict__ rev_new,
+                      std::uint32_t* __restrict__ rev_new_cnt, std::uint32_t* __restrict__ rev_old,
+                      std::uint32_t* __restrict__ rev_old_cnt) {
+    __shared__ std::uint32_t s_cn[kRevTargets], s_co[kRevTargets];
+    const std::uint32_t b = blockIdx.x, lo = b << kRevShift;
+    for (std::uint32_t t = threadIdx.x; t < kRevTargets; t += kRevThreads) {
+        s_cn[t] = 0;
+        s_co[t] = 0;
+    }
+    __syncthreads();
+    const std::uint32_t start = off[std::size_t(b) * nblk];
+    const std::uint32_t end = b + 1 < gridDim.x ? off[std::size_t(b + 1) * nblk] : *total;
+    for (std::uint32_t e = start + threadIdx.x; e < end; e += kRevThreads) {
+        const unsigned long long pr = pairs[e];
+        const std::uint32_t id = static_cast<std::uint32_t>(pr >> 32);
+        const std::uint32_t src = static_cast<std::uint32_t>(pr) & 0x7FFFFFFFu;
+        const bool is_old = (static_cast<std::uint32_t>(pr) & 0x80000000u) != 0;
+        const std::uint32_t pos = atomicAdd(is_old ? &s_co[id - lo] : &s_cn[id - lo], 1u);
+        if (pos < smp) (is_old ? rev_old : rev_new)[std::size_t(id) * smp + pos] = src;
+    }
+    __syncthreads();
+    for (std::uint32_t t = threadIdx.x; t < kRevTargets && lo + t < n; t += kRevThreads) {
+        rev_new_cnt[lo + t] = s_cn[t];
+        rev_old_cnt[lo + t] = s_co[t];
     }
 }
 
@@ -1358,6 +1460,16 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     b.tally.reserve(4);
     fill_u32(b.fill.p, n, 0u, s);
 
+    // reverse lists by target buckets when the bucket count allows
+    const std::uint32_t nb = (n + kRevTargets - 1) >> kRevShift;
+    const bool rev_buckets = nb <= kRevMaxBuckets;
+    if (rev_buckets) {
+        b.rev_pairs.reserve(std::size_t(n) * k);
+        b.rev_hist.reserve(std::size_t(nb) * kRevBlocks);
+        b.rev_tmp.reserve(scan_tmp_words(std::size_t(nb) * kRevBlocks));
+        b.rev_total.reserve(1);
+    }
+
     const std::uint32_t S = 6u * smp, A = 2u * smp;
     b.gids.reserve(std::size_t(n) * S);
     b.gthr.reserve(std::size_t(n) * S);
@@ -1395,17 +1507,44 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     }
     const double stop = delta * double(k) * double(n);
     for (std::uint32_t round = 0; round < iters; ++round) {
-        fill_u32(b.rev_new_cnt.p, n, 0u, s);
-        fill_u32(b.rev_old_cnt.p, n, 0u, s);
         {
             std::unique_ptr<ScopedKernelTimer> t;
             if (gp) t.reset(new ScopedKernelTimer(*gp, "nnd_sample"));
-            const auto sample_fn = k <= 16 ? sample_kernel<16>
-                                   : k <= 32 ? sample_kernel<32> : sample_kernel<64>;
-            sample_fn<<<wave_blocks, 256, 0, s>>>(
-                n, k, ids, flags, count, smp, b.new_list.p, b.new_cnt.p, b.old_list.p,
-                b.old_cnt.p, b.rev_new.p, b.rev_new_cnt.p, b.rev_old.p, b.rev_old_cnt.p);
-            CSG_LAUNCH_CHECK();
+            if (rev_buckets) {
+                const std::size_t hl = std::size_t(nb) * sizeof(std::uint32_t);
+                const auto sample_fn = k <= 16 ? sample_kernel<16, true>
+                                       : k <= 32 ? sample_kernel<32, true>
+                                                 : sample_kernel<64, true>;
+                sample_fn<<<kRevBlocks, 256, hl, s>>>(
+                    n, k, ids, flags, count, smp, b.new_list.p, b.new_cnt.p, b.old_list.p,
+                    b.old_cnt.p, b.rev_new.p, b.rev_new_cnt.p, b.rev_old.p, b.rev_old_cnt.p,
+                    b.rev_hist.p, nb);
+                CSG_LAUNCH_CHECK();
+                exclusive_scan_u32(b.rev_hist.p, std::size_t(nb) * kRevBlocks, b.rev_total.p,
+                                   b.rev_tmp.p, s);
+                const auto scatter_fn = k <= 16 ? rev_scatter_kernel<16>
+                                        : k <= 32 ? rev_scatter_kernel<32>
+                                                  : rev_scatter_kernel<64>;
+                scatter_fn<<<kRevBlocks, 256, hl, s>>>(n, smp, b.new_list.p, b.new_cnt.p,
+                                                       b.old_list.p, b.old_cnt.p, b.rev_hist.p,
+                                                       nb, b.rev_pairs.p);
+                CSG_LAUNCH_CHECK();
+                rev_bucket_kernel<<<nb, kRevThreads, 0, s>>>(
+                    n, smp, b.rev_pairs.p, b.rev_hist.p, kRevBlocks, b.rev_total.p, b.rev_new.p,
+                    b.rev_new_cnt.p, b.rev_old.p, b.rev_old_cnt.p);
+                CSG_LAUNCH_CHECK();
+            } else {
+                fill_u32(b.rev_new_cnt.p, n, 0u, s);
+                fill_u32(b.rev_old_cnt.p, n, 0u, s);
+                const auto sample_fn = k <= 16 ? sample_kernel<16, false>
+                                       : k <= 32 ? sample_kernel<32, false>
+                                                 : sample_kernel<64, false>;
+                sample_fn<<<wave_blocks, 256, 0, s>>>(
+                    n, k, ids, flags, count, smp, b.new_list.p, b.new_cnt.p, b.old_list.p,
+                    b.old_cnt.p, b.rev_new.p, b.rev_new_cnt.p, b.rev_old.p, b.rev_old_cnt.p,
+                    nullptr, 0);
+                CSG_LAUNCH_CHECK();
+            }
         }
         {
             std::unique_ptr<ScopedKernelTimer> t;
