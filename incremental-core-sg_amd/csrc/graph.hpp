@@ -58,6 +58,7 @@ struct DescentBuffers {
     DBuf<std::uint32_t> rev_new, rev_new_cnt, rev_old, rev_old_cnt;
     DBuf<unsigned long long> cand, updates, partial, tally;
     DBuf<std::uint32_t> fill, gids, gcnt;
+    DBuf<float> thr_node;  // each node's join threshold this round
     // reverse lists by target buckets: sampled (target, source) pairs, the
     // per-block bucket histograms (scanned in place), scan scratch, total
     DBuf<unsigned long long> rev_pairs;

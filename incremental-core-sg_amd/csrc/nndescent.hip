@@ -193,7 +193,9 @@ __global__ void __launch_bounds__(256)
 // to the segment. With BUCKETS the reverse lists are left to the bucket
 // passes below and the block only counts its sampled entries per target
 // bucket (hist, bucket-major: hist[b * gridDim.x + block]); otherwise every
-// entry takes its reverse-list slot with a device atomic.
+// entry takes its reverse-list slot with a device atomic. Each node's join
+// threshold goes to thr_node (n floats, small enough to stay in cache for the
+// gather's random reads).
 template <std::uint32_t SEG, bool BUCKETS>
 __global__ void __launch_bounds__(256)
     sample_kernel(std::uint32_t n, std::uint32_t k, const std::uint32_t* __restrict__ ids,
@@ -203,7 +205,8 @@ __global__ void __launch_bounds__(256)
                   std::uint32_t* __restrict__ old_cnt, std::uint32_t* __restrict__ rev_new,
                   std::uint32_t* __restrict__ rev_new_cnt, std::uint32_t* __restrict__ rev_old,
                   std::uint32_t* __restrict__ rev_old_cnt, std::uint32_t* __restrict__ hist,
-                  std::uint32_t nb) {
+                  std::uint32_t nb, const float* __restrict__ dist,
+                  float* __restrict__ thr_node) {
     extern __shared__ std::uint32_t s_hist[];  // nb buckets (BUCKETS only)
     constexpr std::uint32_t P = 64 / SEG;  // nodes per wavefront
     const std::uint32_t lane = threadIdx.x & 63u;
@@ -253,6 +256,10 @@ __global__ void __launch_bounds__(256)
             const std::uint32_t tn = __popcll(mnew & segmask), to = __popcll(mold & segmask);
             new_cnt[v] = tn < max_new ? tn : max_new;
             old_cnt[v] = to < max_old ? to : max_old;
+            // the node's join threshold D[k-1]^2 for this round (to_rank of
+            // its current k-th distance, nndescent.cpp:41-42)
+            const float t = dist[base + k - 1];
+            thr_node[v] = t * t;
         }
     }
     if (BUCKETS) {
@@ -347,15 +354,14 @@ __global__ void __launch_bounds__(kRevThreads)
 // Gather: each node builds its deduplicated candidate list -- A = sampled new
 // u reverse new (first), then old u reverse old -- keeping the first copy of
 // every id, plus each member's threshold D[k-1]^2 (to_rank of its current
-// k-th distance, nndescent.cpp:41-42). cnt = na | ns << 16. One segment of
+// k-th distance, nndescent.cpp:41-42, from the sampling pass's thr_node). cnt = na | ns << 16. One segment of
 // SEG lanes per node, 64 / SEG nodes per wavefront (their dependent loads in
 // flight together); ballots are masked to the segment. First copies are
 // found in a per-node LDS table (id -> smallest position) instead of
 // comparing every entry with all earlier ones.
 template <std::uint32_t SEG>
 __global__ void __launch_bounds__(256)
-    gather_kernel(std::uint32_t n, std::uint32_t k, std::uint32_t smp,
-                  const float* __restrict__ dist, const std::uint32_t* __restrict__ new_list,
+    gather_kernel(std::uint32_t n, std::uint32_t smp, const std::uint32_t* __restrict__ new_list,
                   const std::uint32_t* __restrict__ new_cnt,
                   const std::uint32_t* __restrict__ old_list,
                   const std::uint32_t* __restrict__ old_cnt,
@@ -364,7 +370,7 @@ __global__ void __launch_bounds__(256)
                   const std::uint32_t* __restrict__ rev_old,
                   const std::uint32_t* __restrict__ rev_old_cnt,
                   std::uint32_t* __restrict__ out_ids, float* __restrict__ out_thr,
-                  std::uint32_t* __restrict__ out_cnt) {
+                  std::uint32_t* __restrict__ out_cnt, const float* __restrict__ thr_node) {
     constexpr std::uint32_t P = 64 / SEG;  // nodes per wavefront
     constexpr std::uint32_t R = (6 * kMaxSample + SEG - 1) / SEG;  // entries per lane
     // per node: open-addressing table id -> smallest position (load <= 3/4)
@@ -431,8 +437,7 @@ __global__ void __launch_bounds__(256)
             const std::uint32_t pos = ns + __popcll(m & lt);
             if (keep) {
                 out_ids[std::size_t(v) * S + pos] = id[r];
-                const float t = dist[std::size_t(id[r]) * k + k - 1];
-                out_thr[std::size_t(v) * S + pos] = t * t;
+                out_thr[std::size_t(v) * S + pos] = thr_node[id[r]];
             }
             na += __popcll(__ballot(keep && e < na0) & segmask);
             ns += __popcll(m);
@@ -1455,6 +1460,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     b.rev_old_cnt.reserve(n);
     b.cand.reserve(std::size_t(n) * cap);
     b.fill.reserve(n);
+    b.thr_node.reserve(n);
     b.updates.reserve(1);
     b.partial.reserve(kMergeBlocks);
     b.tally.reserve(4);
@@ -1518,7 +1524,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
                 sample_fn<<<kRevBlocks, 256, hl, s>>>(
                     n, k, ids, flags, count, smp, b.new_list.p, b.new_cnt.p, b.old_list.p,
                     b.old_cnt.p, b.rev_new.p, b.rev_new_cnt.p, b.rev_old.p, b.rev_old_cnt.p,
-                    b.rev_hist.p, nb);
+                    b.rev_hist.p, nb, dist, b.thr_node.p);
                 CSG_LAUNCH_CHECK();
                 exclusive_scan_u32(b.rev_hist.p, std::size_t(nb) * kRevBlocks, b.rev_total.p,
                                    b.rev_tmp.p, s);
@@ -1542,7 +1548,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
                 sample_fn<<<wave_blocks, 256, 0, s>>>(
                     n, k, ids, flags, count, smp, b.new_list.p, b.new_cnt.p, b.old_list.p,
                     b.old_cnt.p, b.rev_new.p, b.rev_new_cnt.p, b.rev_old.p, b.rev_old_cnt.p,
-                    nullptr, 0);
+                    nullptr, 0, dist, b.thr_node.p);
                 CSG_LAUNCH_CHECK();
             }
         }
@@ -1550,9 +1556,9 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
             std::unique_ptr<ScopedKernelTimer> t;
             if (gp) t.reset(new ScopedKernelTimer(*gp, "nnd_gather"));
             gather_kernel<32><<<wave_blocks, 256, 0, s>>>(
-                n, k, smp, dist, b.new_list.p, b.new_cnt.p, b.old_list.p, b.old_cnt.p,
+                n, smp, b.new_list.p, b.new_cnt.p, b.old_list.p, b.old_cnt.p,
                 b.rev_new.p, b.rev_new_cnt.p, b.rev_old.p, b.rev_old_cnt.p, b.gids.p, b.gthr.p,
-                b.gcnt.p);
+                b.gcnt.p, b.thr_node.p);
             CSG_LAUNCH_CHECK();
         }
         if (gram_audit) {
