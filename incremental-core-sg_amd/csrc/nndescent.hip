@@ -393,11 +393,14 @@ __global__ void __launch_bounds__(256)
         const std::uint32_t rn = live ? min(rev_new_cnt[v], smp) : 0u;
         const std::uint32_t no = live ? old_cnt[v] : 0u;
         const std::uint32_t ro = live ? min(rev_old_cnt[v], smp) : 0u;
-        const std::uint32_t na0 = nn + rn, ns0 = na0 + no + ro;
-        for (std::uint32_t h = sub; h < H; h += SEG) {
-            hkey[h] = kNone;
-            hidx[h] = kNone;
-        }
+        const std::uint32_t na0 = nn + rn;
+        // a node without new members has no pairs to join: nothing to gather
+        const std::uint32_t ns0 = na0 ? na0 + no + ro : 0u;
+        if (ns0)
+            for (std::uint32_t h = sub; h < H; h += SEG) {
+                hkey[h] = kNone;
+                hidx[h] = kNone;
+            }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -407,10 +410,12 @@ __global__ void __launch_bounds__(256)
         for (std::uint32_t r = 0; r < R; ++r) {
             const std::uint32_t e = sub + r * SEG;
             std::uint32_t x = kNone;
-            if (e < nn) x = new_list[std::size_t(v) * smp + e];
-            else if (e < na0) x = rev_new[std::size_t(v) * smp + (e - nn)];
-            else if (e < na0 + no) x = old_list[std::size_t(v) * 3 * smp + (e - na0)];
-            else if (e < ns0) x = rev_old[std::size_t(v) * smp + (e - na0 - no)];
+            if (e < ns0) {
+                if (e < nn) x = new_list[std::size_t(v) * smp + e];
+                else if (e < na0) x = rev_new[std::size_t(v) * smp + (e - nn)];
+                else if (e < na0 + no) x = old_list[std::size_t(v) * 3 * smp + (e - na0)];
+                else x = rev_old[std::size_t(v) * smp + (e - na0 - no)];
+            }
             id[r] = x < n ? x : kNone;
             slot[r] = 0;
             if (id[r] != kNone) {
