@@ -723,17 +723,22 @@ __global__ void __launch_bounds__(kJoinThreads) join_reg_kernel(JoinArgs a) {
                 const std::uint32_t r = static_cast<std::uint32_t>(__ffsll(static_cast<unsigned long long>(m)) - 1);
                 m &= m - 1;
                 const std::uint32_t i = i0 + r * G;
-                const float d = l2sqr_reg<STRIDE>(rj, s_rows + i * STRIDE);
-                const unsigned long long hb =
-                    static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32;
-                if ((to_i >> r) & 1ull) {
-                    const std::uint32_t pos = s_base[i] + atomicAdd(&s_cnt[i], 1u);
-                    if (pos < a.cap) a.cand[std::size_t(s_id[i]) * a.cap + pos] = hb | id_j;
+                // a direction whose range starts at the capacity writes
+                // nothing: skip it, and the distance when both do
+                const bool wi = ((to_i >> r) & 1ull) && s_base[i] < a.cap;
+                const bool dj = (to_j >> r) & 1ull;
+                const bool wj = dj && pj < a.cap;
+                if (wi || wj) {
+                    const float d = l2sqr_reg<STRIDE>(rj, s_rows + i * STRIDE);
+                    const unsigned long long hb =
+                        static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32;
+                    if (wi) {
+                        const std::uint32_t pos = s_base[i] + atomicAdd(&s_cnt[i], 1u);
+                        if (pos < a.cap) a.cand[std::size_t(s_id[i]) * a.cap + pos] = hb | id_j;
+                    }
+                    if (wj) a.cand[std::size_t(id_j) * a.cap + pj] = hb | s_id[i];
                 }
-                if ((to_j >> r) & 1ull) {
-                    if (pj < a.cap) a.cand[std::size_t(id_j) * a.cap + pj] = hb | s_id[i];
-                    ++pj;
-                }
+                if (dj) ++pj;
             }
         }
         __syncthreads();
