@@ -100,8 +100,7 @@ __global__ void __launch_bounds__(kThreads)
     scatter_kernel(const std::uint32_t* __restrict__ ids, const float* __restrict__ dist,
                    std::uint32_t n, std::uint32_t k, const std::uint8_t* __restrict__ emit,
                    const std::uint32_t* __restrict__ off, std::uint32_t* __restrict__ cursor,
-                   std::uint32_t* __restrict__ tv, float* __restrict__ tw,
-                   std::uint32_t* __restrict__ tsrc) {
+                   uint4* __restrict__ tmp) {
     const std::size_t total = std::size_t(n) * k;
     for (std::size_t gid = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; gid < total;
          gid += std::size_t(gridDim.x) * blockDim.x) {
@@ -110,9 +109,7 @@ __global__ void __launch_bounds__(kThreads)
         const std::uint32_t q = ids[gid];
         const std::uint32_t u = p < q ? p : q, v = p < q ? q : p;
         const std::uint32_t pos = off[u] + atomicAdd(&cursor[u], 1u);
-        tv[pos] = v;
-        tw[pos] = dist[gid];
-        tsrc[pos] = static_cast<std::uint32_t>(gid);
+        tmp[pos] = make_uint4(v, static_cast<std::uint32_t>(gid), __float_as_uint(dist[gid]), 0u);
     }
 }
 
@@ -122,24 +119,22 @@ __global__ void bridge_scatter_kernel(const std::uint32_t* __restrict__ bu,
                                       std::uint32_t n, std::uint32_t src_base,
                                       const std::uint32_t* __restrict__ off,
                                       std::uint32_t* __restrict__ cursor,
-                                      std::uint32_t* __restrict__ tv, float* __restrict__ tw,
-                                      std::uint32_t* __restrict__ tsrc) {
+                                      uint4* __restrict__ tmp) {
     const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m || bu[i] >= n) return;
     const std::uint32_t u = bu[i];
     const std::uint32_t pos = off[u] + atomicAdd(&cursor[u], 1u);
-    tv[pos] = bv[i];
-    tw[pos] = bw[i];
-    tsrc[pos] = src_base + i;
+    tmp[pos] = make_uint4(bv[i], src_base + i, __float_as_uint(bw[i]), 0u);
 }
 
 // One wavefront per u: rank each entry of u's segment by (v, src) and write it
 // to its final position. Segments of up to 64 entries rank in registers with
-// shuffles; longer ones (hubs) compare against the segment in memory.
+// shuffles; longer ones (hubs) compare against the segment in memory. The
+// scattered entries are one 16-byte record each, (v, src, weight bits, 0), so
+// the scatter writes one transaction per entry.
 __global__ void __launch_bounds__(kThreads)
     segment_sort_kernel(std::uint32_t n, const std::uint32_t* __restrict__ off,
-                        const std::uint32_t* __restrict__ tv, const float* __restrict__ tw,
-                        const std::uint32_t* __restrict__ tsrc, std::uint32_t* __restrict__ eu,
+                        const uint4* __restrict__ tmp, std::uint32_t* __restrict__ eu,
                         std::uint32_t* __restrict__ ev, float* __restrict__ ew) {
     const std::uint32_t waves_per_block = blockDim.x / kWave;
     const std::uint32_t lane = threadIdx.x & (kWave - 1);
@@ -149,8 +144,8 @@ __global__ void __launch_bounds__(kThreads)
         if (L == 0) continue;
         if (L <= kWave) {
             const bool has = lane < L;
-            const std::uint32_t myv = has ? tv[s + lane] : 0xFFFFFFFFu;
-            const std::uint32_t mys = has ? tsrc[s + lane] : 0xFFFFFFFFu;
+            const uint4 t = has ? tmp[s + lane] : make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);
+            const std::uint32_t myv = t.x, mys = t.y;
             std::uint32_t rank = 0;
             for (std::uint32_t j = 0; j < L; ++j) {
                 const std::uint32_t ov = __shfl(myv, j, kWave);
@@ -160,19 +155,20 @@ __global__ void __launch_bounds__(kThreads)
             if (has) {
                 eu[s + rank] = u;
                 ev[s + rank] = myv;
-                ew[s + rank] = tw[s + lane];
+                ew[s + rank] = __uint_as_float(t.z);
             }
         } else {
             for (std::uint32_t i = lane; i < L; i += kWave) {
-                const std::uint32_t myv = tv[s + i], mys = tsrc[s + i];
+                const uint4 t = tmp[s + i];
+                const std::uint32_t myv = t.x, mys = t.y;
                 std::uint32_t rank = 0;
                 for (std::uint32_t j = 0; j < L; ++j) {
-                    const std::uint32_t ov = tv[s + j], os = tsrc[s + j];
-                    rank += (ov < myv || (ov == myv && os < mys)) ? 1u : 0u;
+                    const uint4 o = tmp[s + j];
+                    rank += (o.x < myv || (o.x == myv && o.y < mys)) ? 1u : 0u;
                 }
                 eu[s + rank] = u;
                 ev[s + rank] = myv;
-                ew[s + rank] = tw[s + i];
+                ew[s + rank] = __uint_as_float(t.z);
             }
         }
     }
@@ -306,27 +302,25 @@ void prepare_edges(Graph& g) {
     CSG_CHECK(hipStreamSynchronize(s));
     const std::uint64_t m = total;
 
-    DBuf<std::uint32_t> cursor, tv, tsrc;
-    DBuf<float> tw;
+    DBuf<std::uint32_t> cursor;
+    DBuf<uint4> tmp;  // (v, src, weight bits, 0) per edge, grouped by u
     cursor.reserve(n);
     fill_u32(cursor.p, n, 0u, s);
-    tv.reserve(m);
-    tw.reserve(m);
-    tsrc.reserve(m);
+    tmp.reserve(m);
     scatter_kernel<<<capped_blocks(slots, kThreads), kThreads, 0, s>>>(
-        g.ids.p, g.dist.p, n, k, emit.p, g.eoff.p, cursor.p, tv.p, tw.p, tsrc.p);
+        g.ids.p, g.dist.p, n, k, emit.p, g.eoff.p, cursor.p, tmp.p);
     CSG_LAUNCH_CHECK();
     if (nb) {
         bridge_scatter_kernel<<<blocks_for(nb, kThreads), kThreads, 0, s>>>(
             dbu.p, dbv.p, dbw.p, nb, n, static_cast<std::uint32_t>(slots), g.eoff.p, cursor.p,
-            tv.p, tw.p, tsrc.p);
+            tmp.p);
         CSG_LAUNCH_CHECK();
     }
     g.eu.reserve(m);
     g.ev.reserve(m);
     g.ew.reserve(m);
     segment_sort_kernel<<<capped_blocks(std::size_t(n) * kWave, kThreads, 16384), kThreads, 0,
-                          s>>>(n, g.eoff.p, tv.p, tw.p, tsrc.p, g.eu.p, g.ev.p, g.ew.p);
+                          s>>>(n, g.eoff.p, tmp.p, g.eu.p, g.ev.p, g.ew.p);
     CSG_LAUNCH_CHECK();
 
     // Symmetric adjacency for the vertex-centric Boruvka rounds.
