@@ -206,6 +206,7 @@ int coresg_hip_graph_set_points(coresg_hip_graph* h, const float* rows, int wher
         CSG_CHECK(hipStreamSynchronize(g.stream));
         g.points_set = true;
         g.edges_ready = false;
+        g.emit_ready = false;
     });
 }
 
@@ -228,6 +229,7 @@ int coresg_hip_graph_set_knn(coresg_hip_graph* h, const uint32_t* ids, const flo
         CSG_CHECK(hipStreamSynchronize(g.stream));
         g.knn_set = true;
         g.edges_ready = false;
+        g.emit_ready = false;
         g.hub_ready = false;
     });
 }
@@ -385,6 +387,7 @@ int coresg_hip_graph_set_instances(coresg_hip_graph* h, const uint32_t* offsets,
             CSG_CHECK(hipStreamSynchronize(g.stream));
         }
         g.edges_ready = false;
+        g.emit_ready = false;
         g.hub_ready = false;
     });
 }

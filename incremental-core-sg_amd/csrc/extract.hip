@@ -260,6 +260,40 @@ void check_mpts(const Graph& g, std::uint32_t* mpts) {
     if (*mpts > g.k) throw std::runtime_error("CoreSG::edges: mpts exceeds k_max");
 }
 
+namespace {
+// Ownership flags of every slot (g.emit_flags), optionally their total;
+// valid until the k-NN lists change.
+void emit_state(Graph& g, std::uint64_t* d_total) {
+    const std::size_t slots = std::size_t(g.n) * g.k;
+    g.emit_flags.reserve(slots);
+    emit_count_kernel<<<capped_blocks(slots, kThreads), kThreads, 0, g.stream>>>(
+        g.ids.p, g.dist.p, g.count.p, g.n, g.k, g.emit_flags.p, nullptr,
+        reinterpret_cast<unsigned long long*>(d_total));
+    CSG_LAUNCH_CHECK();
+    g.emit_ready = true;
+}
+
+// Owned-edge count per first endpoint u = min(p, q) from the ownership flags:
+// one thread per node p counts its owned slots with q > p itself and adds the
+// others to their q.
+__global__ void __launch_bounds__(kThreads)
+    emit_deg_kernel(const std::uint32_t* __restrict__ ids, const std::uint8_t* __restrict__ emit,
+                    std::uint32_t n, std::uint32_t k, std::uint32_t* __restrict__ deg) {
+    for (std::uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += gridDim.x * blockDim.x) {
+        std::uint32_t own = 0;
+        for (std::uint32_t j = 0; j < k; ++j) {
+            const std::size_t gid = std::size_t(p) * k + j;
+            if (!emit[gid]) continue;
+            const std::uint32_t q = ids[gid];
+            if (q > p) ++own;
+            else atomicAdd(&deg[q], 1u);
+        }
+        if (own) atomicAdd(&deg[p], own);
+    }
+}
+}  // namespace
+
 void prepare_edges(Graph& g) {
     if (g.edges_ready) return;
     if (!g.knn_set) throw std::runtime_error("prepare_edges: k-NN lists not set");
@@ -271,12 +305,12 @@ void prepare_edges(Graph& g) {
     if (n >= kSecondEnd) throw std::runtime_error("point count exceeds 2^31");
     hipStream_t s = g.stream;
 
-    DBuf<std::uint8_t> emit;
-    emit.reserve(slots);
+    // ownership flags from the build's edge count when the lists are unchanged
+    if (!g.emit_ready) emit_state(g, nullptr);
     g.eoff.reserve(std::size_t(n) + 1);
     fill_u32(g.eoff.p, std::size_t(n) + 1, 0u, s);
-    emit_count_kernel<<<capped_blocks(slots, kThreads), kThreads, 0, s>>>(
-        g.ids.p, g.dist.p, g.count.p, n, k, emit.p, g.eoff.p, nullptr);
+    emit_deg_kernel<<<capped_blocks(n, kThreads), kThreads, 0, s>>>(g.ids.p, g.emit_flags.p, n,
+                                                                    k, g.eoff.p);
     CSG_LAUNCH_CHECK();
 
     DBuf<std::uint32_t> dbu, dbv;
@@ -308,7 +342,7 @@ void prepare_edges(Graph& g) {
     fill_u32(cursor.p, n, 0u, s);
     tmp.reserve(m);
     scatter_kernel<<<capped_blocks(slots, kThreads), kThreads, 0, s>>>(
-        g.ids.p, g.dist.p, n, k, emit.p, g.eoff.p, cursor.p, tmp.p);
+        g.ids.p, g.dist.p, n, k, g.emit_flags.p, g.eoff.p, cursor.p, tmp.p);
     CSG_LAUNCH_CHECK();
     if (nb) {
         bridge_scatter_kernel<<<blocks_for(nb, kThreads), kThreads, 0, s>>>(
@@ -354,14 +388,11 @@ void prepare_edges(Graph& g) {
 // Undirected k-NN edge count (KnnGraph::undirected_edge_count,
 // knn_graph.cpp:202-217) without materialising the list.
 std::uint64_t count_knn_edges(Graph& g) {
-    const std::size_t slots = std::size_t(g.n) * g.k;
     g.counters64.reserve(4);
     std::uint64_t* d_total = g.counters64.p + 3;
     fill_u64(d_total, 1, 0ull, g.stream);
-    emit_count_kernel<<<capped_blocks(slots, kThreads), kThreads, 0, g.stream>>>(
-        g.ids.p, g.dist.p, g.count.p, g.n, g.k, nullptr, nullptr,
-        reinterpret_cast<unsigned long long*>(d_total));
-    CSG_LAUNCH_CHECK();
+    // the ownership state is kept for prepare_edges
+    emit_state(g, d_total);
     std::uint64_t total = 0;
     CSG_CHECK(hipMemcpyAsync(&total, d_total, 8, hipMemcpyDeviceToHost, g.stream));
     CSG_CHECK(hipStreamSynchronize(g.stream));

@@ -94,6 +94,11 @@ struct Graph {
     std::vector<float> bw;
     std::size_t repair_count = 0;
 
+    // Ownership of every slot (knn_graph.cpp:143-166), as computed for the
+    // build's edge count; valid while the k-NN lists are unchanged, and reused
+    // by prepare_edges.
+    bool emit_ready = false;
+    DBuf<std::uint8_t> emit_flags;
     // Deduplicated, (u,v)-sorted CORE-SG edge list.
     bool edges_ready = false;
     std::uint64_t num_edges = 0, knn_edges = 0;

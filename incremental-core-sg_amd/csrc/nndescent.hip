@@ -2187,6 +2187,7 @@ void build_knn(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_stat
     CSG_CHECK(hipStreamSynchronize(s));
     g.knn_set = true;
     g.edges_ready = false;
+    g.emit_ready = false;
     g.hub_ready = false;
     if (st) {
         st->rounds = r.rounds;
@@ -2286,6 +2287,7 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
     g.n = n1;
     g.knn_set = true;
     g.edges_ready = false;
+    g.emit_ready = false;
     g.hub_ready = false;
 
     std::uint64_t applied = 0;
