@@ -88,10 +88,17 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
-__global__ void bridge_count_kernel(const std::uint32_t* __restrict__ bu, std::uint32_t m,
-                                    std::uint32_t n, std::uint32_t* __restrict__ deg) {
+// Bridge edges (bu < bv) counted at both endpoints: deg1 as first endpoint
+// (edge list segments), deg2 as second endpoint (the adjacency's rest).
+__global__ void bridge_count_kernel(const std::uint32_t* __restrict__ bu,
+                                    const std::uint32_t* __restrict__ bv, std::uint32_t m,
+                                    std::uint32_t n, std::uint32_t* __restrict__ deg1,
+                                    std::uint32_t* __restrict__ deg2) {
     const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m && bu[i] < n) atomicAdd(&deg[bu[i]], 1u);
+    if (i < m && bu[i] < n) {
+        atomicAdd(&deg1[bu[i]], 1u);
+        atomicAdd(&deg2[bv[i]], 1u);
+    }
 }
 
 // Scatter emitted slots and bridges into per-u segments (unordered within a
@@ -204,21 +211,13 @@ __global__ void core_init_kernel(const float* __restrict__ dist,
 
 // ---- symmetric adjacency (built once per CORE-SG) --------------------------
 
-__global__ void __launch_bounds__(kThreads)
-    adj_degree_kernel(const std::uint32_t* __restrict__ ev, std::uint64_t m,
-                      std::uint32_t* __restrict__ deg) {
-    for (std::uint64_t e = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
-         e += std::uint64_t(gridDim.x) * blockDim.x)
-        atomicAdd(&deg[ev[e]], 1u);
-}
-
-// Degree of every vertex as first endpoint: its segment of the sorted list.
+// Adds every vertex's degree as first endpoint: its segment of the sorted list.
 __global__ void __launch_bounds__(kThreads)
     first_degree_kernel(const std::uint32_t* __restrict__ eoff, std::uint32_t n,
                         std::uint32_t* __restrict__ deg) {
     for (std::uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < n;
          x += gridDim.x * blockDim.x)
-        deg[x] = eoff[x + 1] - eoff[x];
+        deg[x] += eoff[x + 1] - eoff[x];
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -273,23 +272,31 @@ void emit_state(Graph& g, std::uint64_t* d_total) {
     g.emit_ready = true;
 }
 
-// Owned-edge count per first endpoint u = min(p, q) from the ownership flags:
-// one thread per node p counts its owned slots with q > p itself and adds the
-// others to their q.
+// Owned-edge counts from the ownership flags, per first endpoint u = min(p, q)
+// (deg1: the edge list's segments) and per second endpoint v = max(p, q)
+// (deg2: the adjacency's second-endpoint entries): one thread per node p
+// counts its own side of each owned slot itself and adds the other side to q.
 __global__ void __launch_bounds__(kThreads)
     emit_deg_kernel(const std::uint32_t* __restrict__ ids, const std::uint8_t* __restrict__ emit,
-                    std::uint32_t n, std::uint32_t k, std::uint32_t* __restrict__ deg) {
+                    std::uint32_t n, std::uint32_t k, std::uint32_t* __restrict__ deg1,
+                    std::uint32_t* __restrict__ deg2) {
     for (std::uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n;
          p += gridDim.x * blockDim.x) {
-        std::uint32_t own = 0;
+        std::uint32_t first = 0, second = 0;
         for (std::uint32_t j = 0; j < k; ++j) {
             const std::size_t gid = std::size_t(p) * k + j;
             if (!emit[gid]) continue;
             const std::uint32_t q = ids[gid];
-            if (q > p) ++own;
-            else atomicAdd(&deg[q], 1u);
+            if (q > p) {
+                ++first;
+                atomicAdd(&deg2[q], 1u);
+            } else {
+                ++second;
+                atomicAdd(&deg1[q], 1u);
+            }
         }
-        if (own) atomicAdd(&deg[p], own);
+        if (first) atomicAdd(&deg1[p], first);
+        if (second) atomicAdd(&deg2[p], second);
     }
 }
 }  // namespace
@@ -309,8 +316,10 @@ void prepare_edges(Graph& g) {
     if (!g.emit_ready) emit_state(g, nullptr);
     g.eoff.reserve(std::size_t(n) + 1);
     fill_u32(g.eoff.p, std::size_t(n) + 1, 0u, s);
+    g.adj_off.reserve(std::size_t(n) + 1);
+    fill_u32(g.adj_off.p, std::size_t(n) + 1, 0u, s);
     emit_deg_kernel<<<capped_blocks(n, kThreads), kThreads, 0, s>>>(g.ids.p, g.emit_flags.p, n,
-                                                                    k, g.eoff.p);
+                                                                    k, g.eoff.p, g.adj_off.p);
     CSG_LAUNCH_CHECK();
 
     DBuf<std::uint32_t> dbu, dbv;
@@ -322,8 +331,8 @@ void prepare_edges(Graph& g) {
         CSG_CHECK(hipMemcpyAsync(dbu.p, g.bu.data(), nb * 4, hipMemcpyHostToDevice, s));
         CSG_CHECK(hipMemcpyAsync(dbv.p, g.bv.data(), nb * 4, hipMemcpyHostToDevice, s));
         CSG_CHECK(hipMemcpyAsync(dbw.p, g.bw.data(), nb * 4, hipMemcpyHostToDevice, s));
-        bridge_count_kernel<<<blocks_for(nb, kThreads), kThreads, 0, s>>>(dbu.p, nb, n,
-                                                                           g.eoff.p);
+        bridge_count_kernel<<<blocks_for(nb, kThreads), kThreads, 0, s>>>(dbu.p, dbv.p, nb, n,
+                                                                           g.eoff.p, g.adj_off.p);
         CSG_LAUNCH_CHECK();
     }
 
@@ -357,15 +366,10 @@ void prepare_edges(Graph& g) {
                           s>>>(n, g.eoff.p, tmp.p, g.eu.p, g.ev.p, g.ew.p);
     CSG_LAUNCH_CHECK();
 
-    // Symmetric adjacency for the vertex-centric Boruvka rounds.
-    g.adj_off.reserve(std::size_t(n) + 1);
-    fill_u32(g.adj_off.p, std::size_t(n) + 1, 0u, s);
+    // Symmetric adjacency for the vertex-centric Boruvka rounds: each vertex's
+    // first-endpoint segment plus its second-endpoint count (already in adj_off).
     first_degree_kernel<<<capped_blocks(n, kThreads), kThreads, 0, s>>>(g.eoff.p, n, g.adj_off.p);
     CSG_LAUNCH_CHECK();
-    if (m) {
-        adj_degree_kernel<<<capped_blocks(m, kThreads), kThreads, 0, s>>>(g.ev.p, m, g.adj_off.p);
-        CSG_LAUNCH_CHECK();
-    }
     exclusive_scan_u32(g.adj_off.p, std::size_t(n) + 1, nullptr, g.scan_tmp.p, s);
     g.adj_nbr.reserve(2 * m);
     g.adj_rank.reserve(2 * m);
