@@ -154,9 +154,11 @@ __global__ void __launch_bounds__(kThreads)
             const uint4 t = has ? tmp[s + lane] : make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);
             const std::uint32_t myv = t.x, mys = t.y;
             std::uint32_t rank = 0;
-            for (std::uint32_t j = 0; j < L; ++j) {
-                const std::uint32_t ov = __shfl(myv, j, kWave);
-                const std::uint32_t os = __shfl(mys, j, kWave);
+            for (std::uint32_t j = 0; j < L; ++j) {  // j uniform: a readlane each
+                const std::uint32_t ov = static_cast<std::uint32_t>(
+                    __builtin_amdgcn_readlane(static_cast<int>(myv), static_cast<int>(j)));
+                const std::uint32_t os = static_cast<std::uint32_t>(
+                    __builtin_amdgcn_readlane(static_cast<int>(mys), static_cast<int>(j)));
                 rank += (ov < myv || (ov == myv && os < mys)) ? 1u : 0u;
             }
             if (has) {
