@@ -3,6 +3,11 @@
  * Used by tests/, the smoke check and the CPU-baseline leg of bench.py as the
  * checker. Nothing in the product library links or calls it.
  *
+ * Parity unpinned: the representative cut of the connectivity repair above
+ * max_reps (a stable sort standing in for the reference's partial_sort, whose
+ * tie order is implementation-defined); every other function is checked
+ * against the reference's own vectors or closed forms.
+ *
  * Each function names the reference function it restates (path:line relative
  * to the reference repository). The schedule is the reference's serial one
  * (`--threads 1`), which is the only schedule whose k-NN lists are
