@@ -653,8 +653,8 @@ __global__ void __launch_bounds__(kJoinThreads) join_kernel(JoinArgs a) {
 // Exact local join, member-stationary, for rows of at most kJoinRegStride
 // floats and at most kJoinThreads members: thread t keeps member j = t % ns
 // (its row, threshold and proposal count) in registers and visits the new
-// rows i = t / ns, + G, ... (G = kJoinThreads / ns), so a pair reads one row
-// from LDS. The distance pass keeps, per thread, one bit per visited pair
+// rows i = t / ns, + G, ... (G = kJoinThreads / ns; the threads left over
+// give the last members one more phase), so a pair reads one row from LDS. The distance pass keeps, per thread, one bit per visited pair
 // for each proposal direction instead of the distance; after the per-member
 // reservation the emit pass recomputes the distances of the pairs with a set
 // bit (the same function on the same rows: the same bits). With no distance
@@ -675,15 +675,23 @@ __global__ void __launch_bounds__(kJoinThreads) join_reg_kernel(JoinArgs a) {
         std::uint32_t na, ns;
         for (std::uint32_t x = tid; x < S; x += kJoinThreads) s_cnt[x] = 0;
         if (!join_stage(a, v, S, STRIDE, s_id, s_thr, s_rows, &na, &ns)) continue;
-        const std::uint32_t G = kJoinThreads / ns;
+        // G0 = 128 / ns threads per member (i0 = 0..G0-1, step G0); the
+        // E = 128 - G0 ns threads left over become an extra phase (i0 = G0)
+        // of the last E members, which then step by G0 + 1: every thread has
+        // a member
+        const std::uint32_t G0 = kJoinThreads / ns, E = kJoinThreads - G0 * ns;
         // t / ns by a rounded-up 16-bit reciprocal: exact for t < 128, ns < 500
-        const std::uint32_t i0 = (tid * ((65536u + ns - 1) / ns)) >> 16;
-        const std::uint32_t j = tid - i0 * ns;
-        const bool active = i0 < G;
+        std::uint32_t i0 = (tid * ((65536u + ns - 1) / ns)) >> 16;
+        std::uint32_t j = tid - i0 * ns;
+        if (i0 >= G0) {  // an extra thread
+            j = ns - E + (tid - G0 * ns);
+            i0 = G0;
+        }
+        const std::uint32_t G = j >= ns - E ? G0 + 1 : G0;
         const std::uint32_t iend = j < na ? j : na;  // new x new pairs once (i < j)
         float rj[STRIDE];
         std::uint64_t to_i = 0, to_j = 0;  // bit r: pair (i0 + r G, j) proposes
-        if (active) {
+        {
             const float4* src = reinterpret_cast<const float4*>(s_rows + j * STRIDE);
 #pragma unroll
             for (std::uint32_t q = 0; q < STRIDE / 4; ++q) {
@@ -713,7 +721,7 @@ __global__ void __launch_bounds__(kJoinThreads) join_reg_kernel(JoinArgs a) {
             s_cnt[x] = 0;
         }
         __syncthreads();
-        if (active && (to_i | to_j)) {
+        if (to_i | to_j) {
             const std::uint32_t id_j = s_id[j];
             std::uint32_t pj = to_j ? s_base[j] + atomicAdd(&s_cnt[j],
                                                             static_cast<std::uint32_t>(__popcll(to_j)))
