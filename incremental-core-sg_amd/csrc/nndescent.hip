@@ -650,6 +650,15 @@ __global__ void __launch_bounds__(kJoinThreads) join_kernel(JoinArgs a) {
     }
 }
 
+// The t-th partner of member j in the member-stationary join: new member t,
+// or new member j - 1 - t (mod na) in the circular order.
+__device__ __forceinline__ std::uint32_t join_partner(bool circ, std::uint32_t j,
+                                                      std::uint32_t na, std::uint32_t t) {
+    if (!circ) return t;
+    const std::uint32_t o = t + 1;
+    return j >= o ? j - o : j + na - o;
+}
+
 // Exact local join, member-stationary, for rows of at most kJoinRegStride
 // floats and at most kJoinThreads members: thread t keeps member j = t % ns
 // (its row, threshold and proposal count) in registers and visits the new
@@ -688,9 +697,19 @@ __global__ void __launch_bounds__(kJoinThreads) join_reg_kernel(JoinArgs a) {
             i0 = G0;
         }
         const std::uint32_t G = j >= ns - E ? G0 + 1 : G0;
-        const std::uint32_t iend = j < na ? j : na;  // new x new pairs once (i < j)
+        // partners of member j, t = 0, 1, ..: every new member for an old j;
+        // for a new j the new members i < j, except in a node without old
+        // members (the first round: all pairs new x new), where a new j takes
+        // j - 1 - t (mod na), (na - 1) / 2 of them plus, for an even na, the
+        // opposite one when j >= na / 2: every unordered pair once, about
+        // na / 2 per member instead of j. (With old members the lanes of a
+        // wavefront would then read different rows in the same step instead
+        // of one broadcast row.)
+        const bool circ = j < na && na == ns;
+        const std::uint32_t iend =
+            circ ? (na - 1) / 2 + ((na % 2 == 0 && j >= na / 2) ? 1u : 0u) : (j < na ? j : na);
         float rj[STRIDE];
-        std::uint64_t to_i = 0, to_j = 0;  // bit r: pair (i0 + r G, j) proposes
+        std::uint64_t to_i = 0, to_j = 0;  // bit r: pair (partner i0 + r G, j) proposes
         {
             const float4* src = reinterpret_cast<const float4*>(s_rows + j * STRIDE);
 #pragma unroll
@@ -702,14 +721,21 @@ __global__ void __launch_bounds__(kJoinThreads) join_reg_kernel(JoinArgs a) {
                 rj[4 * q + 3] = f.w;
             }
             const float thr_j = s_thr[j];
-            std::uint64_t bit = 1;
-            for (std::uint32_t i = i0; i < iend; i += G, bit <<= 1) {
+            // the partner order is block-uniform (na == ns): one loop each
+            auto visit = [&](std::uint32_t i, std::uint64_t bit) {
                 const float d = l2sqr_reg<STRIDE>(rj, s_rows + i * STRIDE);
                 if (d < s_thr[i]) {
                     to_i |= bit;
                     atomicAdd(&s_cnt[i], 1u);
                 }
                 if (d < thr_j) to_j |= bit;
+            };
+            std::uint64_t bit = 1;
+            if (na == ns) {
+                for (std::uint32_t t = i0; t < iend; t += G, bit <<= 1)
+                    visit(join_partner(true, j, na, t), bit);
+            } else {
+                for (std::uint32_t t = i0; t < iend; t += G, bit <<= 1) visit(t, bit);
             }
             if (to_j) atomicAdd(&s_cnt[j], static_cast<std::uint32_t>(__popcll(to_j)));
         }
@@ -730,7 +756,7 @@ __global__ void __launch_bounds__(kJoinThreads) join_reg_kernel(JoinArgs a) {
             while (m) {
                 const std::uint32_t r = static_cast<std::uint32_t>(__ffsll(static_cast<unsigned long long>(m)) - 1);
                 m &= m - 1;
-                const std::uint32_t i = i0 + r * G;
+                const std::uint32_t i = join_partner(circ, j, na, i0 + r * G);
                 // a direction whose range starts at the capacity writes
                 // nothing: skip it, and the distance when both do
                 const bool wi = ((to_i >> r) & 1ull) && s_base[i] < a.cap;
