@@ -101,22 +101,40 @@ __global__ void bridge_count_kernel(const std::uint32_t* __restrict__ bu,
     }
 }
 
-// Scatter emitted slots and bridges into per-u segments (unordered within a
-// segment); `src` records a unique origin tag used as the final tie-break.
+// Scatter emitted slots into per-u segments (unordered within a segment);
+// `src` records a unique origin tag used as the final tie-break. Half a
+// wavefront per node p: p's own edges (q > p, u = p) take the first positions
+// of p's segment by a ballot prefix count, no atomic; edges that go to q's
+// segment (q < p) are placed by `cursor[q]`, which starts at q's own count.
 __global__ void __launch_bounds__(kThreads)
     scatter_kernel(const std::uint32_t* __restrict__ ids, const float* __restrict__ dist,
                    std::uint32_t n, std::uint32_t k, const std::uint8_t* __restrict__ emit,
                    const std::uint32_t* __restrict__ off, std::uint32_t* __restrict__ cursor,
                    uint4* __restrict__ tmp) {
-    const std::size_t total = std::size_t(n) * k;
-    for (std::size_t gid = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; gid < total;
-         gid += std::size_t(gridDim.x) * blockDim.x) {
-        if (!emit[gid]) continue;
-        const std::uint32_t p = static_cast<std::uint32_t>(gid / k);
-        const std::uint32_t q = ids[gid];
-        const std::uint32_t u = p < q ? p : q, v = p < q ? q : p;
-        const std::uint32_t pos = off[u] + atomicAdd(&cursor[u], 1u);
-        tmp[pos] = make_uint4(v, static_cast<std::uint32_t>(gid), __float_as_uint(dist[gid]), 0u);
+    constexpr std::uint32_t kSeg = 32;
+    const std::uint32_t lane = __lane_id(), sub = lane % kSeg, half = lane / kSeg;
+    const std::uint32_t segs = gridDim.x * (blockDim.x / kSeg);
+    const std::uint32_t below = (1u << sub) - 1u;
+    for (std::uint32_t p = (blockIdx.x * blockDim.x + threadIdx.x) / kSeg; p < n; p += segs) {
+        const std::uint32_t pos0 = off[p];
+        std::uint32_t own_before = 0;
+        for (std::uint32_t j0 = 0; j0 < k; j0 += kSeg) {
+            const std::uint32_t j = j0 + sub;
+            const std::size_t gid = std::size_t(p) * k + j;
+            const bool e = j < k && emit[gid];
+            const std::uint32_t q = e ? ids[gid] : 0u;
+            const bool own = e && q > p;
+            const std::uint32_t mask =
+                static_cast<std::uint32_t>(__ballot(own) >> (half * kSeg));
+            if (e) {
+                const std::uint32_t pos =
+                    own ? pos0 + own_before + __builtin_popcount(mask & below)
+                        : off[q] + atomicAdd(&cursor[q], 1u);
+                tmp[pos] = make_uint4(own ? q : p, static_cast<std::uint32_t>(gid),
+                                      __float_as_uint(dist[gid]), 0u);
+            }
+            own_before += __builtin_popcount(mask);
+        }
     }
 }
 
@@ -135,49 +153,56 @@ __global__ void bridge_scatter_kernel(const std::uint32_t* __restrict__ bu,
 }
 
 // One wavefront per u: rank each entry of u's segment by (v, src) and write it
-// to its final position. Segments of up to 64 entries rank in registers with
-// shuffles; longer ones (hubs) compare against the segment in memory. The
-// scattered entries are one 16-byte record each, (v, src, weight bits, 0), so
-// the scatter writes one transaction per entry.
+// to its final position e in the sorted list, and to both endpoints' entries of
+// the symmetric adjacency: u's first-endpoint entries keep the sorted order
+// (position adj_off[u] + rank), v's second-endpoint entries follow v's own
+// segment, placed by a cursor (`cursor[v]` starts at adj_off[v] + deg1[v]).
+// The scattered entries are one 16-byte record each, (v, src, weight bits, 0),
+// so the scatter writes one transaction per entry.
 __global__ void __launch_bounds__(kThreads)
     segment_sort_kernel(std::uint32_t n, const std::uint32_t* __restrict__ off,
-                        const uint4* __restrict__ tmp, std::uint32_t* __restrict__ eu,
-                        std::uint32_t* __restrict__ ev, float* __restrict__ ew) {
+                        const uint4* __restrict__ tmp, const std::uint32_t* __restrict__ adj_off,
+                        std::uint32_t* __restrict__ cursor, std::uint32_t* __restrict__ eu,
+                        std::uint32_t* __restrict__ ev, float* __restrict__ ew,
+                        AdjEntry* __restrict__ adj) {
     const std::uint32_t waves_per_block = blockDim.x / kWave;
     const std::uint32_t lane = threadIdx.x & (kWave - 1);
     for (std::uint32_t u = blockIdx.x * waves_per_block + threadIdx.x / kWave; u < n;
          u += gridDim.x * waves_per_block) {
         const std::uint32_t s = off[u], L = off[u + 1] - s;
-        if (L == 0) continue;
-        if (L <= kWave) {
-            const bool has = lane < L;
-            const uint4 t = has ? tmp[s + lane] : make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);
+        // Each lane ranks one entry of a 64-entry chunk against every chunk of
+        // the segment, broadcast from registers by readlane (segments of
+        // hundreds of entries exist: the k-NN graph's hubs).
+        for (std::uint32_t i0 = 0; i0 < L; i0 += kWave) {
+            const bool has = i0 + lane < L;
+            const uint4 t =
+                has ? tmp[s + i0 + lane] : make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);
             const std::uint32_t myv = t.x, mys = t.y;
             std::uint32_t rank = 0;
-            for (std::uint32_t j = 0; j < L; ++j) {  // j uniform: a readlane each
-                const std::uint32_t ov = static_cast<std::uint32_t>(
-                    __builtin_amdgcn_readlane(static_cast<int>(myv), static_cast<int>(j)));
-                const std::uint32_t os = static_cast<std::uint32_t>(
-                    __builtin_amdgcn_readlane(static_cast<int>(mys), static_cast<int>(j)));
-                rank += (ov < myv || (ov == myv && os < mys)) ? 1u : 0u;
+            for (std::uint32_t j0 = 0; j0 < L; j0 += kWave) {
+                const std::uint32_t jn = (L - j0 < kWave) ? L - j0 : kWave;
+                std::uint32_t cv = myv, cs = mys;
+                if (j0 != i0 && lane < jn) {
+                    const uint2 o = *reinterpret_cast<const uint2*>(tmp + s + j0 + lane);
+                    cv = o.x;
+                    cs = o.y;
+                }
+                for (std::uint32_t j = 0; j < jn; ++j) {  // j uniform: a readlane each
+                    const std::uint32_t ov = static_cast<std::uint32_t>(
+                        __builtin_amdgcn_readlane(static_cast<int>(cv), static_cast<int>(j)));
+                    const std::uint32_t os = static_cast<std::uint32_t>(
+                        __builtin_amdgcn_readlane(static_cast<int>(cs), static_cast<int>(j)));
+                    rank += (ov < myv || (ov == myv && os < mys)) ? 1u : 0u;
+                }
             }
             if (has) {
-                eu[s + rank] = u;
-                ev[s + rank] = myv;
-                ew[s + rank] = __uint_as_float(t.z);
-            }
-        } else {
-            for (std::uint32_t i = lane; i < L; i += kWave) {
-                const uint4 t = tmp[s + i];
-                const std::uint32_t myv = t.x, mys = t.y;
-                std::uint32_t rank = 0;
-                for (std::uint32_t j = 0; j < L; ++j) {
-                    const uint4 o = tmp[s + j];
-                    rank += (o.x < myv || (o.x == myv && o.y < mys)) ? 1u : 0u;
-                }
-                eu[s + rank] = u;
-                ev[s + rank] = myv;
-                ew[s + rank] = __uint_as_float(t.z);
+                const std::uint32_t e = s + rank;
+                const float w = __uint_as_float(t.z);
+                eu[e] = u;
+                ev[e] = myv;
+                ew[e] = w;
+                adj[adj_off[u] + rank] = AdjEntry{myv, e, w};
+                adj[atomicAdd(&cursor[myv], 1u)] = AdjEntry{u | kSecondEnd, e, w};
             }
         }
     }
@@ -222,27 +247,13 @@ __global__ void __launch_bounds__(kThreads)
         deg[x] += eoff[x + 1] - eoff[x];
 }
 
+// Second-endpoint entries of vertex x start after its first-endpoint segment.
 __global__ void __launch_bounds__(kThreads)
-    adj_fill_kernel(const std::uint32_t* __restrict__ eu, const std::uint32_t* __restrict__ ev,
-                    const float* __restrict__ ew, std::uint64_t m,
-                    const std::uint32_t* __restrict__ eoff, const std::uint32_t* __restrict__ off,
-                    std::uint32_t* __restrict__ cursor, std::uint32_t* __restrict__ nbr,
-                    std::uint32_t* __restrict__ rank, float* __restrict__ w) {
-    for (std::uint64_t e = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
-         e += std::uint64_t(gridDim.x) * blockDim.x) {
-        const std::uint32_t u = eu[e], v = ev[e];
-        const float we = ew[e];
-        // first-endpoint entries keep the sorted list's order (no atomic);
-        // second-endpoint entries follow them, placed by a cursor
-        const std::uint32_t pu = off[u] + static_cast<std::uint32_t>(e - eoff[u]);
-        nbr[pu] = v;
-        rank[pu] = static_cast<std::uint32_t>(e);
-        w[pu] = we;
-        const std::uint32_t pv = off[v] + (eoff[v + 1] - eoff[v]) + atomicAdd(&cursor[v], 1u);
-        nbr[pv] = u | kSecondEnd;
-        rank[pv] = static_cast<std::uint32_t>(e);
-        w[pv] = we;
-    }
+    adj_base_kernel(const std::uint32_t* __restrict__ eoff, const std::uint32_t* __restrict__ off,
+                    std::uint32_t n, std::uint32_t* __restrict__ cursor) {
+    for (std::uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < n;
+         x += gridDim.x * blockDim.x)
+        cursor[x] = off[x] + (eoff[x + 1] - eoff[x]);
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -278,10 +289,11 @@ void emit_state(Graph& g, std::uint64_t* d_total) {
 // (deg1: the edge list's segments) and per second endpoint v = max(p, q)
 // (deg2: the adjacency's second-endpoint entries): one thread per node p
 // counts its own side of each owned slot itself and adds the other side to q.
+// own[p]: p's edges from its own list with p as first endpoint.
 __global__ void __launch_bounds__(kThreads)
     emit_deg_kernel(const std::uint32_t* __restrict__ ids, const std::uint8_t* __restrict__ emit,
                     std::uint32_t n, std::uint32_t k, std::uint32_t* __restrict__ deg1,
-                    std::uint32_t* __restrict__ deg2) {
+                    std::uint32_t* __restrict__ deg2, std::uint32_t* __restrict__ own) {
     for (std::uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n;
          p += gridDim.x * blockDim.x) {
         std::uint32_t first = 0, second = 0;
@@ -299,6 +311,7 @@ __global__ void __launch_bounds__(kThreads)
         }
         if (first) atomicAdd(&deg1[p], first);
         if (second) atomicAdd(&deg2[p], second);
+        own[p] = first;
     }
 }
 }  // namespace
@@ -320,8 +333,12 @@ void prepare_edges(Graph& g) {
     fill_u32(g.eoff.p, std::size_t(n) + 1, 0u, s);
     g.adj_off.reserve(std::size_t(n) + 1);
     fill_u32(g.adj_off.p, std::size_t(n) + 1, 0u, s);
-    emit_deg_kernel<<<capped_blocks(n, kThreads), kThreads, 0, s>>>(g.ids.p, g.emit_flags.p, n,
-                                                                    k, g.eoff.p, g.adj_off.p);
+    // cursor: each first endpoint's next free position in its segment, past
+    // the edges from its own list
+    DBuf<std::uint32_t> cursor;
+    cursor.reserve(n);
+    emit_deg_kernel<<<capped_blocks(n, kThreads), kThreads, 0, s>>>(
+        g.ids.p, g.emit_flags.p, n, k, g.eoff.p, g.adj_off.p, cursor.p);
     CSG_LAUNCH_CHECK();
 
     DBuf<std::uint32_t> dbu, dbv;
@@ -347,12 +364,9 @@ void prepare_edges(Graph& g) {
     CSG_CHECK(hipStreamSynchronize(s));
     const std::uint64_t m = total;
 
-    DBuf<std::uint32_t> cursor;
     DBuf<uint4> tmp;  // (v, src, weight bits, 0) per edge, grouped by u
-    cursor.reserve(n);
-    fill_u32(cursor.p, n, 0u, s);
     tmp.reserve(m);
-    scatter_kernel<<<capped_blocks(slots, kThreads), kThreads, 0, s>>>(
+    scatter_kernel<<<capped_blocks(std::size_t(n) * 32, kThreads), kThreads, 0, s>>>(
         g.ids.p, g.dist.p, n, k, g.emit_flags.p, g.eoff.p, cursor.p, tmp.p);
     CSG_LAUNCH_CHECK();
     if (nb) {
@@ -361,28 +375,23 @@ void prepare_edges(Graph& g) {
             tmp.p);
         CSG_LAUNCH_CHECK();
     }
-    g.eu.reserve(m);
-    g.ev.reserve(m);
-    g.ew.reserve(m);
-    segment_sort_kernel<<<capped_blocks(std::size_t(n) * kWave, kThreads, 16384), kThreads, 0,
-                          s>>>(n, g.eoff.p, tmp.p, g.eu.p, g.ev.p, g.ew.p);
-    CSG_LAUNCH_CHECK();
-
-    // Symmetric adjacency for the vertex-centric Boruvka rounds: each vertex's
-    // first-endpoint segment plus its second-endpoint count (already in adj_off).
+    // Symmetric adjacency offsets: each vertex's first-endpoint segment plus
+    // its second-endpoint count (already in adj_off); the scatter is done with
+    // `cursor`, which now holds each vertex's second-endpoint start.
     first_degree_kernel<<<capped_blocks(n, kThreads), kThreads, 0, s>>>(g.eoff.p, n, g.adj_off.p);
     CSG_LAUNCH_CHECK();
     exclusive_scan_u32(g.adj_off.p, std::size_t(n) + 1, nullptr, g.scan_tmp.p, s);
-    g.adj_nbr.reserve(2 * m);
-    g.adj_rank.reserve(2 * m);
-    g.adj_w.reserve(2 * m);
-    if (m) {
-        fill_u32(cursor.p, n, 0u, s);
-        adj_fill_kernel<<<capped_blocks(m, kThreads), kThreads, 0, s>>>(
-            g.eu.p, g.ev.p, g.ew.p, m, g.eoff.p, g.adj_off.p, cursor.p, g.adj_nbr.p,
-            g.adj_rank.p, g.adj_w.p);
-        CSG_LAUNCH_CHECK();
-    }
+    adj_base_kernel<<<capped_blocks(n, kThreads), kThreads, 0, s>>>(g.eoff.p, g.adj_off.p, n,
+                                                                     cursor.p);
+    CSG_LAUNCH_CHECK();
+    g.eu.reserve(m);
+    g.ev.reserve(m);
+    g.ew.reserve(m);
+    g.adj.reserve(2 * m);
+    segment_sort_kernel<<<capped_blocks(std::size_t(n) * kWave, kThreads, 16384), kThreads, 0,
+                          s>>>(n, g.eoff.p, tmp.p, g.adj_off.p, cursor.p, g.eu.p, g.ev.p, g.ew.p,
+                               g.adj.p);
+    CSG_LAUNCH_CHECK();
     CSG_CHECK(hipStreamSynchronize(s));
     g.num_edges = m;
     std::uint64_t nbv = 0;

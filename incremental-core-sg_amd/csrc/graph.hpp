@@ -11,6 +11,14 @@
 
 namespace coresg_hip {
 
+// One symmetric-adjacency entry: the other endpoint (bit 31 when seen from the
+// edge's second endpoint), the edge's rank in the sorted list, its base
+// weight. Written and read as one 12-byte access.
+struct AdjEntry {
+    std::uint32_t nbr, rank;
+    float w;
+};
+
 // Owning device allocation (hipMalloc / hipFree) with its element count.
 template <typename T>
 struct DBuf {
@@ -107,9 +115,9 @@ struct Graph {
     // Symmetric CSR adjacency of the same edges: for vertex x, entries
     // adj_off[x] .. adj_off[x+1]-1 hold the other endpoint (bit 31 set when x
     // is the edge's second endpoint), the edge's rank in the sorted list and
-    // its base weight.
-    DBuf<std::uint32_t> adj_off, adj_nbr, adj_rank;
-    DBuf<float> adj_w;
+    // its base weight, one 12-byte record each.
+    DBuf<std::uint32_t> adj_off;
+    DBuf<AdjEntry> adj;
 
     // Extraction scratch.
     DBuf<float> core;

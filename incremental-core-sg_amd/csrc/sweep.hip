@@ -55,9 +55,7 @@ struct Sweep {
     std::uint64_t m;            // CORE-SG edges
     const std::uint32_t* live;  // device word, bit b: slot b still running
     const std::uint32_t* off;   // adjacency
-    const std::uint32_t* nbr;
-    const std::uint32_t* rank;
-    const float* w;
+    const AdjEntry* adj;
     const std::uint32_t* eu;  // edge list (by rank)
     const std::uint32_t* ev;
     float* core;  // n*B, interleaved
@@ -213,34 +211,29 @@ __global__ void __launch_bounds__(kThreads) sweep_round_kernel(Sweep s) {
             if (act) {
                 // four entries in flight per lane
                 for (; j + 3 * P < e1; j += 4 * P) {
-                    std::uint32_t raw[4], rk[4];
-                    float wj[4];
+                    AdjEntry a[4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        raw[u] = s.nbr[j + u * P];
-                        rk[u] = s.rank[j + u * P];
-                        wj[u] = s.w[j + u * P];
-                    }
+                    for (int u = 0; u < 4; ++u) a[u] = s.adj[j + u * P];
                     std::uint32_t cq_comp[4];
                     float cq[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
-                        const std::size_t iq = std::size_t(raw[u] & ~kSecondEnd) * s.B + b;
+                        const std::size_t iq = std::size_t(a[u].nbr & ~kSecondEnd) * s.B + b;
                         cq[u] = s.core[iq];
                         cq_comp[u] = FIRST ? ~0u : s.comp[iq];
                     }
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         if (!FIRST && cq_comp[u] == cv) continue;
-                        const std::uint64_t key = entry_key(raw[u], wj[u], cx, cq[u], rk[u]);
+                        const std::uint64_t key = entry_key(a[u].nbr, a[u].w, cx, cq[u], a[u].rank);
                         key_min = key < key_min ? key : key_min;
                     }
                 }
                 for (; j < e1; j += P) {
-                    const std::uint32_t raw = s.nbr[j];
-                    const std::size_t iq = std::size_t(raw & ~kSecondEnd) * s.B + b;
+                    const AdjEntry a = s.adj[j];
+                    const std::size_t iq = std::size_t(a.nbr & ~kSecondEnd) * s.B + b;
                     if (!FIRST && s.comp[iq] == cv) continue;
-                    const std::uint64_t key = entry_key(raw, s.w[j], cx, s.core[iq], s.rank[j]);
+                    const std::uint64_t key = entry_key(a.nbr, a.w, cx, s.core[iq], a.rank);
                     key_min = key < key_min ? key : key_min;
                 }
             }
@@ -669,9 +662,7 @@ void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_
     fill_u32(d_rounds.p, B, 0u, s);
     sw.live = d_live;
     sw.off = g.adj_off.p;
-    sw.nbr = g.adj_nbr.p;
-    sw.rank = g.adj_rank.p;
-    sw.w = g.adj_w.p;
+    sw.adj = g.adj.p;
     sw.eu = g.eu.p;
     sw.ev = g.ev.p;
     sw.core = g.core.p;
