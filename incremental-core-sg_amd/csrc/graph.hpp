@@ -59,6 +59,55 @@ struct DBuf {
     std::size_t bytes() const { return n * sizeof(T); }
 };
 
+// Pinned host allocation (hipHostMalloc / hipHostFree): the target of
+// asynchronous device-to-host copies.
+template <typename T>
+struct HostBuf {
+    T* p = nullptr;
+    std::size_t n = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    ~HostBuf() { release(); }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    // Ensure capacity for `count` elements (contents are not preserved).
+    void reserve(std::size_t count) {
+        if (count <= n && p) return;
+        release();
+        CSG_CHECK(hipHostMalloc(reinterpret_cast<void**>(&p), (count ? count : 1) * sizeof(T),
+                                hipHostMallocDefault));
+        n = count;
+    }
+};
+
+// Two reusable events (no timing).
+struct EventPair {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    EventPair() = default;
+    EventPair(const EventPair&) = delete;
+    EventPair& operator=(const EventPair&) = delete;
+    ~EventPair() {
+        for (hipEvent_t& x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+    void create() {
+        for (hipEvent_t& x : e)
+            if (!x) CSG_CHECK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+    }
+};
+
+// One NN-Descent round's counts as copied back to the host.
+struct RoundReadback {
+    unsigned long long upd;
+    unsigned long long tally[3];       // join work (profiling)
+    unsigned long long gram_tally[2];  // Gram-filter work (profiling)
+    unsigned int gram_audit[2];        // Gram-filter audit maxima (profiling)
+};
+
 // NN-Descent round scratch: sampled lists, bounded reverse lists, gathered
 // candidates and the per-node proposal buffers (kept across builds).
 struct DescentBuffers {
@@ -74,6 +123,10 @@ struct DescentBuffers {
     DBuf<float> gthr;
     DBuf<unsigned long long> gram_tally;
     DBuf<unsigned int> gram_audit;
+    // device-side convergence flag and the per-round counts read back
+    DBuf<std::uint32_t> halt;
+    HostBuf<RoundReadback> readback;
+    EventPair round_done;
 };
 
 // Accumulated device time of one kernel (HIP events on the graph's stream).

@@ -206,8 +206,9 @@ __global__ void __launch_bounds__(256)
                   std::uint32_t* __restrict__ rev_new_cnt, std::uint32_t* __restrict__ rev_old,
                   std::uint32_t* __restrict__ rev_old_cnt, std::uint32_t* __restrict__ hist,
                   std::uint32_t nb, const float* __restrict__ dist,
-                  float* __restrict__ thr_node) {
+                  float* __restrict__ thr_node, const std::uint32_t* __restrict__ halt) {
     extern __shared__ std::uint32_t s_hist[];  // nb buckets (BUCKETS only)
+    if (*halt) return;
     constexpr std::uint32_t P = 64 / SEG;  // nodes per wavefront
     const std::uint32_t lane = threadIdx.x & 63u;
     const std::uint32_t sub = lane % SEG, seg = lane / SEG;
@@ -285,8 +286,10 @@ __global__ void __launch_bounds__(256)
                        const std::uint32_t* __restrict__ old_list,
                        const std::uint32_t* __restrict__ old_cnt,
                        const std::uint32_t* __restrict__ off, std::uint32_t nb,
-                       unsigned long long* __restrict__ pairs) {
+                       unsigned long long* __restrict__ pairs,
+                       const std::uint32_t* __restrict__ halt) {
     extern __shared__ std::uint32_t s_cur[];  // nb bucket cursors
+    if (*halt) return;
     constexpr std::uint32_t P = 64 / SEG;
     const std::uint32_t lane = threadIdx.x & 63u;
     const std::uint32_t sub = lane % SEG, seg = lane / SEG;
@@ -322,8 +325,10 @@ __global__ void __launch_bounds__(kRevThreads)
                       const std::uint32_t* __restrict__ off, std::uint32_t nblk,
                       const std::uint32_t* __restrict__ total, std::uint32_t* __restrict__ rev_new,
                       std::uint32_t* __restrict__ rev_new_cnt, std::uint32_t* __restrict__ rev_old,
-                      std::uint32_t* __restrict__ rev_old_cnt) {
+                      std::uint32_t* __restrict__ rev_old_cnt,
+                      const std::uint32_t* __restrict__ halt) {
     __shared__ std::uint32_t s_cn[kRevTargets], s_co[kRevTargets];
+    if (*halt) return;
     const std::uint32_t b = blockIdx.x, lo = b << kRevShift;
     for (std::uint32_t t = threadIdx.x; t < kRevTargets; t += kRevThreads) {
         s_cn[t] = 0;
@@ -370,7 +375,9 @@ __global__ void __launch_bounds__(256)
                   const std::uint32_t* __restrict__ rev_old,
                   const std::uint32_t* __restrict__ rev_old_cnt,
                   std::uint32_t* __restrict__ out_ids, float* __restrict__ out_thr,
-                  std::uint32_t* __restrict__ out_cnt, const float* __restrict__ thr_node) {
+                  std::uint32_t* __restrict__ out_cnt, const float* __restrict__ thr_node,
+                  const std::uint32_t* __restrict__ halt) {
+    if (*halt) return;
     constexpr std::uint32_t P = 64 / SEG;  // nodes per wavefront
     constexpr std::uint32_t R = (6 * kMaxSample + SEG - 1) / SEG;  // entries per lane
     // per node: open-addressing table id -> smallest position (load <= 3/4)
@@ -469,6 +476,7 @@ struct JoinArgs {
     // both nullptr when not profiling
     unsigned long long* tally;
     unsigned int* audit;
+    const std::uint32_t* halt;  // nonzero: the descent has converged, skip
 };
 
 // Exact squared distance with `a` held in registers (compile-time stride),
@@ -621,6 +629,7 @@ __device__ __forceinline__ void join_emit(const JoinArgs& a, const std::uint32_t
 template <std::uint32_t STRIDE>
 __global__ void __launch_bounds__(kJoinThreads) join_kernel(JoinArgs a) {
     extern __shared__ float lds[];
+    if (*a.halt) return;
     const std::uint32_t S = 6u * a.smp;
     const std::uint32_t stride = STRIDE ? STRIDE : a.stride;
     std::uint32_t* s_id = reinterpret_cast<std::uint32_t*>(lds);
@@ -673,6 +682,7 @@ template <std::uint32_t STRIDE>
 __global__ void __launch_bounds__(kJoinThreads) join_reg_kernel(JoinArgs a) {
     static_assert(STRIDE % 8 == 0 && STRIDE <= kJoinRegStride, "row kept in registers");
     extern __shared__ float lds[];
+    if (*a.halt) return;
     const std::uint32_t S = 6u * a.smp;  // <= kJoinThreads (host-checked)
     std::uint32_t* s_id = reinterpret_cast<std::uint32_t*>(lds);
     float* s_thr = lds + S;
@@ -879,6 +889,7 @@ __device__ __forceinline__ unsigned int wave_max_u32(unsigned int v) {
 template <std::uint32_t STRIDE, bool BF16>
 __global__ void __launch_bounds__(kJoinThreads) join_gram_kernel(JoinArgs a) {
     extern __shared__ float lds[];
+    if (*a.halt) return;
     const std::uint32_t S = 6u * a.smp, A = 2u * a.smp, P = S + 1;
     const std::uint32_t stride = STRIDE ? STRIDE : a.stride;
     const std::uint32_t KP = (stride + 15u) & ~15u;
@@ -1208,8 +1219,9 @@ __global__ void __launch_bounds__(64)
                  float* __restrict__ dist, std::uint8_t* __restrict__ flags,
                  std::uint32_t* __restrict__ count, const unsigned long long* __restrict__ cand,
                  std::uint32_t* __restrict__ fill, std::uint32_t cap,
-                 unsigned long long* __restrict__ partial) {
+                 unsigned long long* __restrict__ partial, const std::uint32_t* __restrict__ halt) {
     __shared__ unsigned long long s_key[kMaxCap];
+    if (halt && *halt) return;
     __shared__ unsigned long long s_top[kMaxK];
     __shared__ float s_ld[kMaxK], s_td[kMaxK];
     __shared__ std::uint32_t s_list[kListSet];
@@ -1310,19 +1322,23 @@ __global__ void __launch_bounds__(64)
 
 using MergeFn = void (*)(std::uint32_t, std::uint32_t, std::uint32_t, std::uint32_t*, float*,
                          std::uint8_t*, std::uint32_t*, const unsigned long long*, std::uint32_t*,
-                         std::uint32_t, unsigned long long*);
+                         std::uint32_t, unsigned long long*, const std::uint32_t*);
 
 MergeFn pick_merge(std::uint32_t) { return merge_kernel; }
 
 constexpr std::uint32_t kMergeBlocks = 65536;
 
 // One block of kSumThreads adds the per-block partials (loads unrolled so
-// that many are in flight per thread).
+// that many are in flight per thread). With `halt`, a total at or below
+// `stop` sets it (the descent has converged: the next round's kernels return
+// at once), and a round that was skipped leaves the total as it was.
 constexpr unsigned kSumThreads = 1024;
 __global__ void __launch_bounds__(kSumThreads)
     sum_partials_kernel(const unsigned long long* __restrict__ partial, std::uint32_t m,
-                        unsigned long long* __restrict__ out) {
+                        unsigned long long* __restrict__ out, std::uint32_t* __restrict__ halt,
+                        double stop) {
     __shared__ unsigned long long s_w[kSumThreads / 64];
+    if (halt && *halt) return;
     unsigned long long v = 0;
 #pragma unroll 8
     for (std::uint32_t i = threadIdx.x; i < m; i += kSumThreads) v += partial[i];
@@ -1333,6 +1349,7 @@ __global__ void __launch_bounds__(kSumThreads)
         unsigned long long t = 0;
         for (unsigned w = 0; w < kSumThreads / 64; ++w) t += s_w[w];
         out[0] = t;
+        if (halt && double(t) <= stop) *halt = 1u;
     }
 }
 
@@ -1352,14 +1369,19 @@ __global__ void __launch_bounds__(256)
         dists += static_cast<unsigned long long>(na) * ns - static_cast<unsigned long long>(na) * (na + 1) / 2;
         props += fill[v] < cap ? fill[v] : cap;
     }
+    __shared__ unsigned long long s_t[3][4];
     rows = wave_sum(rows);
     dists = wave_sum(dists);
     props = wave_sum(props);
     if ((threadIdx.x & 63u) == 0) {
-        atomicAdd(&out[0], rows);
-        atomicAdd(&out[1], dists);
-        atomicAdd(&out[2], props);
+        s_t[0][threadIdx.x >> 6] = rows;
+        s_t[1][threadIdx.x >> 6] = dists;
+        s_t[2][threadIdx.x >> 6] = props;
     }
+    __syncthreads();
+    if (threadIdx.x < 3)
+        atomicAdd(&out[threadIdx.x], s_t[threadIdx.x][0] + s_t[threadIdx.x][1] +
+                                         s_t[threadIdx.x][2] + s_t[threadIdx.x][3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1544,8 +1566,17 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         CSG_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(join_fn),
                                       hipFuncAttributeMaxDynamicSharedMemorySize,
                                       static_cast<int>(lds)));
+    // Convergence is decided on the device: the round whose update count is at
+    // or below `stop` sets `halt`, and every kernel of the rounds after it
+    // returns at once. The host enqueues round r + 1 before it reads round
+    // r's counts (through pinned memory and an event), so the device never
+    // waits for the host between rounds.
+    b.halt.reserve(1);
+    fill_u32(b.halt.p, 1, 0u, s);
+    b.readback.reserve(iters);
+    b.round_done.create();
     JoinArgs ja{pts, n, stride, smp, b.gids.p, b.gthr.p, b.gcnt.p, b.cand.p, b.fill.p, cap,
-                nullptr, nullptr};
+                nullptr, nullptr, b.halt.p};
     const unsigned merge_blocks = static_cast<unsigned>(std::min<std::size_t>(n, kMergeBlocks));
     const bool tallies = gp && gp->profiling;
     const bool gram_audit = tallies && gram != kGramExact;
@@ -1556,7 +1587,24 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         ja.audit = gp->audit ? b.gram_audit.p : nullptr;
     }
     const double stop = delta * double(k) * double(n);
+    // Reads round r's counts once its copies have landed; true: it converged.
+    auto consume = [&](std::uint32_t r) -> bool {
+        CSG_CHECK(hipEventSynchronize(b.round_done.e[r & 1u]));
+        const RoundReadback& rb = b.readback.p[r];
+        if (tallies) gp->add_work("nnd_join", {rb.tally[0], rb.tally[1], rb.tally[2], n});
+        if (gram_audit) {
+            gp->add_work("nnd_gram", {rb.gram_tally[0], rb.gram_tally[1], 0, 0});
+            gp->max_work("nnd_gram", 2, rb.gram_audit[0]);
+            gp->max_work("nnd_gram", 3, rb.gram_audit[1]);
+        }
+        res.updates += rb.upd;
+        res.rounds = r + 1;
+        return double(rb.upd) <= stop;
+    };
+    std::uint32_t last = 0;
+    bool converged = false;
     for (std::uint32_t round = 0; round < iters; ++round) {
+        last = round;
         {
             std::unique_ptr<ScopedKernelTimer> t;
             if (gp) t.reset(new ScopedKernelTimer(*gp, "nnd_sample"));
@@ -1568,7 +1616,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
                 sample_fn<<<kRevBlocks, 256, hl, s>>>(
                     n, k, ids, flags, count, smp, b.new_list.p, b.new_cnt.p, b.old_list.p,
                     b.old_cnt.p, b.rev_new.p, b.rev_new_cnt.p, b.rev_old.p, b.rev_old_cnt.p,
-                    b.rev_hist.p, nb, dist, b.thr_node.p);
+                    b.rev_hist.p, nb, dist, b.thr_node.p, b.halt.p);
                 CSG_LAUNCH_CHECK();
                 exclusive_scan_u32(b.rev_hist.p, std::size_t(nb) * kRevBlocks, b.rev_total.p,
                                    b.rev_tmp.p, s);
@@ -1577,11 +1625,11 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
                                                   : rev_scatter_kernel<64>;
                 scatter_fn<<<kRevBlocks, 256, hl, s>>>(n, smp, b.new_list.p, b.new_cnt.p,
                                                        b.old_list.p, b.old_cnt.p, b.rev_hist.p,
-                                                       nb, b.rev_pairs.p);
+                                                       nb, b.rev_pairs.p, b.halt.p);
                 CSG_LAUNCH_CHECK();
                 rev_bucket_kernel<<<nb, kRevThreads, 0, s>>>(
                     n, smp, b.rev_pairs.p, b.rev_hist.p, kRevBlocks, b.rev_total.p, b.rev_new.p,
-                    b.rev_new_cnt.p, b.rev_old.p, b.rev_old_cnt.p);
+                    b.rev_new_cnt.p, b.rev_old.p, b.rev_old_cnt.p, b.halt.p);
                 CSG_LAUNCH_CHECK();
             } else {
                 fill_u32(b.rev_new_cnt.p, n, 0u, s);
@@ -1592,7 +1640,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
                 sample_fn<<<wave_blocks, 256, 0, s>>>(
                     n, k, ids, flags, count, smp, b.new_list.p, b.new_cnt.p, b.old_list.p,
                     b.old_cnt.p, b.rev_new.p, b.rev_new_cnt.p, b.rev_old.p, b.rev_old_cnt.p,
-                    nullptr, 0, dist, b.thr_node.p);
+                    nullptr, 0, dist, b.thr_node.p, b.halt.p);
                 CSG_LAUNCH_CHECK();
             }
         }
@@ -1602,7 +1650,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
             gather_kernel<32><<<wave_blocks, 256, 0, s>>>(
                 n, smp, b.new_list.p, b.new_cnt.p, b.old_list.p, b.old_cnt.p,
                 b.rev_new.p, b.rev_new_cnt.p, b.rev_old.p, b.rev_old_cnt.p, b.gids.p, b.gthr.p,
-                b.gcnt.p, b.thr_node.p);
+                b.gcnt.p, b.thr_node.p, b.halt.p);
             CSG_LAUNCH_CHECK();
         }
         if (gram_audit) {
@@ -1617,38 +1665,37 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         }
         if (tallies) {
             fill_u64(reinterpret_cast<std::uint64_t*>(b.tally.p), 3, 0ull, s);
-            join_tally_kernel<<<256, 256, 0, s>>>(n, b.gcnt.p, b.fill.p, cap, b.tally.p);
+            join_tally_kernel<<<capped_blocks(n, 256, 2048), 256, 0, s>>>(n, b.gcnt.p, b.fill.p,
+                                                                          cap, b.tally.p);
             CSG_LAUNCH_CHECK();
         }
         {
             std::unique_ptr<ScopedKernelTimer> t;
             if (gp) t.reset(new ScopedKernelTimer(*gp, "nnd_merge"));
             pick_merge(k)<<<merge_blocks, 64, 0, s>>>(n, n, k, ids, dist, flags, count, b.cand.p,
-                                                     b.fill.p, cap, b.partial.p);
+                                                     b.fill.p, cap, b.partial.p, b.halt.p);
             CSG_LAUNCH_CHECK();
         }
-        sum_partials_kernel<<<1, kSumThreads, 0, s>>>(b.partial.p, merge_blocks, b.updates.p);
+        sum_partials_kernel<<<1, kSumThreads, 0, s>>>(b.partial.p, merge_blocks, b.updates.p,
+                                                      b.halt.p, stop);
         CSG_LAUNCH_CHECK();
-        unsigned long long upd = 0, tl[3] = {0, 0, 0};
-        CSG_CHECK(hipMemcpyAsync(&upd, b.updates.p, 8, hipMemcpyDeviceToHost, s));
-        if (tallies) CSG_CHECK(hipMemcpyAsync(tl, b.tally.p, sizeof(tl), hipMemcpyDeviceToHost, s));
-        unsigned long long gt[2] = {0, 0};
-        unsigned int ga[2] = {0, 0};
+        RoundReadback* rb = b.readback.p + round;
+        CSG_CHECK(hipMemcpyAsync(&rb->upd, b.updates.p, 8, hipMemcpyDeviceToHost, s));
+        if (tallies)
+            CSG_CHECK(hipMemcpyAsync(rb->tally, b.tally.p, 24, hipMemcpyDeviceToHost, s));
         if (gram_audit) {
-            CSG_CHECK(hipMemcpyAsync(gt, b.gram_tally.p, sizeof(gt), hipMemcpyDeviceToHost, s));
-            CSG_CHECK(hipMemcpyAsync(ga, b.gram_audit.p, sizeof(ga), hipMemcpyDeviceToHost, s));
+            CSG_CHECK(hipMemcpyAsync(rb->gram_tally, b.gram_tally.p, 16, hipMemcpyDeviceToHost, s));
+            CSG_CHECK(hipMemcpyAsync(rb->gram_audit, b.gram_audit.p, 8, hipMemcpyDeviceToHost, s));
         }
-        CSG_CHECK(hipStreamSynchronize(s));
-        if (tallies) gp->add_work("nnd_join", {tl[0], tl[1], tl[2], n});
-        if (gram_audit) {
-            gp->add_work("nnd_gram", {gt[0], gt[1], 0, 0});
-            gp->max_work("nnd_gram", 2, ga[0]);
-            gp->max_work("nnd_gram", 3, ga[1]);
+        CSG_CHECK(hipEventRecord(b.round_done.e[round & 1u], s));
+        if (round > 0 && consume(round - 1)) {
+            converged = true;  // this round's kernels returned at once
+            break;
         }
-        res.updates += upd;
-        res.rounds = round + 1;
-        if (double(upd) <= stop) break;
     }
+    if (!converged) consume(last);
+    // the skipped round's copies land in `readback` too: wait for them
+    CSG_CHECK(hipEventSynchronize(b.round_done.e[last & 1u]));
     return res;
 }
 
@@ -2398,9 +2445,10 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
             const unsigned merge_blocks = static_cast<unsigned>(std::min<std::size_t>(n0, kMergeBlocks));
             pick_merge(k)<<<merge_blocks, 64, 0, s>>>(n0, n1, k, g.ids.p, g.dist.p, g.flags.p,
                                                      g.count.p, b.cand.p, b.fill.p, cap,
-                                                     b.partial.p);
+                                                     b.partial.p, nullptr);
             CSG_LAUNCH_CHECK();
-            sum_partials_kernel<<<1, kSumThreads, 0, s>>>(b.partial.p, merge_blocks, b.updates.p);
+            sum_partials_kernel<<<1, kSumThreads, 0, s>>>(b.partial.p, merge_blocks, b.updates.p,
+                                                          nullptr, 0.0);
             CSG_LAUNCH_CHECK();
             unsigned long long upd = 0;
             CSG_CHECK(hipMemcpyAsync(&upd, b.updates.p, 8, hipMemcpyDeviceToHost, s));
