@@ -266,6 +266,38 @@ def test_recall_at_scale_vs_reference_cli():
     assert ours >= ref[0]
 
 
+def test_descent_stops_on_device():
+    """Convergence is decided on the device: the round whose update count is
+    at or below delta * k * n stops the descent, and the rounds already
+    enqueued after it leave the lists untouched. delta = 1000 stops after
+    round 1 (a node counts at most one update per proposal it receives, and
+    it receives at most the join capacity, far below 1000 k); the lists must
+    then look like a plain one-round build's: a second sampling pass would
+    have turned most of their new entries old (k = 16 <= sample)."""
+    print("[gpu] descent stops on device", flush=True)
+    n, k = 20000, 16
+    x = datagen.gaussian_mixture(n, 16, 20, 0.5, 2.0, 0.05, seed=3)
+    news = []
+    for delta, iters in ((1000.0, 10), (0.0, 1)):
+        g = chip.Graph(n, 16, k)
+        g.set_points(x)
+        st = g.build_knn(chip.build_params(sample=16, iters=iters, delta=delta))
+        _, _, flags, count = g.get_knn()
+        g.close()
+        assert st["rounds"] == 1 and st["updates"] > 0
+        assert int(count.min()) == k
+        news.append(int(np.count_nonzero(flags)))
+    print(f"  new entries after one round: stopped {news[0]}, plain {news[1]}", flush=True)
+    assert news[1] > n  # the merge marks its insertions new
+    assert abs(news[0] - news[1]) <= 0.05 * news[1]
+    # a mid-run stop: fewer rounds than allowed, each counted once
+    g = chip.Graph(n, 16, k)
+    g.set_points(x)
+    st = g.build_knn(chip.build_params(sample=16, iters=30, delta=0.01))
+    g.close()
+    assert 1 < st["rounds"] < 30
+
+
 @pytest.mark.parametrize("sample", [16, 24])
 @pytest.mark.parametrize("d", [5, 16, 29, 40, 64, 96])
 def test_join_strides_exact_distances(d, sample):
