@@ -43,12 +43,15 @@ constexpr std::uint32_t kMaxK = 64;        // list length handled by one wavefro
 constexpr std::uint32_t kMaxSample = 32;   // reference cap (cuda_common.cuh:32)
 constexpr std::uint32_t kJoinThreads = 128;
 // reverse lists by target buckets: kRevTargets targets per bucket, at most
-// kRevMaxBuckets buckets (n <= 2^24; larger graphs use one device atomic per
-// entry), sampling and scatter on a grid of kRevBlocks blocks
-constexpr std::uint32_t kRevShift = 11;
+// kRevMaxBuckets buckets (n <= 2^25; larger graphs use one device atomic per
+// entry), sampling and scatter on a grid of kRevBlocks blocks. Every block
+// writes one run per bucket: 4096-target buckets and 2048 blocks keep these
+// runs long (about 100 entries at n = 1M, k = 32), so their partly written
+// lines stay few.
+constexpr std::uint32_t kRevShift = 12;
 constexpr std::uint32_t kRevTargets = 1u << kRevShift;
 constexpr std::uint32_t kRevMaxBuckets = 8192;
-constexpr unsigned kRevBlocks = 4096;
+constexpr unsigned kRevBlocks = 2048;
 constexpr unsigned kRevThreads = 1024;
 constexpr std::uint32_t kJoinRegStride = 64;  // widest row a join thread keeps in registers
 
