@@ -1569,11 +1569,11 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         CSG_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(join_fn),
                                       hipFuncAttributeMaxDynamicSharedMemorySize,
                                       static_cast<int>(lds)));
-    // Convergence is decided on the device: the round whose update count is at
-    // or below `stop` sets `halt`, and every kernel of the rounds after it
-    // returns at once. The host enqueues round r + 1 before it reads round
-    // r's counts (through pinned memory and an event), so the device never
-    // waits for the host between rounds.
+    // Convergence (nndescent.cpp:172, 204) is decided on the device: the round
+    // whose update count is at or below `stop` sets `halt`, and every kernel of
+    // the rounds after it returns at once. The host enqueues round r + 1
+    // before it reads round r's counts (through pinned memory and an event),
+    // so the device never waits for the host between rounds.
     b.halt.reserve(1);
     fill_u32(b.halt.p, 1, 0u, s);
     b.readback.reserve(iters);
