@@ -41,6 +41,7 @@ namespace {
 
 constexpr std::uint32_t kMaxK = 64;        // list length handled by one wavefront
 constexpr std::uint32_t kMaxSample = 32;   // reference cap (cuda_common.cuh:32)
+constexpr std::uint32_t kSmallMerge = 16;  // merge survivors ranked by counting
 constexpr std::uint32_t kJoinThreads = 128;
 // reverse lists by target buckets: kRevTargets targets per bucket, at most
 // kRevMaxBuckets buckets (n <= 2^25; larger graphs use one device atomic per
@@ -1105,6 +1106,43 @@ __device__ __forceinline__ std::uint32_t sort_unique_top(const unsigned long lon
     return static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
 }
 
+// The same result as sort_unique_top<1> for nv <= kSmallMerge survivors, by
+// counting instead of a 64-key sorting network (about 10 nv VALU operations
+// against about 230): each lane's key is compared with every survivor,
+// broadcast by readlane, once to find later copies of a key and once more,
+// over first copies only, for its rank among the distinct keys.
+__device__ __forceinline__ std::uint32_t small_unique_top(const unsigned long long* s_key,
+                                                          std::uint32_t nv, std::uint32_t k,
+                                                          unsigned long long* s_top) {
+    const std::uint32_t lane = __lane_id();
+    const unsigned long long key = lane < nv ? s_key[lane] : ~0ull;
+    const int lo = static_cast<int>(static_cast<std::uint32_t>(key));
+    const int hi = static_cast<int>(static_cast<std::uint32_t>(key >> 32));
+    // survivors are never ~0 (the empty key), so lanes >= nv find no copy
+    bool dup = false;
+    for (std::uint32_t j = 0; j < nv; ++j) {
+        const unsigned long long kj =
+            (static_cast<unsigned long long>(static_cast<std::uint32_t>(
+                 __builtin_amdgcn_readlane(hi, static_cast<int>(j))))
+             << 32) |
+            static_cast<std::uint32_t>(__builtin_amdgcn_readlane(lo, static_cast<int>(j)));
+        dup = dup || (kj == key && j < lane);
+    }
+    const std::uint64_t dmask = __ballot(dup);
+    std::uint32_t rank = 0;
+    for (std::uint32_t j = 0; j < nv; ++j) {
+        if ((dmask >> j) & 1ull) continue;  // j uniform: a later copy is not counted
+        const unsigned long long kj =
+            (static_cast<unsigned long long>(static_cast<std::uint32_t>(
+                 __builtin_amdgcn_readlane(hi, static_cast<int>(j))))
+             << 32) |
+            static_cast<std::uint32_t>(__builtin_amdgcn_readlane(lo, static_cast<int>(j)));
+        rank += kj < key ? 1u : 0u;
+    }
+    if (lane < nv && !dup && rank < k) s_top[rank] = key;
+    return nv - static_cast<std::uint32_t>(__popcll(dmask));
+}
+
 // The same result as sort_unique_top<R> for 64 R / 2 < nv <= 64 R survivors
 // (nv <= kMaxCap), without sorting them all: the keys go to registers; a
 // sorted sample of 64 (spread over s_key) gives a threshold T with about 72
@@ -1277,7 +1315,8 @@ __global__ void __launch_bounds__(64)
             // 2. sort, keep first copies, rank
             std::uint32_t nvalid = 0;
             if (nv > 0) {
-                if (nv <= 64) nvalid = sort_unique_top<1>(s_key, nv, k, s_top);
+                if (nv <= kSmallMerge) nvalid = small_unique_top(s_key, nv, k, s_top);
+                else if (nv <= 64) nvalid = sort_unique_top<1>(s_key, nv, k, s_top);
                 else if (nv <= 128) nvalid = sort_unique_top<2>(s_key, nv, k, s_top);
                 else if (nv <= 256) nvalid = select_top_large<4>(s_key, nv, k, s_top);
                 else nvalid = select_top_large<8>(s_key, nv, k, s_top);
