@@ -135,6 +135,9 @@ def lib():
         L.orc_sg_mst.restype = C.c_size_t
         L.orc_sg_mst.argtypes = [C.c_void_p, C.c_uint32, _u32p, _u32p, _f32p, _u32p,
                                  C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
+        L.orc_sg_exact_mst.restype = C.c_size_t
+        L.orc_sg_exact_mst.argtypes = [C.c_void_p, C.c_uint32, _u32p, _u32p, _f32p, _u32p,
+                                       C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
         L.orc_sg_sampled_recall.restype = C.c_double
         L.orc_sg_sampled_recall.argtypes = [C.c_void_p, C.c_size_t, C.c_uint64,
                                             C.POINTER(C.c_double)]
@@ -404,6 +407,21 @@ class CoreSG:
         nc = C.c_uint32()
         wt = C.c_double()
         t = lib().orc_sg_mst(self.h, mpts, u, v, w, lab, C.byref(nc), C.byref(wt))
+        if t == C.c_size_t(-1).value:
+            raise ValueError("mpts exceeds k_max")
+        return dict(u=u[:t], v=v[:t], w=w[:t], labels=lab[:self.n], ncomp=nc.value,
+                    weight=wt.value)
+
+    def exact_mst(self, mpts: int):
+        """CoreSG::exact_mst: Prim over the complete MRD graph (coresg.cpp:419-430)."""
+        n = max(self.n, 1)
+        u = np.empty(n, np.uint32)
+        v = np.empty(n, np.uint32)
+        w = np.empty(n, np.float32)
+        lab = np.empty(n, np.uint32)
+        nc = C.c_uint32()
+        wt = C.c_double()
+        t = lib().orc_sg_exact_mst(self.h, mpts, u, v, w, lab, C.byref(nc), C.byref(wt))
         if t == C.c_size_t(-1).value:
             raise ValueError("mpts exceeds k_max")
         return dict(u=u[:t], v=v[:t], w=w[:t], labels=lab[:self.n], ncomp=nc.value,

@@ -1215,6 +1215,81 @@ size_t orc_sg_mst(const orc_sg* sg, uint32_t mpts, uint32_t* out_u, uint32_t* ou
     return t;
 }
 
+/* src/coresg.cpp:419-430 (CoreSG::exact_mst): core distances for mpts
+ * (coresg.cpp:68-70), then mst_prim_virtual (mst.cpp:172-220) whose row i is
+ * CpuBackend::mrd_block's row (cpu_backend.cpp:61-90), i.e. orc_mrd_block.
+ * Edges in Prim acceptance order, canonicalised u<v; labels and component
+ * count from a union-find over them (mst.cpp:211-212, :43-46); weight = the
+ * double sum in acceptance order (types.hpp:58-62). Returns the edge count,
+ * or (size_t)-1 for mpts above k_max. */
+size_t orc_sg_exact_mst(const orc_sg* sg, uint32_t mpts, uint32_t* out_u, uint32_t* out_v,
+                        float* out_w, uint32_t* out_labels, uint32_t* out_ncomp,
+                        double* out_weight) {
+    if (mpts == 0) mpts = 1;
+    if (mpts > sg->k) return (size_t)-1;
+    const uint32_t n = sg->n;
+    float* core = (float*)malloc((n ? n : 1) * 4);
+    orc_core_distances(sg->dist, sg->count, n, sg->k, mpts, core);
+    float* row = (float*)malloc((n ? n : 1) * 4);
+    float* best = (float*)malloc((n ? n : 1) * 4);
+    uint32_t* from = (uint32_t*)malloc((n ? n : 1) * 4);
+    uint8_t* in_tree = (uint8_t*)calloc(n ? n : 1, 1);
+    union_find uf;
+    uf_init(&uf, n);
+    size_t t = 0;
+    double weight = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        best[i] = INFINITY;
+        from[i] = ORC_INVALID;
+    }
+    if (n >= 2) {
+        uint32_t current = 0;
+        in_tree[0] = 1;
+        best[0] = 0.0f;
+        for (uint32_t step = 1; step < n; ++step) {
+            orc_mrd_block(sg->pts, n, sg->stride, core, current, (size_t)current + 1, row);
+            for (uint32_t j = 0; j < n; ++j) {
+                if (in_tree[j]) continue;
+                if (row[j] < best[j]) {
+                    best[j] = row[j];
+                    from[j] = current;
+                }
+            }
+            uint32_t pick = ORC_INVALID;
+            float pick_w = INFINITY;
+            for (uint32_t j = 0; j < n; ++j) {
+                if (in_tree[j]) continue;
+                if (best[j] < pick_w || (best[j] == pick_w && pick == ORC_INVALID)) {
+                    pick_w = best[j];
+                    pick = j;
+                }
+            }
+            if (pick == ORC_INVALID) break;
+            in_tree[pick] = 1;
+            if (from[pick] != ORC_INVALID && pick_w < INFINITY) {
+                const uint32_t a = from[pick], b = pick;
+                out_u[t] = a < b ? a : b;
+                out_v[t] = a < b ? b : a;
+                out_w[t] = pick_w;
+                weight += (double)pick_w;
+                uf_unite(&uf, a, b);
+                ++t;
+            }
+            current = pick;
+        }
+    }
+    if (out_labels) uf_labels(&uf, out_labels);
+    if (out_ncomp) *out_ncomp = n ? uf.components : 0;
+    if (out_weight) *out_weight = weight;
+    uf_free(&uf);
+    free(core);
+    free(row);
+    free(best);
+    free(from);
+    free(in_tree);
+    return t;
+}
+
 /* src/eval.cpp:24-97 (sampled_recall): queries drawn by a partial
  * Fisher-Yates over mt19937_64(seed); exact top-k of each query by squared
  * distance, i.e. the k smallest (rank, id) pairs; recall = mean hit fraction;

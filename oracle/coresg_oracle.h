@@ -127,6 +127,9 @@ size_t orc_sg_edges(const orc_sg* sg, uint32_t mpts, uint32_t* u, uint32_t* v, f
 /* CoreSG::mst(mpts) with the single-threaded (Kruskal) schedule. */
 size_t orc_sg_mst(const orc_sg* sg, uint32_t mpts, uint32_t* out_u, uint32_t* out_v,
                   float* out_w, uint32_t* out_labels, uint32_t* out_ncomp, double* out_weight);
+size_t orc_sg_exact_mst(const orc_sg* sg, uint32_t mpts, uint32_t* out_u, uint32_t* out_v,
+                        float* out_w, uint32_t* out_labels, uint32_t* out_ncomp,
+                        double* out_weight);
 /* eval.cpp sampled_recall: returns recall, writes distance ratio. */
 double orc_sg_sampled_recall(const orc_sg* sg, size_t queries, uint64_t seed, double* ratio);
 

@@ -16,6 +16,7 @@
 //   brute  DATA DIM K OUT            brute_force_knn
 //   state  CORESG OUT                hubness / connectivity / edges after load()
 //   mst    CORESG MPTS OUT           CoreSG::mst(mpts), Kruskal order
+//   exact  CORESG MPTS OUT           CoreSG::exact_mst(mpts), Prim acceptance order
 //   mrd    DATA DIM IDS CORE OUT     CPU mrd_block over a gathered subset
 //   recall CORESG QUERIES SEED OUT   sampled_recall against brute force
 
@@ -208,6 +209,20 @@ int run(int argc, char** argv) {
         std::vector<double> w = {r.weight};
         t.vec("weight", 10, w);
         dump_edges(t, "edges", sg.edges(mpts));
+        return 0;
+    }
+    if (cmd == "exact" && argc == 5) {
+        CoreSG sg(single_thread_params());
+        sg.load(argv[2]);
+        const std::uint32_t mpts = std::strtoul(argv[3], nullptr, 10);
+        MstResult r = sg.exact_mst(mpts);
+        Tagged t(argv[4]);
+        dump_edges(t, "mst", r.edges);
+        t.vec("component", 4, r.component);
+        std::vector<std::uint32_t> nc = {r.num_components};
+        t.vec("num_components", 4, nc);
+        std::vector<double> w = {r.weight};
+        t.vec("weight", 10, w);
         return 0;
     }
     if (cmd == "mrd" && argc == 7) {

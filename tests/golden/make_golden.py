@@ -6,7 +6,7 @@ deterministic serial one) on small seeded inputs, and stores inputs plus the
 reference's outputs as compressed .npz data fixtures next to this script. The
 reference is only needed to *write* fixtures; tests read the committed files.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [--exact-only]
 """
 from __future__ import annotations
 
@@ -129,11 +129,36 @@ def make_case(name, spec, tmp):
           f" selected={out['state_selected'].size}")
 
 
+EXACT_MPTS = {"mix_d16": [1, 4, 16], "mix_d24": [2, 32], "brute_d8": [1, 12], "islands": [3],
+              "islands_repair": [5], "ties": [1, 3, 8]}
+
+
+def make_exact(tmp):
+    """CoreSG::exact_mst (Prim over the complete MRD graph) for some cases, read
+    from the .coresg bytes already stored in each case's fixture."""
+    out = {}
+    dump = os.path.join(tmp, "dump.bin")
+    for name, mpts in EXACT_MPTS.items():
+        with np.load(os.path.join(HERE, f"{name}.npz")) as z:
+            raw = z["coresg_bytes"]
+        sg_path = os.path.join(tmp, f"{name}.coresg")
+        raw.tofile(sg_path)
+        for m in mpts:
+            run([os.path.join(REF, "ref_dump"), "exact", sg_path, str(m), dump])
+            t = read_tagged(dump)
+            for key in ("mst_u", "mst_v", "mst_w", "component", "num_components", "weight"):
+                out[f"{name}_exact{m}_{key}"] = t[key]
+        print(f"exact_mst: {name} mpts={mpts}")
+    np.savez_compressed(os.path.join(HERE, "exact_mst.npz"), **out)
+
+
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle"), "ref"], check=True)
     with tempfile.TemporaryDirectory() as tmp:
-        for name, spec in CASES.items():
-            make_case(name, spec, tmp)
+        if "--exact-only" not in sys.argv:
+            for name, spec in CASES.items():
+                make_case(name, spec, tmp)
+        make_exact(tmp)
 
 
 if __name__ == "__main__":

@@ -114,6 +114,60 @@ def test_mst_sweep_bitexact(name):
         assert np.float64(r["weight"]).view(np.uint64) == g[f"mst{m}_weight"].view(np.uint64)[0]
 
 
+EXACT = golden("exact_mst")
+EXACT_RUNS = sorted({(k.split("_exact")[0], int(k.split("_exact")[1].split("_")[0]))
+                     for k in EXACT})
+
+
+@pytest.mark.parametrize("name,mpts", EXACT_RUNS)
+def test_exact_mst_bitexact(name, mpts):
+    """CoreSG::exact_mst (coresg.cpp:419-430): Prim's edges in acceptance order,
+    labels, component count and weight, as the reference wrote them."""
+    _, sg = built(name)
+    r = sg.exact_mst(mpts)
+    p = f"{name}_exact{mpts}_"
+    assert np.array_equal(r["u"], EXACT[p + "mst_u"])
+    assert np.array_equal(r["v"], EXACT[p + "mst_v"])
+    assert np.array_equal(r["w"].view(np.uint32), EXACT[p + "mst_w"].view(np.uint32))
+    assert np.array_equal(r["labels"], EXACT[p + "component"])
+    assert r["ncomp"] == int(EXACT[p + "num_components"][0])
+    assert np.float64(r["weight"]).view(np.uint64) == EXACT[p + "weight"].view(np.uint64)[0]
+
+
+def test_exact_mst_theorem1_on_exact_knn():
+    """Theorem 1 (test_coresg.cpp:32-61): with exact k-NN lists and the exact
+    MST at k_max as the bridge set (MissingEdgeMode::ExactMst, coresg.cpp:157-163,
+    bridges stored with their base distance), the CORE-SG holds an MST of the
+    complete MRD graph for every mpts <= k_max, so the sorted weights of
+    exact_mst equal those of Kruskal over the reweighted CORE-SG. brute_d8's
+    lists come from brute force, so they are exact; Prim over every pair
+    shares no code with the edge list and Kruskal."""
+    g, sg = built("brute_d8")
+    ids, dist, _, count = sg.graph()
+    n, k = sg.n, sg.k
+    padded = oracle.pad(g["data"])
+    ku, kv, kw = oracle.undirected_edges(ids, dist, count, n, k)
+    b = sg.exact_mst(k)
+    _, bw = oracle.l2sqr_pairs(padded, np.stack([b["u"], b["v"]], 1).astype(np.uint32))
+    u = np.concatenate([ku, b["u"]])
+    v = np.concatenate([kv, b["v"]])
+    w = np.concatenate([kw, bw])
+    cnt = count.astype(np.int64)
+    for m in (1, 2, 5, 12):
+        col = np.minimum(m, np.maximum(cnt, 1)) - 1
+        core = np.where(cnt > 0, dist.reshape(n, k)[np.arange(n), col], np.inf).astype(np.float32)
+        rw = np.maximum(w, np.maximum(core[u], core[v]))
+        _, _, mw, _, nc, _ = oracle.kruskal(n, u, v, rw)
+        e = sg.exact_mst(m)
+        assert e["ncomp"] == nc == 1
+        assert np.array_equal(np.sort(e["w"]).view(np.uint32), np.sort(mw).view(np.uint32))
+    # A planted mistake in the reweight (the second endpoint's core distance
+    # left out) breaks it.
+    rw = np.maximum(w, core[u])
+    _, _, mw, _, _, _ = oracle.kruskal(n, u, v, rw)
+    assert not np.array_equal(np.sort(e["w"]), np.sort(mw))
+
+
 @pytest.mark.parametrize("name", CASES)
 def test_mrd_block_and_prim_bitexact(name):
     g = golden(name)
