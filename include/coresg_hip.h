@@ -225,6 +225,15 @@ int coresg_hip_extract(coresg_hip_graph* g, const uint32_t* mpts, uint32_t count
                        coresg_hip_mst_info* info, uint32_t* out_u, uint32_t* out_v,
                        float* out_w, uint32_t* out_labels);
 
+/* CoreSG::exact_mst (coresg.cpp:419-430): Prim over the complete mutual-
+ * reachability graph of the resident points (mst_prim_virtual, mst.cpp:172-220,
+ * with CpuBackend::mrd_block rows, cpu_backend.cpp:61-90) at min(mpts, k_max),
+ * keeping Prim's tie rules. out_u/v/w receive up to n-1 edges in acceptance
+ * order (u < v), out_labels n dense labels by first appearance; any output may
+ * be NULL. info->rounds = Prim steps. */
+int coresg_hip_exact_mst(coresg_hip_graph* g, uint32_t mpts, coresg_hip_mst_info* info,
+                         uint32_t* out_u, uint32_t* out_v, float* out_w, uint32_t* out_labels);
+
 /* ---- 3. serving session -------------------------------------------------- */
 
 /* One device-resident CORE-SG behind a request queue served by a worker

@@ -119,6 +119,8 @@ SIGNATURES = [
     ("coresg_hip_edges", C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp]),
     ("coresg_hip_extract", C.c_int, [_vp, _vp, C.c_uint32, C.POINTER(MstInfo), _vp, _vp, _vp,
                                      _vp]),
+    ("coresg_hip_exact_mst", C.c_int, [_vp, C.c_uint32, C.POINTER(MstInfo), _vp, _vp, _vp,
+                                       _vp]),
     ("coresg_hip_sampled_recall", C.c_int, [_vp, C.c_size_t, C.c_uint64,
                                             C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("coresg_hip_graph_set_profiling", C.c_int, [_vp, C.c_int]),
@@ -355,6 +357,24 @@ class Graph:
                 r["labels"] = ol[i * self.n:(i + 1) * self.n]
             out.append(r)
         return out
+
+    def exact_mst(self, mpts: int, labels: bool = False) -> dict:
+        """CoreSG::exact_mst (coresg.cpp:419-430): Prim over the complete MRD
+        graph of the resident points; edges in acceptance order."""
+        info = MstInfo()
+        n1 = max(self.n - 1, 1)
+        ou = np.empty(n1, np.uint32)
+        ov = np.empty(n1, np.uint32)
+        ow = np.empty(n1, np.float32)
+        ol = np.empty(max(self.n, 1), np.uint32) if labels else None
+        check(lib().coresg_hip_exact_mst(self.h, mpts, C.byref(info), ptr(ou), ptr(ov), ptr(ow),
+                                         ptr(ol)))
+        t = info.num_edges
+        r = dict(mpts=info.mpts, num_edges=t, num_components=info.num_components,
+                 rounds=info.rounds, weight=info.weight, u=ou[:t], v=ov[:t], w=ow[:t])
+        if labels:
+            r["labels"] = ol[:self.n]
+        return r
 
     def sampled_recall(self, queries: int, seed: int):
         r = C.c_double()

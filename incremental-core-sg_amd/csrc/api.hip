@@ -326,6 +326,14 @@ int coresg_hip_extract(coresg_hip_graph* h, const uint32_t* mpts, uint32_t count
     });
 }
 
+int coresg_hip_exact_mst(coresg_hip_graph* h, uint32_t mpts, coresg_hip_mst_info* info,
+                         uint32_t* out_u, uint32_t* out_v, float* out_w, uint32_t* out_labels) {
+    return guarded([&] {
+        bind(h->g);
+        exact_mst(h->g, mpts, info, out_u, out_v, out_w, out_labels);
+    });
+}
+
 int coresg_hip_graph_set_profiling(coresg_hip_graph* h, int on) {
     return guarded([&] {
         h->g.resolve_timers();

@@ -1,0 +1,292 @@
+// The exact MST of the complete mutual-reachability graph over the resident
+// points: CoreSG::exact_mst (coresg.cpp:419-430), i.e. mst_prim_virtual
+// (mst.cpp:172-220) whose row of vertex i is CpuBackend::mrd_block's row
+// (cpu_backend.cpp:61-90): max(d(i, j), max(core[i], core[j])) with the exact
+// FP32 distance.
+//
+// Prim is sequential in its picks, and its tie rules fix which tree comes out
+// when weights tie (MRD weights tie often: every edge inside a dense region
+// takes the larger core distance). They are kept exactly:
+//   * best[j] improves only on a strictly smaller weight, so from[j] is the
+//     earliest tree vertex reaching the minimum;
+//   * the pick is the smallest j among the not-yet-tree vertices of minimum
+//     best[j], i.e. the minimum of (best bits << 32 | j) over them (weights
+//     are non-negative, so their bit patterns order like the floats).
+//
+// One cooperative launch runs every step. Block b owns vertices
+// [b*chunk, (b+1)*chunk): their best/from/in-tree state is touched by no other
+// block. A step is: the new tree vertex's row is read into LDS, each owned
+// vertex is relaxed against it, the block's minimum key goes to a candidate
+// slot, one grid barrier, and then every block reduces the same candidate
+// array to the same pick. Candidate arrays alternate by step parity, so the
+// next step's writes never meet a block still reading this step's. The owner
+// of the pick records the step's edge. The barrier counter only grows
+// (target = step * blocks), and every wait is bounded: a wait that exceeds
+// the bound sets an error flag and ends its block, after which every other
+// block ends at its next wait, so the launch always drains.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/coresg_hip.h"
+#include "common.hpp"
+#include "graph.hpp"
+
+namespace coresg_hip {
+namespace {
+
+constexpr unsigned kThreads = 256;
+constexpr std::uint32_t kMaxStride = 1024;  // row of the current vertex, kept in LDS
+constexpr std::uint32_t kSpinLimit = 1u << 22;
+constexpr std::uint32_t kErrBarrier = 1u;
+
+struct Prim {
+    const float* pts;
+    const float* core;
+    float* best;
+    std::uint32_t* from;
+    std::uint8_t* in_tree;
+    std::uint64_t* cand;  // 2 * blocks
+    std::uint32_t* out_a;  // per step: smaller endpoint, kNone when the step adds no edge
+    std::uint32_t* out_b;
+    float* out_w;
+    std::uint32_t* bar;
+    std::uint32_t* err;
+    std::uint32_t n, stride, chunk;
+};
+
+// core_dist (knn_graph.hpp:157-163) of every vertex for min(mpts, k)
+// (coresg.cpp:68-70): +inf for an empty list, else the entry at position
+// min(mpts, count) - 1, mpts 0 reading the first.
+__global__ void __launch_bounds__(kThreads)
+    exact_core_kernel(const float* __restrict__ dist, const std::uint32_t* __restrict__ count,
+                      std::uint32_t n, std::uint32_t k, std::uint32_t mpts,
+                      float* __restrict__ core) {
+    for (std::uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += gridDim.x * blockDim.x) {
+        const std::uint32_t c = count[v];
+        std::uint32_t pos = mpts == 0 ? 0 : mpts - 1;
+        if (pos >= c) pos = c - 1;
+        core[v] = c == 0 ? INFINITY : dist[std::size_t(v) * k + pos];
+    }
+}
+
+// Bounded grid barrier; returns false (for every thread of the block) when
+// the wait gave up.
+__device__ __forceinline__ bool grid_wait(const Prim& p, std::uint32_t target, bool* s_ok) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(p.bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        bool ok = true;
+        std::uint32_t it = 0;
+        while (__hip_atomic_load(p.bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++it > kSpinLimit) {
+                atomicOr(p.err, kErrBarrier);
+                ok = false;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        *s_ok = ok;
+    }
+    __syncthreads();
+    return *s_ok;
+}
+
+__global__ void __launch_bounds__(kThreads) exact_prim_kernel(Prim p) {
+    __shared__ __align__(16) float crow[kMaxStride];
+    __shared__ std::uint64_t wmin[kThreads / kWave];
+    __shared__ bool s_ok;
+    const std::uint32_t tid = threadIdx.x, G = gridDim.x;
+    const std::uint32_t j0 = blockIdx.x * p.chunk;
+    const std::uint32_t j1 = j0 + p.chunk < p.n ? j0 + p.chunk : p.n;
+    for (std::uint32_t j = j0 + tid; j < j1; j += kThreads) {
+        p.best[j] = j == 0 ? 0.0f : INFINITY;
+        p.from[j] = kNone;
+        p.in_tree[j] = j == 0;
+    }
+    std::uint32_t current = 0;
+    for (std::uint32_t step = 1; step < p.n; ++step) {
+        const float* row = p.pts + std::size_t(current) * p.stride;
+        for (std::uint32_t t = tid; t < p.stride; t += kThreads) crow[t] = row[t];
+        const float cc = p.core[current];
+        __syncthreads();
+        std::uint64_t key_min = kNone64;
+        for (std::uint32_t j = j0 + tid; j < j1; j += kThreads) {
+            if (p.in_tree[j]) continue;
+            const float d = sqrt_exact(l2sqr_exact(crow, p.pts + std::size_t(j) * p.stride, p.stride));
+            const float w = fmax_ref(d, fmax_ref(cc, p.core[j]));
+            float b = p.best[j];
+            if (w < b) {
+                b = w;
+                p.best[j] = w;
+                p.from[j] = current;
+            }
+            const std::uint64_t key = (std::uint64_t(fbits(b)) << 32) | j;
+            key_min = key < key_min ? key : key_min;
+        }
+        key_min = wave_min_u64(key_min);
+        if ((tid & (kWave - 1)) == 0) wmin[tid / kWave] = key_min;
+        __syncthreads();
+        std::uint64_t* cand = p.cand + std::size_t(step & 1u) * G;
+        if (tid == 0) {
+            std::uint64_t m = wmin[0];
+#pragma unroll
+            for (unsigned w = 1; w < kThreads / kWave; ++w) m = wmin[w] < m ? wmin[w] : m;
+            cand[blockIdx.x] = m;
+        }
+        if (!grid_wait(p, step * G, &s_ok)) return;
+        std::uint64_t k = kNone64;
+        for (std::uint32_t i = tid; i < G; i += kThreads) {
+            const std::uint64_t c =
+                __hip_atomic_load(&cand[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            k = c < k ? c : k;
+        }
+        k = wave_min_u64(k);
+        __syncthreads();  // every wave is done with wmin from the block reduction
+        if ((tid & (kWave - 1)) == 0) wmin[tid / kWave] = k;
+        __syncthreads();
+        k = wmin[0];
+#pragma unroll
+        for (unsigned w = 1; w < kThreads / kWave; ++w) k = wmin[w] < k ? wmin[w] : k;
+        if (k == kNone64) break;  // no vertex left outside the tree
+        const std::uint32_t pick = static_cast<std::uint32_t>(k);
+        if (tid == 0 && pick >= j0 && pick < j1) {
+            const float pw = __uint_as_float(static_cast<std::uint32_t>(k >> 32));
+            const std::uint32_t f = p.from[pick];
+            p.in_tree[pick] = 1;
+            if (f != kNone && pw < INFINITY) {
+                p.out_a[step - 1] = f < pick ? f : pick;
+                p.out_b[step - 1] = f < pick ? pick : f;
+                p.out_w[step - 1] = pw;
+            } else {
+                p.out_a[step - 1] = kNone;
+            }
+        }
+        current = pick;
+        // The next step's first __syncthreads (after the row load) orders the
+        // owner's in_tree write before any of its threads reads it.
+    }
+}
+
+std::uint32_t find_root(std::vector<std::uint32_t>& parent, std::uint32_t x) {
+    while (parent[x] != x) {
+        parent[x] = parent[parent[x]];
+        x = parent[x];
+    }
+    return x;
+}
+
+}  // namespace
+
+void exact_mst(Graph& g, std::uint32_t mpts, void* info_out, std::uint32_t* out_u,
+               std::uint32_t* out_v, float* out_w, std::uint32_t* out_labels) {
+    auto* info = static_cast<coresg_hip_mst_info*>(info_out);
+    if (!g.points_set) throw std::runtime_error("exact_mst: points not set");
+    if (!g.knn_set) throw std::runtime_error("exact_mst: k-NN lists not set");
+    if (g.stride > kMaxStride)
+        throw std::runtime_error("exact_mst: row stride above " + std::to_string(kMaxStride));
+    const std::uint32_t n = g.n;
+    const std::uint32_t m = mpts < g.k ? mpts : g.k;
+    std::vector<std::uint32_t> hu, hv;
+    std::vector<float> hw;
+    if (n >= 2) {
+        int cus = 0, coop = 0, per_cu = 0;
+        CSG_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g.device));
+        CSG_CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, g.device));
+        if (!coop) throw std::runtime_error("exact_mst: device lacks cooperative launch");
+        CSG_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(exact_prim_kernel), kThreads, 0));
+        if (per_cu < 1 || cus < 1) throw std::runtime_error("exact_mst: kernel does not fit a CU");
+        const std::uint32_t want = (n + kThreads - 1) / kThreads;
+        const std::uint32_t blocks = want < std::uint32_t(cus) ? want : std::uint32_t(cus);
+        const std::uint32_t chunk = (n + blocks - 1) / blocks;
+        if (std::uint64_t(n) * blocks >= (1ull << 32))
+            throw std::runtime_error("exact_mst: n too large for the step counter");
+
+        DBuf<float> core, best, ow;
+        DBuf<std::uint32_t> from, oa, ob, ctl;
+        DBuf<std::uint8_t> in_tree;
+        DBuf<std::uint64_t> cand;
+        core.reserve(n);
+        best.reserve(n);
+        from.reserve(n);
+        in_tree.reserve(n);
+        oa.reserve(n - 1);
+        ob.reserve(n - 1);
+        ow.reserve(n - 1);
+        cand.reserve(2 * std::size_t(blocks));
+        ctl.reserve(2);
+        CSG_CHECK(hipMemsetAsync(ctl.p, 0, 2 * sizeof(std::uint32_t), g.stream));
+        CSG_CHECK(hipMemsetAsync(oa.p, 0xFF, 4 * std::size_t(n - 1), g.stream));
+        exact_core_kernel<<<blocks_for(n, kThreads), kThreads, 0, g.stream>>>(
+            g.dist.p, g.count.p, n, g.k, m, core.p);
+        CSG_LAUNCH_CHECK();
+        Prim p{g.pts.p, core.p, best.p, from.p, in_tree.p, cand.p, oa.p, ob.p, ow.p,
+               ctl.p, ctl.p + 1, n, g.stride, chunk};
+        void* args[] = {&p};
+        {
+            ScopedKernelTimer t(g, "exact_prim");
+            CSG_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(exact_prim_kernel),
+                                                 dim3(blocks), dim3(kThreads), args, 0,
+                                                 g.stream));
+        }
+        std::uint32_t err = 0;
+        std::vector<std::uint32_t> a(n - 1), b(n - 1);
+        std::vector<float> w(n - 1);
+        CSG_CHECK(hipMemcpyAsync(&err, ctl.p + 1, 4, hipMemcpyDeviceToHost, g.stream));
+        CSG_CHECK(hipMemcpyAsync(a.data(), oa.p, 4 * (n - 1), hipMemcpyDeviceToHost, g.stream));
+        CSG_CHECK(hipMemcpyAsync(b.data(), ob.p, 4 * (n - 1), hipMemcpyDeviceToHost, g.stream));
+        CSG_CHECK(hipMemcpyAsync(w.data(), ow.p, 4 * (n - 1), hipMemcpyDeviceToHost, g.stream));
+        CSG_CHECK(hipStreamSynchronize(g.stream));
+        if (err) throw std::runtime_error("exact_mst: grid barrier wait gave up");
+        for (std::uint32_t s = 0; s + 1 < n; ++s) {
+            if (a[s] == kNone) continue;
+            hu.push_back(a[s]);
+            hv.push_back(b[s]);
+            hw.push_back(w[s]);
+        }
+    }
+    // Labels by first appearance over a union-find of the accepted edges and
+    // the weight as the double sum in acceptance order (mst.cpp:43-59;
+    // types.hpp:58-62).
+    std::vector<std::uint32_t> parent(n);
+    for (std::uint32_t i = 0; i < n; ++i) parent[i] = i;
+    std::uint32_t comps = n;
+    double weight = 0.0;
+    for (std::size_t e = 0; e < hu.size(); ++e) {
+        const std::uint32_t ra = find_root(parent, hu[e]), rb = find_root(parent, hv[e]);
+        if (ra != rb) {
+            parent[ra < rb ? rb : ra] = ra < rb ? ra : rb;
+            --comps;
+        }
+        weight += double(hw[e]);
+    }
+    if (out_labels) {
+        std::vector<std::uint32_t> remap(n, kNone);
+        std::uint32_t next = 0;
+        for (std::uint32_t i = 0; i < n; ++i) {
+            const std::uint32_t r = find_root(parent, i);
+            if (remap[r] == kNone) remap[r] = next++;
+            out_labels[i] = remap[r];
+        }
+    }
+    if (out_u) std::memcpy(out_u, hu.data(), hu.size() * 4);
+    if (out_v) std::memcpy(out_v, hv.data(), hv.size() * 4);
+    if (out_w) std::memcpy(out_w, hw.data(), hw.size() * 4);
+    if (info) {
+        info->mpts = mpts;
+        info->num_edges = static_cast<std::uint32_t>(hu.size());
+        info->num_components = comps;
+        info->rounds = n >= 2 ? n - 1 : 0;
+        info->weight = weight;
+    }
+}
+
+}  // namespace coresg_hip

@@ -250,6 +250,10 @@ void reweighted_edges(Graph& g, std::uint32_t mpts, std::uint32_t* u, std::uint3
                       float* w);
 // Per-instance MST summaries of a batch of mpts values (info[i * count + q]).
 void extract_instances(Graph& g, const std::uint32_t* mpts, std::uint32_t count, void* info);
+// CoreSG::exact_mst over the resident points (exact.hip); info is a
+// coresg_hip_mst_info.
+void exact_mst(Graph& g, std::uint32_t mpts, void* info, std::uint32_t* out_u,
+               std::uint32_t* out_v, float* out_w, std::uint32_t* out_labels);
 // Undirected k-NN edge count of every instance.
 std::vector<std::uint64_t> knn_edges_per_instance(Graph& g);
 

@@ -422,8 +422,6 @@ class CoreSG:
         nc = C.c_uint32()
         wt = C.c_double()
         t = lib().orc_sg_exact_mst(self.h, mpts, u, v, w, lab, C.byref(nc), C.byref(wt))
-        if t == C.c_size_t(-1).value:
-            raise ValueError("mpts exceeds k_max")
         return dict(u=u[:t], v=v[:t], w=w[:t], labels=lab[:self.n], ncomp=nc.value,
                     weight=wt.value)
 

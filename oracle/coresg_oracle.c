@@ -1215,18 +1215,16 @@ size_t orc_sg_mst(const orc_sg* sg, uint32_t mpts, uint32_t* out_u, uint32_t* ou
     return t;
 }
 
-/* src/coresg.cpp:419-430 (CoreSG::exact_mst): core distances for mpts
- * (coresg.cpp:68-70), then mst_prim_virtual (mst.cpp:172-220) whose row i is
+/* src/coresg.cpp:419-430 (CoreSG::exact_mst): core distances for
+ * min(mpts, k_max) (coresg.cpp:68-70; mpts 0 reads column 0, knn_graph.hpp:157-163), then mst_prim_virtual (mst.cpp:172-220) whose row i is
  * CpuBackend::mrd_block's row (cpu_backend.cpp:61-90), i.e. orc_mrd_block.
  * Edges in Prim acceptance order, canonicalised u<v; labels and component
  * count from a union-find over them (mst.cpp:211-212, :43-46); weight = the
- * double sum in acceptance order (types.hpp:58-62). Returns the edge count,
- * or (size_t)-1 for mpts above k_max. */
+ * double sum in acceptance order (types.hpp:58-62). Returns the edge count. */
 size_t orc_sg_exact_mst(const orc_sg* sg, uint32_t mpts, uint32_t* out_u, uint32_t* out_v,
                         float* out_w, uint32_t* out_labels, uint32_t* out_ncomp,
                         double* out_weight) {
-    if (mpts == 0) mpts = 1;
-    if (mpts > sg->k) return (size_t)-1;
+    if (mpts > sg->k) mpts = sg->k;
     const uint32_t n = sg->n;
     float* core = (float*)malloc((n ? n : 1) * 4);
     orc_core_distances(sg->dist, sg->count, n, sg->k, mpts, core);

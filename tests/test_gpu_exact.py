@@ -1,0 +1,112 @@
+"""CoreSG::exact_mst on the GPU (Prim over the complete mutual-reachability
+graph, coresg.cpp:419-430): bit-exact against the reference's own Prim output
+(tests/golden/exact_mst.npz), against the oracle on a fresh graph, and by
+Theorem 1 (test_coresg.cpp:32-61) at a size the oracle is not run at."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import golden, load_pkg_module
+
+chip = load_pkg_module("coresg_hip")
+coresg_io = load_pkg_module("coresg_io")
+datagen = load_pkg_module("datagen")
+pytestmark = pytest.mark.gpu
+
+EXACT = golden("exact_mst")
+RUNS = sorted({(k.split("_exact")[0], int(k.split("_exact")[1].split("_")[0])) for k in EXACT})
+
+
+def graph_from_file(f):
+    g = chip.Graph(f.n, f.dim, f.k_max)
+    g.set_points(f.points)
+    g.set_knn(f.ids, f.dist, f.flags, f.count)
+    u, v, w = f.bridge_uvw()
+    g.set_bridges(u, v, w, f.repair_count)
+    return g
+
+
+def assert_same(r, ref):
+    assert r["num_edges"] == ref["u"].size
+    assert np.array_equal(r["u"], ref["u"]) and np.array_equal(r["v"], ref["v"])
+    assert np.array_equal(r["w"].view(np.uint32), ref["w"].view(np.uint32))
+    assert np.array_equal(r["labels"], ref["labels"])
+    assert r["num_components"] == ref["ncomp"]
+    assert np.float64(r["weight"]).view(np.uint64) == np.float64(ref["weight"]).view(np.uint64)
+
+
+@pytest.mark.parametrize("name,mpts", RUNS)
+def test_exact_mst_matches_reference(name, mpts):
+    print(f"[gpu] exact_mst vs reference: {name} mpts={mpts}", flush=True)
+    f = coresg_io.loads(golden(name)["coresg_bytes"].tobytes())
+    g = graph_from_file(f)
+    r = g.exact_mst(mpts, labels=True)
+    p = f"{name}_exact{mpts}_"
+    assert_same(r, dict(u=EXACT[p + "mst_u"], v=EXACT[p + "mst_v"], w=EXACT[p + "mst_w"],
+                        labels=EXACT[p + "component"],
+                        ncomp=int(EXACT[p + "num_components"][0]),
+                        weight=EXACT[p + "weight"][0]))
+    assert r["rounds"] == f.n - 1
+    g.close()
+
+
+def test_exact_mst_matches_oracle_and_clamps_mpts(tmp_path):
+    """A fresh device build of 3000 points (12 blocks of 250 vertices): device
+    Prim == oracle Prim on the same lists, edge for edge; mpts 0
+    reads the first column like mpts 1, and mpts above k_max is clamped
+    (coresg.cpp:68-70); a second call on a smaller graph gives its own answer."""
+    x = datagen.gaussian_mixture(3000, 16, 10, 0.3, 2.0, 0.05, 5)
+    print("[gpu] exact_mst vs oracle: n=3000 d=16 k=16", flush=True)
+    g = chip.Graph(x.shape[0], x.shape[1], 16)
+    g.set_points(x)
+    g.build(chip.build_params())
+    ids, dist, flags, count = g.get_knn()
+    u, v, w, rc = g.get_bridges()
+    fobj = coresg_io.CoreSGFile(16, x.shape[1], 16, 0, 0, oracle.pad(x), ids, dist, flags, count,
+                                coresg_io.bridges_from_uvw(u, v, w), rc)
+    path = os.path.join(str(tmp_path), "g.coresg")
+    coresg_io.save(path, fobj)
+    sg = oracle.CoreSG.load(path)
+    for m in (1, 7, 16):
+        assert_same(g.exact_mst(m, labels=True), sg.exact_mst(m))
+    a = g.exact_mst(0)
+    b = g.exact_mst(1)
+    assert np.array_equal(a["w"].view(np.uint32), b["w"].view(np.uint32))
+    c = g.exact_mst(40)
+    d = g.exact_mst(16)
+    assert np.array_equal(c["u"], d["u"]) and np.array_equal(c["w"], d["w"])
+    g.close()
+    small = chip.Graph(700, 16, 16)
+    small.set_points(x[:700])
+    small.build(chip.build_params(brute_force_limit=1000))
+    r = small.exact_mst(4)
+    assert r["num_edges"] == 699 and r["num_components"] == 1
+    small.close()
+
+
+def test_exact_mst_theorem1_at_20k():
+    """Theorem 1: exact k-NN lists (brute force) plus the exact MST at k_max as
+    the bridge set, stored with base distances (coresg.cpp:157-163), make the
+    CORE-SG MST of every mpts <= k_max weigh what the complete-graph MST
+    weighs, weight for weight: the sorted weights of the device sweep (edge
+    list + Boruvka) equal those of the device Prim, two unrelated code paths."""
+    x = datagen.gaussian_mixture(20000, 16, 10, 0.3, 2.0, 0.05, 0)
+    print("[gpu] exact_mst theorem 1: n=20000 d=16 k=16", flush=True)
+    g = chip.Graph(x.shape[0], x.shape[1], 16)
+    g.set_points(x)
+    g.build_knn(chip.build_params(brute_force_limit=20000))
+    b = g.exact_mst(16)
+    assert b["num_components"] == 1
+    _, bw = oracle.l2sqr_pairs(oracle.pad(x), np.stack([b["u"], b["v"]], 1).astype(np.uint32))
+    g.set_bridges(b["u"], b["v"], bw, 0)
+    mp = [1, 2, 4, 8, 16]
+    res = g.extract(mp, edges=True)
+    for m, r in zip(mp, res):
+        e = g.exact_mst(m)
+        assert r["num_edges"] == e["num_edges"] == 19999
+        assert np.array_equal(np.sort(r["w"]).view(np.uint32), np.sort(e["w"]).view(np.uint32))
+    g.close()
