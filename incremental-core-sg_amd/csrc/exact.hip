@@ -175,6 +175,161 @@ __global__ void __launch_bounds__(kThreads) exact_prim_kernel(Prim p) {
     }
 }
 
+// The same Prim with each block's vertices resident: their rows in LDS, one
+// column per vertex (consecutive lanes read consecutive words), their best /
+// from / in-tree state in registers (vertex j0 + tid + r * kThreads in slot
+// r). A step's only global traffic is the new tree vertex's row, one 64-bit
+// candidate word per block and the polls of the other blocks' words.
+//
+// Candidate word: tag (step & 3) in bits 62-63, then the key best bits << 31 |
+// j (weights are non-negative, so their bits fit in 31; j < 2^31). A block
+// publishes its word with one relaxed device-scope store into the buffer of
+// the step's parity and polls the G words of that buffer until every tag is
+// the step's: the words are the barrier and the data at once, with no
+// counter and no release/acquire cache maintenance (the points are read-only
+// and everything else a step reads is in LDS or registers). A block can run
+// at most one step ahead of the slowest -- to finish step s+1 it needs every
+// block's word of step s+1 -- so the other buffer still holds step s-1's tags
+// while step s is read, and two tag bits tell them apart. Zero-filled buffers
+// hold tag 0, which no step polls for in a zero-filled buffer (step 4 polls
+// buffer 0 only after step 2 wrote tag 2 there).
+constexpr std::uint64_t kPayload = (1ull << 62) - 1;
+
+template <std::uint32_t R>
+__global__ void __launch_bounds__(kThreads) exact_prim_lds_kernel(Prim p) {
+    extern __shared__ __align__(16) float lds[];
+    constexpr std::uint32_t ld = R * kThreads;  // columns: the block's vertices
+    float* crow = lds + std::size_t(ld) * p.stride;
+    __shared__ std::uint64_t wmin[kThreads / kWave];
+    const std::uint32_t tid = threadIdx.x, G = gridDim.x;
+    const std::uint32_t j0 = blockIdx.x * p.chunk;
+    const std::uint32_t j1 = j0 + p.chunk < p.n ? j0 + p.chunk : p.n;
+    float best[R], core[R];
+    std::uint32_t from[R];
+    bool out[R];  // outside the tree and a real vertex
+#pragma unroll
+    for (std::uint32_t r = 0; r < R; ++r) {
+        const std::uint32_t j = j0 + tid + r * kThreads;
+        const bool real = j < j1;
+        best[r] = INFINITY;
+        from[r] = kNone;
+        out[r] = real && j != 0;
+        core[r] = real ? p.core[j] : 0.0f;
+        for (std::uint32_t i = 0; i < p.stride; ++i)
+            lds[std::size_t(i) * ld + tid + r * kThreads] =
+                real ? p.pts[std::size_t(j) * p.stride + i] : 0.0f;
+    }
+    std::uint32_t current = 0;
+    for (std::uint32_t step = 1; step < p.n; ++step) {
+        const float* row = p.pts + std::size_t(current) * p.stride;
+        for (std::uint32_t t = tid; t < p.stride; t += kThreads) crow[t] = row[t];
+        const float cc = p.core[current];
+        __syncthreads();
+        std::uint64_t key_min = kPayload;
+#pragma unroll
+        for (std::uint32_t r = 0; r < R; ++r) {
+            if (!out[r]) continue;
+            const float* col = lds + tid + r * kThreads;
+            float acc0[8], acc1[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                acc0[q] = 0.0f;
+                acc1[q] = 0.0f;
+            }
+            std::uint32_t i = 0;
+            for (; i + 16 <= p.stride; i += 16) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const float d0 = crow[i + q] - col[std::size_t(i + q) * ld];
+                    const float d1 = crow[i + 8 + q] - col[std::size_t(i + 8 + q) * ld];
+                    acc0[q] = __builtin_fmaf(d0, d0, acc0[q]);
+                    acc1[q] = __builtin_fmaf(d1, d1, acc1[q]);
+                }
+            }
+            if (i + 8 <= p.stride) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const float d0 = crow[i + q] - col[std::size_t(i + q) * ld];
+                    acc0[q] = __builtin_fmaf(d0, d0, acc0[q]);
+                }
+            }
+            float v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = acc0[q] + acc1[q];
+            const float l0 = v[0] + v[4], l1 = v[1] + v[5], l2 = v[2] + v[6], l3 = v[3] + v[7];
+            const float d = sqrt_exact((l0 + l1) + (l2 + l3));
+            const float w = fmax_ref(d, fmax_ref(cc, core[r]));
+            if (w < best[r]) {
+                best[r] = w;
+                from[r] = current;
+            }
+            const std::uint64_t key =
+                (std::uint64_t(fbits(best[r])) << 31) | (j0 + tid + r * kThreads);
+            key_min = key < key_min ? key : key_min;
+        }
+        key_min = wave_min_u64(key_min);
+        if ((tid & (kWave - 1)) == 0) wmin[tid / kWave] = key_min;
+        __syncthreads();
+        std::uint64_t* cand = p.cand + std::size_t(step & 1u) * G;
+        const std::uint64_t tag = std::uint64_t(step & 3u) << 62;
+        if (tid == 0) {
+            std::uint64_t m = wmin[0];
+#pragma unroll
+            for (unsigned q = 1; q < kThreads / kWave; ++q) m = wmin[q] < m ? wmin[q] : m;
+            __hip_atomic_store(&cand[blockIdx.x], tag | m, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // Poll every block's word of this step (G <= kThreads: one per thread).
+        std::uint64_t seen = kPayload;
+        std::uint32_t it = 0;
+        bool ready = tid >= G;
+        for (;;) {
+            if (!ready) {
+                const std::uint64_t c =
+                    __hip_atomic_load(&cand[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((c & ~kPayload) == tag) {
+                    ready = true;
+                    seen = c & kPayload;
+                }
+            }
+            if (__syncthreads_and(ready)) break;
+            if (++it > kSpinLimit) {
+                if (tid == 0) atomicOr(p.err, kErrBarrier);
+                return;  // block-uniform: every thread saw the same count
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        seen = wave_min_u64(seen);
+        if ((tid & (kWave - 1)) == 0) wmin[tid / kWave] = seen;
+        __syncthreads();
+        std::uint64_t k = wmin[0];
+#pragma unroll
+        for (unsigned q = 1; q < kThreads / kWave; ++q) k = wmin[q] < k ? wmin[q] : k;
+        if (k == kPayload) break;  // no vertex left outside the tree
+        const std::uint32_t pick = static_cast<std::uint32_t>(k & 0x7FFFFFFFu);
+        if (pick >= j0 && pick < j1 && (pick - j0) % kThreads == tid) {
+            const std::uint32_t r = (pick - j0) / kThreads;
+            const float pw = __uint_as_float(static_cast<std::uint32_t>(k >> 31));
+#pragma unroll
+            for (std::uint32_t q = 0; q < R; ++q) {
+                if (q != r) continue;
+                out[q] = false;
+                const std::uint32_t f = from[q];
+                if (f != kNone && pw < INFINITY) {
+                    p.out_a[step - 1] = f < pick ? f : pick;
+                    p.out_b[step - 1] = f < pick ? pick : f;
+                    p.out_w[step - 1] = pw;
+                } else {
+                    p.out_a[step - 1] = kNone;
+                }
+            }
+        }
+        current = pick;
+    }
+}
+
+using PrimKernel = void (*)(Prim);
+
 std::uint32_t find_root(std::vector<std::uint32_t>& parent, std::uint32_t x) {
     while (parent[x] != x) {
         parent[x] = parent[parent[x]];
@@ -205,10 +360,38 @@ void exact_mst(Graph& g, std::uint32_t mpts, void* info_out, std::uint32_t* out_
             &per_cu, reinterpret_cast<const void*>(exact_prim_kernel), kThreads, 0));
         if (per_cu < 1 || cus < 1) throw std::runtime_error("exact_mst: kernel does not fit a CU");
         const std::uint32_t want = (n + kThreads - 1) / kThreads;
-        const std::uint32_t blocks = want < std::uint32_t(cus) ? want : std::uint32_t(cus);
+        // At most one block per CU (co-resident) and per thread of a block
+        // (the resident kernel polls one candidate word per thread).
+        const std::uint32_t cap = std::uint32_t(cus) < kThreads ? std::uint32_t(cus) : kThreads;
+        const std::uint32_t blocks = want < cap ? want : cap;
         const std::uint32_t chunk = (n + blocks - 1) / blocks;
         if (std::uint64_t(n) * blocks >= (1ull << 32))
             throw std::runtime_error("exact_mst: n too large for the step counter");
+        // Resident variant when the block's rows fit in LDS and the keys' index
+        // field: R vertices per thread.
+        PrimKernel kern = exact_prim_kernel;
+        std::size_t shmem = 0;
+        const std::uint32_t per = (chunk + kThreads - 1) / kThreads;
+        const std::uint32_t R = per <= 1 ? 1 : per <= 2 ? 2 : per <= 4 ? 4 : 0;
+        int lds_max = 0;
+        CSG_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                        g.device));
+        const std::size_t need = (std::size_t(R) * kThreads + 1) * g.stride * sizeof(float);
+        if (R != 0 && n < (1u << 31) && need + 4096 <= std::size_t(lds_max)) {
+            kern = R == 1 ? exact_prim_lds_kernel<1>
+                          : R == 2 ? exact_prim_lds_kernel<2> : exact_prim_lds_kernel<4>;
+            shmem = need;
+            CSG_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          static_cast<int>(shmem)));
+            int fits = 0;
+            CSG_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &fits, reinterpret_cast<const void*>(kern), kThreads, shmem));
+            if (fits < 1) {
+                kern = exact_prim_kernel;
+                shmem = 0;
+            }
+        }
 
         DBuf<float> core, best, ow;
         DBuf<std::uint32_t> from, oa, ob, ctl;
@@ -224,6 +407,8 @@ void exact_mst(Graph& g, std::uint32_t mpts, void* info_out, std::uint32_t* out_
         cand.reserve(2 * std::size_t(blocks));
         ctl.reserve(2);
         CSG_CHECK(hipMemsetAsync(ctl.p, 0, 2 * sizeof(std::uint32_t), g.stream));
+        CSG_CHECK(hipMemsetAsync(cand.p, 0, 2 * std::size_t(blocks) * sizeof(std::uint64_t),
+                                 g.stream));
         CSG_CHECK(hipMemsetAsync(oa.p, 0xFF, 4 * std::size_t(n - 1), g.stream));
         exact_core_kernel<<<blocks_for(n, kThreads), kThreads, 0, g.stream>>>(
             g.dist.p, g.count.p, n, g.k, m, core.p);
@@ -233,9 +418,9 @@ void exact_mst(Graph& g, std::uint32_t mpts, void* info_out, std::uint32_t* out_
         void* args[] = {&p};
         {
             ScopedKernelTimer t(g, "exact_prim");
-            CSG_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(exact_prim_kernel),
-                                                 dim3(blocks), dim3(kThreads), args, 0,
-                                                 g.stream));
+            CSG_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern),
+                                                 dim3(blocks), dim3(kThreads), args,
+                                                 static_cast<unsigned>(shmem), g.stream));
         }
         std::uint32_t err = 0;
         std::vector<std::uint32_t> a(n - 1), b(n - 1);

@@ -110,3 +110,23 @@ def test_exact_mst_theorem1_at_20k():
         assert r["num_edges"] == e["num_edges"] == 19999
         assert np.array_equal(np.sort(r["w"]).view(np.uint32), np.sort(e["w"]).view(np.uint32))
     g.close()
+
+
+def test_exact_mst_wide_rows_generic_kernel(tmp_path):
+    """Rows of 512 floats do not fit the resident kernel's LDS, so the generic
+    kernel (state and rows in global memory, counter barrier) runs; lists from
+    the oracle's brute force, checked against the oracle's Prim edge for edge."""
+    x = datagen.gaussian_mixture(600, 512, 5, 0.3, 2.0, 0.05, 7)
+    print("[gpu] exact_mst generic kernel: n=600 d=512 k=8", flush=True)
+    ids, dist, flags, count = oracle.brute_force(oracle.pad(x), 8)
+    g = chip.Graph(600, 512, 8)
+    g.set_points(x)
+    g.set_knn(ids, dist, flags, count)
+    fobj = coresg_io.CoreSGFile(8, 512, 512, 0, 0, oracle.pad(x), ids, dist, flags, count,
+                                np.zeros((0, 3), np.uint32), 0)
+    path = os.path.join(str(tmp_path), "w.coresg")
+    coresg_io.save(path, fobj)
+    sg = oracle.CoreSG.load(path)
+    for m in (1, 8):
+        assert_same(g.exact_mst(m, labels=True), sg.exact_mst(m))
+    g.close()
