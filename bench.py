@@ -178,6 +178,65 @@ def serving_run(chip, datagen, k, seed_n, batches, insert_rate, extract_rate, bu
         hbm_budget_bytes=budget, graph_device_bytes=dev_bytes)
 
 
+def exact_mst_run(chip, datagen, coresg_io, cpu_n=5000):
+    """CoreSG::exact_mst (Prim over the complete MRD graph, one cooperative
+    launch) at mpts = k_max on cfg1's shape (20k x 16) and on the first 100k
+    points of the headline generator (d=24), plus a bounded CPU comparison: the
+    oracle's Prim (single thread) and the device on the first `cpu_n` points."""
+    out = {}
+    cases = (("mixture-20k-d16", datagen.gaussian_mixture(20_000, 16, 10, 0.3, 2.0, 0.05, 0), 16),
+             ("mixture-100k-d24", datagen.gaussian_mixture(100_000, 24, 32, 0.2, 3.0, 0.02, 1), 32))
+    for name, x, k in cases:
+        g = chip.Graph(x.shape[0], x.shape[1], k)
+        g.set_points(x)
+        g.build(chip.build_params())
+        g.exact_mst(k)
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            r = g.exact_mst(k)
+            ts.append(time.perf_counter() - t0)
+        ms = 1e3 * min(ts)
+        out[name] = dict(n=int(x.shape[0]), d=int(x.shape[1]), mpts=k, ms=ms,
+                         us_per_step=1e3 * ms / (x.shape[0] - 1), num_edges=r["num_edges"],
+                         weight=r["weight"])
+        g.close()
+        print(f"[bench] exact_mst {name}: {ms:.1f} ms", file=sys.stderr, flush=True)
+    # Bounded CPU comparison on the first cpu_n points of the 100k case.
+    x = cases[1][1][:cpu_n]
+    k = cases[1][2]
+    g = chip.Graph(x.shape[0], x.shape[1], k)
+    g.set_points(x)
+    g.build(chip.build_params())
+    g.exact_mst(k)
+    t0 = time.perf_counter()
+    g.exact_mst(k)
+    dev_ms = 1e3 * (time.perf_counter() - t0)
+    ids, dist, flags, count = g.get_knn()
+    u, v, w, rc = g.get_bridges()
+    g.close()
+    cpu_ms = None
+    try:
+        sys.path.insert(0, REPO)
+        import oracle
+        with tempfile.TemporaryDirectory() as tmp:
+            path = os.path.join(tmp, "s.coresg")
+            coresg_io.save(path, coresg_io.CoreSGFile(
+                k, x.shape[1], oracle.stride_of(x.shape[1]), 0, 0, oracle.pad(x), ids, dist,
+                flags, count, coresg_io.bridges_from_uvw(u, v, w), rc))
+            sg = oracle.CoreSG.load(path)
+            t0 = time.perf_counter()
+            sg.exact_mst(k)
+            cpu_ms = 1e3 * (time.perf_counter() - t0)
+    except Exception as e:  # the CPU comparison is optional
+        print(f"[bench] exact_mst CPU comparison skipped: {e}", file=sys.stderr, flush=True)
+    out["cpu_baseline"] = dict(kind="port", cores=1, n=cpu_n, d=int(x.shape[1]), cpu_ms=cpu_ms,
+                               device_ms=dev_ms,
+                               sample=f"first {cpu_n} points of mixture-100k-d24, mpts={k}; "
+                                      "oracle Prim (mst.cpp:172-220 restated in C), one thread")
+    return out
+
+
 def mixed_precision_run(chip, datagen, n=2_000_000, d=128, k=32, mpts=(4, 16, 32)):
     """The mixed-precision distance path: a 64-component anisotropic mixture
     (eigenvalues log-uniform in [0.01, 4]), built with the exact FP32 join and
@@ -308,6 +367,8 @@ def main():
     ap.add_argument("--recall-queries", type=int, default=300)
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched uneven instances (256 mixtures, d in {8,16,32,64})")
+    ap.add_argument("--no-exact", action="store_true",
+                    help="skip the exact complete-graph MST (Prim) measurement")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision measurement (n=2M, d=128, BF16 Gram filter)")
     ap.add_argument("--no-serving", action="store_true",
@@ -441,6 +502,10 @@ def main():
     if not args.no_batched:
         batched = batched_run(chip, datagen)
 
+    exact = None
+    if not args.no_exact:
+        exact = exact_mst_run(chip, datagen, _load("coresg_io"))
+
     mixed = None
     if not args.no_mixed:
         mixed = mixed_precision_run(chip, datagen)
@@ -493,6 +558,7 @@ def main():
         "cpu_baseline": cpu,
         "serving": serving,
         "mixed_precision": mixed,
+        "exact_mst": exact,
         "batched_instances": batched,
     }
     print(json.dumps(out))
