@@ -131,6 +131,11 @@ typedef struct coresg_hip_build_params {
      * cannot pass either threshold test and compute every other pair in exact
      * FP32, so all modes propose the same candidates. */
     uint32_t gram;
+    /* bridge set (MissingEdgeMode, coresg.hpp:54-58; coresg.cpp:143-169):
+     * 0 = AntiHub (default), 1 = ExactMst (the exact MST at k_max, stored
+     * with base distances), 2 = None. Modes 1 and 2 apply to single-instance
+     * graphs. */
+    uint32_t missing_mode;
 } coresg_hip_build_params;
 
 typedef struct coresg_hip_build_stats {

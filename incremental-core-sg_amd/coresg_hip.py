@@ -41,7 +41,8 @@ class BuildParams(C.Structure):
                 ("min_selected", C.c_size_t), ("max_selected", C.c_size_t),
                 ("min_per_component", C.c_size_t), ("ensure_connected", C.c_int),
                 ("reps_per_component", C.c_size_t), ("max_reps", C.c_size_t),
-                ("join_capacity", C.c_uint32), ("gram", C.c_uint32)]
+                ("join_capacity", C.c_uint32), ("gram", C.c_uint32),
+                ("missing_mode", C.c_uint32)]
 
 
 class BuildStats(C.Structure):

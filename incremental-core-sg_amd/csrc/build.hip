@@ -193,12 +193,22 @@ void build_bridges(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_
     g.repair_count = 0;
     std::uint64_t knn_edges = 0;
     double bridge_s = 0.0;
+    if (p.missing_mode > 2) throw std::runtime_error("build_bridges: unknown missing_mode");
+    if (p.missing_mode != 0 && !g.inst_off.empty())
+        throw std::runtime_error("build_bridges: missing_mode 1/2 needs a single-instance graph");
     if (g.inst_off.empty()) {
         knn_edges = count_knn_edges(g);
         const std::size_t budget = antihub_budget(p, g.dim, knn_edges, g.n);
         g.selected = select_antihubs(g, budget, p.min_per_component);
         const auto t1 = Clock::now();
-        mst_over_subset(g, g.selected, &g.bu, &g.bv, &g.bw);
+        // coresg.cpp:148-166: the anti-hub selection is made in every mode
+        // (the hubness index is refreshed regardless); the bridges follow the
+        // mode.
+        if (p.missing_mode == 0) {
+            mst_over_subset(g, g.selected, &g.bu, &g.bv, &g.bw);
+        } else if (p.missing_mode == 1) {
+            exact_mst_bridges(g, &g.bu, &g.bv, &g.bw);
+        }
         bridge_s += since(t1);
     } else {
         // per instance, as if built alone: budget from the instance's own
@@ -231,7 +241,8 @@ void build_bridges(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_
     const auto t1 = Clock::now();
     if (st) {
         st->knn_edges = knn_edges;
-        st->antihubs_selected = g.selected.size();
+        // set in the anti-hub mode only (coresg.cpp:151-155)
+        st->antihubs_selected = p.missing_mode == 0 ? g.selected.size() : 0;
         st->repair_edges = 0;
     }
     repair_connectivity(g, p, st);

@@ -330,6 +330,17 @@ __global__ void __launch_bounds__(kThreads) exact_prim_lds_kernel(Prim p) {
 
 using PrimKernel = void (*)(Prim);
 
+// Base distance of each (u, v) pair, metric_distance (distance.hpp:196-201).
+__global__ void __launch_bounds__(kThreads)
+    edge_base_kernel(const float* __restrict__ pts, std::uint32_t stride,
+                     const std::uint32_t* __restrict__ u, const std::uint32_t* __restrict__ v,
+                     std::size_t m, float* __restrict__ w) {
+    for (std::size_t e = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
+         e += std::size_t(gridDim.x) * blockDim.x)
+        w[e] = sqrt_exact(l2sqr_exact(pts + std::size_t(u[e]) * stride,
+                                      pts + std::size_t(v[e]) * stride, stride));
+}
+
 std::uint32_t find_root(std::vector<std::uint32_t>& parent, std::uint32_t x) {
     while (parent[x] != x) {
         parent[x] = parent[parent[x]];
@@ -472,6 +483,35 @@ void exact_mst(Graph& g, std::uint32_t mpts, void* info_out, std::uint32_t* out_
         info->rounds = n >= 2 ? n - 1 : 0;
         info->weight = weight;
     }
+}
+
+// MissingEdgeMode::ExactMst bridges (coresg.cpp:157-163): the exact MST at
+// k_max, each edge re-stored with its base distance.
+void exact_mst_bridges(Graph& g, std::vector<std::uint32_t>* bu, std::vector<std::uint32_t>* bv,
+                       std::vector<float>* bw) {
+    const std::size_t cap = g.n >= 2 ? g.n - 1 : 1;
+    bu->assign(cap, 0);
+    bv->assign(cap, 0);
+    bw->assign(cap, 0.0f);
+    coresg_hip_mst_info info{};
+    exact_mst(g, g.k, &info, bu->data(), bv->data(), bw->data(), nullptr);
+    const std::size_t m = info.num_edges;
+    bu->resize(m);
+    bv->resize(m);
+    bw->resize(m);
+    if (m == 0) return;
+    DBuf<std::uint32_t> du, dv;
+    DBuf<float> dw;
+    du.reserve(m);
+    dv.reserve(m);
+    dw.reserve(m);
+    CSG_CHECK(hipMemcpyAsync(du.p, bu->data(), 4 * m, hipMemcpyHostToDevice, g.stream));
+    CSG_CHECK(hipMemcpyAsync(dv.p, bv->data(), 4 * m, hipMemcpyHostToDevice, g.stream));
+    edge_base_kernel<<<capped_blocks(m, kThreads), kThreads, 0, g.stream>>>(g.pts.p, g.stride, du.p,
+                                                                           dv.p, m, dw.p);
+    CSG_LAUNCH_CHECK();
+    CSG_CHECK(hipMemcpyAsync(bw->data(), dw.p, 4 * m, hipMemcpyDeviceToHost, g.stream));
+    CSG_CHECK(hipStreamSynchronize(g.stream));
 }
 
 }  // namespace coresg_hip

@@ -254,6 +254,10 @@ void extract_instances(Graph& g, const std::uint32_t* mpts, std::uint32_t count,
 // coresg_hip_mst_info.
 void exact_mst(Graph& g, std::uint32_t mpts, void* info, std::uint32_t* out_u,
                std::uint32_t* out_v, float* out_w, std::uint32_t* out_labels);
+// MissingEdgeMode::ExactMst bridges: the exact MST at k_max with base
+// distances (exact.hip).
+void exact_mst_bridges(Graph& g, std::vector<std::uint32_t>* bu, std::vector<std::uint32_t>* bv,
+                       std::vector<float>* bw);
 // Undirected k-NN edge count of every instance.
 std::vector<std::uint64_t> knn_edges_per_instance(Graph& g);
 

@@ -149,6 +149,20 @@ def make_exact(tmp):
             for key in ("mst_u", "mst_v", "mst_w", "component", "num_components", "weight"):
                 out[f"{name}_exact{m}_{key}"] = t[key]
         print(f"exact_mst: {name} mpts={mpts}")
+    # MissingEdgeMode::ExactMst builds (coresg.cpp:157-163) by the reference CLI,
+    # with brute-force lists so any exact k-NN reproduces them.
+    for name in ("brute_d8", "islands"):
+        with np.load(os.path.join(HERE, f"{name}.npz")) as z:
+            x, k = z["data"], int(z["k"])
+        data_path = os.path.join(tmp, f"{name}_x.bin")
+        x.tofile(data_path)
+        sg_path = os.path.join(tmp, f"{name}_exactmode.coresg")
+        run([os.path.join(REF, "coresg_cli"), "build", "--data", data_path, "--dim",
+             str(x.shape[1]), "--kmax", str(k), "--threads", "1", "--instances", "1", "--gpu", "0",
+             "--verbose", "0", "--brute-force-limit", "100000", "--missing", "exact", "--out",
+             sg_path])
+        out[f"{name}_exactmode_coresg_bytes"] = np.frombuffer(open(sg_path, "rb").read(), np.uint8)
+        print(f"exact-mode build: {name}")
     np.savez_compressed(os.path.join(HERE, "exact_mst.npz"), **out)
 
 

@@ -5,6 +5,7 @@ Theorem 1 (test_coresg.cpp:32-61) at a size the oracle is not run at."""
 from __future__ import annotations
 
 import os
+import re
 
 import numpy as np
 import pytest
@@ -18,7 +19,8 @@ datagen = load_pkg_module("datagen")
 pytestmark = pytest.mark.gpu
 
 EXACT = golden("exact_mst")
-RUNS = sorted({(k.split("_exact")[0], int(k.split("_exact")[1].split("_")[0])) for k in EXACT})
+RUNS = sorted({(m.group(1), int(m.group(2)))
+               for m in (re.match(r"^(.*)_exact(\d+)_", k) for k in EXACT) if m})
 
 
 def graph_from_file(f):
@@ -129,4 +131,31 @@ def test_exact_mst_wide_rows_generic_kernel(tmp_path):
     sg = oracle.CoreSG.load(path)
     for m in (1, 8):
         assert_same(g.exact_mst(m, labels=True), sg.exact_mst(m))
+    g.close()
+
+
+@pytest.mark.parametrize("name", ["brute_d8", "islands"])
+def test_exactmode_build_matches_reference(name):
+    """A device build with missing_mode = 1 (ExactMst) and brute-force lists
+    against the reference CLI's `build --missing exact` on the same data: the
+    same lists, the same bridges (Prim order, base distances) bit for bit, no
+    repair edges; missing_mode = 2 (None) leaves no bridges."""
+    print(f"[gpu] exact-mode build vs reference: {name}", flush=True)
+    f = coresg_io.loads(EXACT[f"{name}_exactmode_coresg_bytes"].tobytes())
+    x = golden(name)["data"]
+    g = chip.Graph(x.shape[0], x.shape[1], f.k_max)
+    g.set_points(x)
+    st = g.build(chip.build_params(brute_force_limit=100000, missing_mode=1))
+    ids, dist, _, count = g.get_knn()
+    assert np.array_equal(ids, f.ids) and np.array_equal(count, f.count)
+    assert np.array_equal(dist.view(np.uint32), f.dist.view(np.uint32))
+    u, v, w, rc = g.get_bridges()
+    fu, fv, fw = f.bridge_uvw()
+    assert np.array_equal(u, fu) and np.array_equal(v, fv)
+    assert np.array_equal(w.view(np.uint32), fw.view(np.uint32))
+    assert rc == f.repair_count == 0
+    assert st["antihubs_selected"] == 0
+    st = g.build(chip.build_params(brute_force_limit=100000, missing_mode=2))
+    u, _, _, _ = g.get_bridges()
+    assert u.size == st["repair_edges"]
     g.close()

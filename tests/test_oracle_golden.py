@@ -4,6 +4,7 @@ against closed forms and brute force for what the fixtures leave open."""
 from __future__ import annotations
 
 import os
+import re
 
 import numpy as np
 import pytest
@@ -115,8 +116,8 @@ def test_mst_sweep_bitexact(name):
 
 
 EXACT = golden("exact_mst")
-EXACT_RUNS = sorted({(k.split("_exact")[0], int(k.split("_exact")[1].split("_")[0]))
-                     for k in EXACT})
+EXACT_RUNS = sorted({(m.group(1), int(m.group(2)))
+                     for m in (re.match(r"^(.*)_exact(\d+)_", k) for k in EXACT) if m})
 
 
 @pytest.mark.parametrize("name,mpts", EXACT_RUNS)
@@ -132,6 +133,23 @@ def test_exact_mst_bitexact(name, mpts):
     assert np.array_equal(r["labels"], EXACT[p + "component"])
     assert r["ncomp"] == int(EXACT[p + "num_components"][0])
     assert np.float64(r["weight"]).view(np.uint64) == EXACT[p + "weight"].view(np.uint64)[0]
+
+
+@pytest.mark.parametrize("name", ["brute_d8", "islands"])
+def test_exactmode_bridges_from_oracle(name, tmp_path):
+    """MissingEdgeMode::ExactMst (coresg.cpp:157-163) as the reference CLI built
+    it: the bridges are the exact MST at k_max over the file's lists, in Prim
+    order, each re-stored with its base distance."""
+    f = coresg_io.loads(EXACT[f"{name}_exactmode_coresg_bytes"].tobytes())
+    assert f.mode == 1 and f.repair_count == 0
+    path = os.path.join(str(tmp_path), "e.coresg")
+    coresg_io.save(path, f)
+    sg = oracle.CoreSG.load(path)
+    e = sg.exact_mst(f.k_max)
+    _, bw = oracle.l2sqr_pairs(f.points, np.stack([e["u"], e["v"]], 1).astype(np.uint32))
+    u, v, w = f.bridge_uvw()
+    assert np.array_equal(u, e["u"]) and np.array_equal(v, e["v"])
+    assert np.array_equal(w.view(np.uint32), bw.view(np.uint32))
 
 
 def test_exact_mst_theorem1_on_exact_knn():
