@@ -12,6 +12,9 @@
 //   * mrd_block: rows [r0, r1) of the m x m mutual-reachability matrix,
 //     bit-identical to the host backend's (same distance summation order).
 //   * trim: releases the calling thread's resident MRD operands.
+// It also offers DeviceExtract (device_extract.hpp): CoreSG's extraction half
+// (all_msts, exact_mst) on the device, over the CORE-SG the caller holds.
+#include <algorithm>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -19,6 +22,7 @@
 
 #include "coresg/backend.hpp"
 #include "coresg_hip.h"
+#include "device_extract.hpp"
 
 namespace coresg {
 namespace {
@@ -33,7 +37,45 @@ bool metric_to_hip(MetricKind m, int* code) {
     return false;
 }
 
-class HipBackend : public Backend {
+void hip_check(int ok, const char* what) {
+    if (!ok) throw std::runtime_error(std::string(what) + ": " + coresg_hip_last_error());
+}
+
+// A device-resident copy of a host CORE-SG: points, k-NN lists and bridges.
+struct DeviceGraph {
+    coresg_hip_graph* g = nullptr;
+    DeviceGraph(int device, const CoreSG& sg) {
+        const Dataset& d = sg.data();
+        const KnnGraph& knn = sg.knn();
+        const auto n = static_cast<std::uint32_t>(knn.size());
+        g = coresg_hip_graph_create(device, n, d.dim(), knn.k());
+        hip_check(g != nullptr, "coresg_hip_graph_create");
+        hip_check(coresg_hip_graph_set_points(g, d.data(), CORESG_HIP_HOST, 1), "set_points");
+        hip_check(coresg_hip_graph_set_knn(g, knn.ids_storage().data(),
+                                           knn.dist_storage().data(),
+                                           knn.flag_storage().data(),
+                                           knn.count_storage().data(), CORESG_HIP_HOST),
+                  "set_knn");
+        const std::vector<Edge>& b = sg.missing_edges();
+        std::vector<std::uint32_t> u(b.size()), v(b.size());
+        std::vector<float> w(b.size());
+        for (std::size_t i = 0; i < b.size(); ++i) {
+            u[i] = b[i].u;
+            v[i] = b[i].v;
+            w[i] = b[i].w;
+        }
+        hip_check(coresg_hip_graph_set_bridges(g, u.data(), v.data(), w.data(), b.size(),
+                                               sg.repair_edge_count()),
+                  "set_bridges");
+    }
+    ~DeviceGraph() {
+        if (g) coresg_hip_graph_destroy(g);
+    }
+    DeviceGraph(const DeviceGraph&) = delete;
+    DeviceGraph& operator=(const DeviceGraph&) = delete;
+};
+
+class HipBackend : public Backend, public DeviceExtract {
 public:
     HipBackend(const std::vector<int>& device_ids, bool oom_fallback)
         : ids_(device_ids), oom_fallback_(oom_fallback) {
@@ -129,6 +171,53 @@ public:
 
     void trim() override { coresg_hip_release_thread_cache(); }
 
+    std::vector<MstResult> extract_msts(const CoreSG& sg,
+                                        const std::vector<std::uint32_t>& mpts) override {
+        const std::size_t n = sg.knn().size();
+        if (n < 2 || mpts.empty()) return sg.all_msts(mpts);
+        DeviceGraph dg(device_for(0), sg);
+        const std::size_t q = mpts.size(), n1 = n - 1;
+        std::vector<coresg_hip_mst_info> info(q);
+        std::vector<std::uint32_t> u(q * n1), v(q * n1), lab(q * n);
+        std::vector<float> w(q * n1);
+        hip_check(coresg_hip_extract(dg.g, mpts.data(), static_cast<std::uint32_t>(q), info.data(),
+                                     u.data(), v.data(), w.data(), lab.data()),
+                  "coresg_hip_extract");
+        std::vector<MstResult> out(q);
+        for (std::size_t i = 0; i < q; ++i) {
+            MstResult& r = out[i];
+            r.edges.reserve(info[i].num_edges);
+            for (std::size_t e = 0; e < info[i].num_edges; ++e)
+                r.edges.emplace_back(u[i * n1 + e], v[i * n1 + e], w[i * n1 + e]);
+            // Kruskal's acceptance order is the edge_less order of the tree.
+            std::sort(r.edges.begin(), r.edges.end(), EdgeLess());
+            r.component.assign(lab.begin() + i * n, lab.begin() + (i + 1) * n);
+            r.num_components = info[i].num_components;
+            r.weight = info[i].weight;
+        }
+        return out;
+    }
+
+    MstResult exact_mst(const CoreSG& sg, std::uint32_t mpts) override {
+        int code = 0;
+        const std::size_t n = sg.knn().size();
+        if (!metric_to_hip(sg.data().metric(), &code) || n < 2) return sg.exact_mst(mpts);
+        DeviceGraph dg(device_for(0), sg);
+        coresg_hip_mst_info info{};
+        std::vector<std::uint32_t> u(n - 1), v(n - 1), lab(n);
+        std::vector<float> w(n - 1);
+        hip_check(coresg_hip_exact_mst(dg.g, mpts, &info, u.data(), v.data(), w.data(),
+                                       lab.data()),
+                  "coresg_hip_exact_mst");
+        MstResult r;
+        r.edges.reserve(info.num_edges);
+        for (std::size_t e = 0; e < info.num_edges; ++e) r.edges.emplace_back(u[e], v[e], w[e]);
+        r.component = std::move(lab);
+        r.num_components = info.num_components;
+        r.weight = info.weight;
+        return r;
+    }
+
 private:
     int device_for(unsigned slot) const { return ids_[slot < ids_.size() ? slot : 0]; }
 
@@ -139,6 +228,16 @@ private:
 };
 
 }  // namespace
+
+std::vector<MstResult> all_msts_routed(const CoreSG& sg, const std::vector<std::uint32_t>& mpts) {
+    if (auto* dev = dynamic_cast<DeviceExtract*>(sg.backend())) return dev->extract_msts(sg, mpts);
+    return sg.all_msts(mpts);
+}
+
+MstResult exact_mst_routed(const CoreSG& sg, std::uint32_t mpts) {
+    if (auto* dev = dynamic_cast<DeviceExtract*>(sg.backend())) return dev->exact_mst(sg, mpts);
+    return sg.exact_mst(mpts);
+}
 
 std::unique_ptr<Backend> make_hip_backend(const std::vector<int>& device_ids,
                                           bool oom_fallback) {

@@ -49,3 +49,22 @@ def test_reference_cli_with_hip_backend(tmp_path):
                         "1,4,16"], check=True, capture_output=True, text=True).stdout
     strip = lambda s: [re.sub(r"\(.*s\)", "", l) for l in s.splitlines() if l.startswith("mpts")]
     assert strip(a) == strip(b)
+
+
+EXTRACT_CHECK = os.path.join(REPO, "oracle", "_ref", "extract_check")
+
+
+@pytest.mark.skipif(not os.path.exists(EXTRACT_CHECK), reason="integration build not present")
+@pytest.mark.parametrize("name,mpts", [("mix_d16", "1 2 4 8 16"), ("ties", "1 3 8"),
+                                       ("islands_repair", "1 3 5")])
+def test_reference_coresg_routes_extraction_to_device(name, mpts, tmp_path):
+    """The reference's CoreSG with HipBackend: all_msts and exact_mst routed
+    through the DeviceExtract capability give what the host backend gives, edge
+    for edge (tests/cpp/extract_check.cpp)."""
+    print(f"[gpu] DeviceExtract routing: {name} mpts {mpts}", flush=True)
+    path = tmp_path / "g.coresg"
+    golden(name)["coresg_bytes"].tofile(path)
+    r = subprocess.run([EXTRACT_CHECK, str(path)] + mpts.split(), capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "equal (backend hip)" in r.stdout
