@@ -58,9 +58,9 @@ EXTRACT_CHECK = os.path.join(REPO, "oracle", "_ref", "extract_check")
 @pytest.mark.parametrize("name,mpts", [("mix_d16", "1 2 4 8 16"), ("ties", "1 3 8"),
                                        ("islands_repair", "1 3 5")])
 def test_reference_coresg_routes_extraction_to_device(name, mpts, tmp_path):
-    """The reference's CoreSG with HipBackend: all_msts and exact_mst routed
-    through the DeviceExtract capability give what the host backend gives, edge
-    for edge (tests/cpp/extract_check.cpp)."""
+    """The reference's CoreSG with HipBackend: all_msts routed through the
+    DeviceExtract capability gives what the host backend gives, edge for edge
+    (tests/cpp/extract_check.cpp)."""
     print(f"[gpu] DeviceExtract routing: {name} mpts {mpts}", flush=True)
     path = tmp_path / "g.coresg"
     golden(name)["coresg_bytes"].tofile(path)
@@ -68,3 +68,14 @@ def test_reference_coresg_routes_extraction_to_device(name, mpts, tmp_path):
                        timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "equal (backend hip)" in r.stdout
+
+
+@pytest.mark.skip(reason="HipBackend::exact_mst routing is switched off: in the extract_check "
+                         "process its cooperative launch once returned hipErrorIllegalState "
+                         "after an extract_msts call, with no cause found yet")
+def test_reference_coresg_routes_exact_mst_to_device(tmp_path):
+    path = tmp_path / "g.coresg"
+    golden("mix_d16")["coresg_bytes"].tofile(path)
+    r = subprocess.run([EXTRACT_CHECK, "--exact", str(path), "1", "4", "16"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
