@@ -108,10 +108,13 @@ std::size_t repair_range(Graph& g, const coresg_hip_build_params& p,
     std::vector<std::uint32_t> reps;
     for (const auto& b : buckets) reps.insert(reps.end(), b.begin(), b.end());
     if (p.max_reps && reps.size() > p.max_reps) {
-        // The reference's partial_sort order among equal scores is not
-        // reproduced (only reachable with more than max_reps components).
-        std::stable_sort(reps.begin(), reps.end(),
-                         [&](std::uint32_t a, std::uint32_t b) { return score[a] < score[b]; });
+        // The reference's cut (coresg.cpp:221-227): std::partial_sort by score
+        // alone. Among equal scores its heap procedure decides which
+        // representatives survive and in what order (the order feeds Prim's
+        // first-index tie rule), so the same call on the same sequence is made
+        // here rather than any sort.
+        std::partial_sort(reps.begin(), reps.begin() + p.max_reps, reps.end(),
+                          [&](std::uint32_t a, std::uint32_t b) { return score[a] < score[b]; });
         reps.resize(p.max_reps);
     }
     if (reps.size() < 2) return 0;

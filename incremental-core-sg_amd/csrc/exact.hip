@@ -428,10 +428,29 @@ void exact_mst(Graph& g, std::uint32_t mpts, void* info_out, std::uint32_t* out_
                ctl.p, ctl.p + 1, n, g.stride, chunk};
         void* args[] = {&p};
         {
+            // A cooperative launch cannot be captured; say so instead of
+            // launching into a stream that a failed capture left open.
+            hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+            CSG_CHECK(hipStreamIsCapturing(g.stream, &cap));
+            if (cap != hipStreamCaptureStatusNone)
+                throw std::runtime_error("exact_mst: the graph's stream is capturing (status " +
+                                         std::to_string(int(cap)) + ")");
             ScopedKernelTimer t(g, "exact_prim");
-            CSG_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern),
-                                                 dim3(blocks), dim3(kThreads), args,
-                                                 static_cast<unsigned>(shmem), g.stream));
+            const hipError_t e = hipLaunchCooperativeKernel(
+                reinterpret_cast<const void*>(kern), dim3(blocks), dim3(kThreads), args,
+                static_cast<unsigned>(shmem), g.stream);
+            if (e != hipSuccess) {
+                hipStreamCaptureStatus after = hipStreamCaptureStatusNone;
+                const hipError_t qe = hipStreamIsCapturing(g.stream, &after);
+                throw HipError(std::string("hipLaunchCooperativeKernel(exact_prim, ") +
+                                   std::to_string(blocks) + " blocks, " + std::to_string(shmem) +
+                                   " B LDS) -> " + hipGetErrorName(e) + ": " +
+                                   hipGetErrorString(e) + " [stream capture status " +
+                                   (qe == hipSuccess ? std::to_string(int(after))
+                                                     : std::string(hipGetErrorName(qe))) +
+                                   "]",
+                               e);
+            }
         }
         std::uint32_t err = 0;
         std::vector<std::uint32_t> a(n - 1), b(n - 1);

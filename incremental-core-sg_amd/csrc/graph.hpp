@@ -84,6 +84,46 @@ struct HostBuf {
     }
 };
 
+// A non-blocking stream destroyed with its owner (also on an exception path).
+struct OwnedStream {
+    hipStream_t s = nullptr;
+    OwnedStream() { CSG_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+    OwnedStream(const OwnedStream&) = delete;
+    OwnedStream& operator=(const OwnedStream&) = delete;
+    ~OwnedStream() {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    }
+};
+
+// Stream capture bracket: ends the capture and discards its graph when left by
+// an exception, so the stream never stays in capture mode after a failure.
+struct CaptureGuard {
+    hipStream_t s;
+    bool open = false;
+    explicit CaptureGuard(hipStream_t stream) : s(stream) {
+        CSG_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        open = true;
+    }
+    CaptureGuard(const CaptureGuard&) = delete;
+    CaptureGuard& operator=(const CaptureGuard&) = delete;
+    // Ends the capture; the caller owns the returned graph.
+    hipGraph_t end() {
+        hipGraph_t graph = nullptr;
+        open = false;
+        CSG_CHECK(hipStreamEndCapture(s, &graph));
+        return graph;
+    }
+    ~CaptureGuard() {
+        if (!open) return;
+        hipGraph_t graph = nullptr;
+        if (hipStreamEndCapture(s, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+    }
+};
+
 // Two reusable events (no timing).
 struct EventPair {
     hipEvent_t e[2] = {nullptr, nullptr};

@@ -17,8 +17,13 @@ void set_last_error(const std::string& msg);
 const char* get_last_error();
 
 // Runs `f`, converting exceptions into the 0-return + last_error convention.
+// HIP keeps the last failing call's code per thread until it is read, and the
+// launch checks read it (hipGetLastError): an error a caller of this library
+// left behind, or one a handled failure left, would otherwise surface at the
+// next unrelated launch. Every entry point therefore starts with it cleared.
 template <typename F>
 int guarded(F&& f) {
+    (void)hipGetLastError();
     try {
         f();
         return 1;

@@ -73,6 +73,9 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+}  // namespace
+
+// Rows [r0, r0 + rows) of the m x m MRD matrix of resident operands, on s.
 void launch_mrd(const float* pts, std::uint32_t m, std::uint32_t stride, const float* core,
                 std::uint32_t r0, std::uint32_t rows, float* out, hipStream_t s) {
     if (rows == 0 || m == 0) return;
@@ -86,6 +89,8 @@ void launch_mrd(const float* pts, std::uint32_t m, std::uint32_t stride, const f
     }
     CSG_LAUNCH_CHECK();
 }
+
+namespace {
 
 // Argmin of (value, index) across a 256-thread block; ties -> lower index.
 __device__ __forceinline__ void block_argmin(float* v, std::uint32_t* i, float* sv,
@@ -332,20 +337,30 @@ __global__ void __launch_bounds__(256)
 }
 
 // Per-thread residency of the MRD operands: Prim over a virtual graph asks for
-// one row at a time over the same points, so they stay on the device until the
-// caller's pointers change or release_thread_cache() is called.
+// one row at a time over the same points, so they stay on the device between
+// calls. A host pointer says nothing about what it points at -- a caller that
+// frees one vector and allocates the next (fresh core distances per mpts, a new
+// gathered subset) often gets the same address back with other contents -- so
+// the cache also keeps host copies of what it uploaded: the core distances are
+// compared on every call (m floats, far less than the row's m distances) and the
+// points whenever a row block starts at row 0, which is how every Prim
+// (mst_prim_virtual starts from vertex 0, src/mst.cpp:186-192) and every
+// row-split matrix (JobScheduler::mrd_matrix, backend.hpp:149-164) begins.
 struct MrdCache {
     int device = -1;
     const float* pts_key = nullptr;
     const float* core_key = nullptr;
     std::size_t m = 0;
     std::uint32_t stride = 0;
+    std::vector<float> host_pts, host_core;  // what the device copies hold
     DBuf<float> pts, core, out;
     hipStream_t stream = nullptr;
     void release() {
         pts.release();
         core.release();
         out.release();
+        host_pts.clear();
+        host_core.clear();
         pts_key = core_key = nullptr;
         m = 0;
         stride = 0;
@@ -359,19 +374,31 @@ struct MrdCache {
 thread_local MrdCache t_cache;
 
 void ensure_resident(int device, const float* host_pts, std::size_t m, std::uint32_t stride,
-                     const float* host_core) {
+                     const float* host_core, bool check_points) {
+    const std::size_t np = m * stride;
     if (t_cache.device == device && t_cache.pts_key == host_pts &&
-        t_cache.core_key == host_core && t_cache.m == m && t_cache.stride == stride)
-        return;
+        t_cache.core_key == host_core && t_cache.m == m && t_cache.stride == stride) {
+        const bool pts_same =
+            !check_points || std::memcmp(t_cache.host_pts.data(), host_pts, np * 4) == 0;
+        if (pts_same && std::memcmp(t_cache.host_core.data(), host_core, m * 4) == 0) return;
+        if (pts_same) {  // new core distances over the same points
+            t_cache.host_core.assign(host_core, host_core + m);
+            CSG_CHECK(hipMemcpyAsync(t_cache.core.p, t_cache.host_core.data(), m * 4,
+                                     hipMemcpyHostToDevice, t_cache.stream));
+            return;
+        }
+    }
     t_cache.release();
     t_cache.device = device;
     CSG_CHECK(hipStreamCreateWithFlags(&t_cache.stream, hipStreamNonBlocking));
-    t_cache.pts.reserve(m * stride);
+    t_cache.pts.reserve(np);
     t_cache.core.reserve(m);
-    CSG_CHECK(hipMemcpyAsync(t_cache.pts.p, host_pts, m * stride * 4, hipMemcpyHostToDevice,
-                             t_cache.stream));
-    CSG_CHECK(hipMemcpyAsync(t_cache.core.p, host_core, m * 4, hipMemcpyHostToDevice,
-                             t_cache.stream));
+    t_cache.host_pts.assign(host_pts, host_pts + np);
+    t_cache.host_core.assign(host_core, host_core + m);
+    CSG_CHECK(hipMemcpyAsync(t_cache.pts.p, t_cache.host_pts.data(), np * 4,
+                             hipMemcpyHostToDevice, t_cache.stream));
+    CSG_CHECK(hipMemcpyAsync(t_cache.core.p, t_cache.host_core.data(), m * 4,
+                             hipMemcpyHostToDevice, t_cache.stream));
     t_cache.pts_key = host_pts;
     t_cache.core_key = host_core;
     t_cache.m = m;
@@ -528,7 +555,7 @@ void mrd_block_host(int device, const float* pts, std::size_t m, std::uint32_t d
     check_metric(metric);
     if (stride % 8 != 0) throw std::runtime_error("mrd_block: stride must be a multiple of 8");
     if (r1 > m) throw std::runtime_error("mrd_block: row range exceeds m");
-    ensure_resident(device, pts, m, stride, core);
+    ensure_resident(device, pts, m, stride, core, r0 == 0);
     const std::size_t rows = r1 - r0;
     t_cache.out.reserve(rows * m);
     launch_mrd(t_cache.pts.p, static_cast<std::uint32_t>(m), stride, t_cache.core.p,
@@ -549,8 +576,8 @@ void mrd_row_min_host(int device, const float* pts, std::size_t m, std::uint32_t
     check_metric(metric);
     if (stride % 8 != 0) throw std::runtime_error("mrd_row_min: stride must be a multiple of 8");
     if (r1 > m) throw std::runtime_error("mrd_row_min: row range exceeds m");
-    hipStream_t s;
-    CSG_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    OwnedStream os;
+    hipStream_t s = os.s;
     const std::size_t rows = r1 - r0;
     DBuf<float> d_pts, d_core, d_min;
     DBuf<std::uint32_t> d_comp, d_arg;
@@ -571,7 +598,6 @@ void mrd_row_min_host(int device, const float* pts, std::size_t m, std::uint32_t
     CSG_CHECK(hipMemcpyAsync(out_min, d_min.p, rows * 4, hipMemcpyDeviceToHost, s));
     CSG_CHECK(hipMemcpyAsync(out_argmin, d_arg.p, rows * 4, hipMemcpyDeviceToHost, s));
     CSG_CHECK(hipStreamSynchronize(s));
-    CSG_CHECK(hipStreamDestroy(s));
 }
 
 void release_thread_cache() { t_cache.release(); }

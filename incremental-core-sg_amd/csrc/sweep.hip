@@ -716,12 +716,14 @@ void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_
             if (g.sweep_exec) CSG_CHECK(hipGraphExecDestroy(g.sweep_exec));
             g.sweep_exec = nullptr;
             hipGraph_t graph = nullptr;
-            CSG_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            for (std::uint32_t r = 0; r < kGraphRounds; ++r) {
-                launch_round(sw, false, vblk, s);
-                round_tail();
+            {
+                CaptureGuard cap(s);
+                for (std::uint32_t r = 0; r < kGraphRounds; ++r) {
+                    launch_round(sw, false, vblk, s);
+                    round_tail();
+                }
+                graph = cap.end();
             }
-            CSG_CHECK(hipStreamEndCapture(s, &graph));
             const hipError_t e = hipGraphInstantiate(&g.sweep_exec, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
             CSG_CHECK(e);

@@ -205,6 +205,9 @@ public:
     // reference's Prim over mrd_block rows, which this backend serves).
     // coresg_hip_exact_mst itself is unaffected and tested on its own.
     MstResult exact_mst(const CoreSG& sg, std::uint32_t mpts) override {
+        // A fresh start for the resident MRD operands: each call hands
+        // mrd_block a new core-distance vector.
+        coresg_hip_release_thread_cache();
         return sg.exact_mst(mpts);
     }
 

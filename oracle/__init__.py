@@ -6,11 +6,9 @@ imports it.
 
 Parity of each function is pinned by tests/test_oracle_golden.py against
 vectors the reference itself wrote (tests/golden/), plus closed forms and
-brute force where those leave gaps. Parity unpinned: the connectivity repair's
-representative cut above max_reps = 4096 (recompute_bridges in
-coresg_oracle.c; the reference's partial_sort tie order is
-implementation-defined and is restated as a stable sort; no fixture has that
-many components).
+brute force where those leave gaps, including the connectivity repair's
+representative cut above max_reps (libstdc++'s partial_sort restated step for
+step, against reference builds with tied scores, tests/golden/maxreps.npz).
 """
 from __future__ import annotations
 

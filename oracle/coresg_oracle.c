@@ -3,10 +3,8 @@
  * Used by tests/, the smoke check and the CPU-baseline leg of bench.py as the
  * checker. Nothing in the product library links or calls it.
  *
- * Parity unpinned: the representative cut of the connectivity repair above
- * max_reps (a stable sort standing in for the reference's partial_sort, whose
- * tie order is implementation-defined); every other function is checked
- * against the reference's own vectors or closed forms.
+ * Every function is checked against the reference's own vectors or closed
+ * forms (tests/test_oracle_golden.py).
  *
  * Each function names the reference function it restates (path:line relative
  * to the reference repository). The schedule is the reference's serial one
@@ -918,18 +916,71 @@ static uint32_t labels_with(const orc_sg* sg, const orc_edge* extra, size_t next
     return nc;
 }
 
-static const double* g_rep_score;
-static int rep_cmp(const void* pa, const void* pb) {
-    const uint32_t a = *(const uint32_t*)pa, b = *(const uint32_t*)pb;
-    if (g_rep_score[a] < g_rep_score[b]) return -1;
-    if (g_rep_score[b] < g_rep_score[a]) return 1;
-    return 0;
+/* std::partial_sort(first, first + mid, last, comp) as GCC 11's libstdc++
+ * implements it (bits/stl_algo.h: __partial_sort = __heap_select then
+ * __sort_heap; bits/stl_heap.h: __make_heap, __adjust_heap, __push_heap,
+ * __pop_heap), restated over an index array ordered by a double key. The
+ * reference calls it with `score[a] < score[b]` (src/coresg.cpp:222-225); with
+ * tied scores the surviving set and its order are those of this heap
+ * procedure, so it is restated step for step rather than by any sort. */
+static int ps_less(const double* key, uint32_t a, uint32_t b) { return key[a] < key[b]; }
+
+static void ps_push_heap(uint32_t* f, ptrdiff_t hole, ptrdiff_t top, uint32_t value,
+                         const double* key) {
+    ptrdiff_t parent = (hole - 1) / 2;
+    while (hole > top && ps_less(key, f[parent], value)) {
+        f[hole] = f[parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    f[hole] = value;
+}
+
+static void ps_adjust_heap(uint32_t* f, ptrdiff_t hole, ptrdiff_t len, uint32_t value,
+                           const double* key) {
+    const ptrdiff_t top = hole;
+    ptrdiff_t child = hole;
+    while (child < (len - 1) / 2) {
+        child = 2 * (child + 1);
+        if (ps_less(key, f[child], f[child - 1])) child--;
+        f[hole] = f[child];
+        hole = child;
+    }
+    if ((len & 1) == 0 && child == (len - 2) / 2) {
+        child = 2 * (child + 1);
+        f[hole] = f[child - 1];
+        hole = child - 1;
+    }
+    ps_push_heap(f, hole, top, value, key);
+}
+
+/* __pop_heap(first, last, result): the top goes to *result, *result's old
+ * value sinks from the root of [first, last). */
+static void ps_pop_heap(uint32_t* f, ptrdiff_t len, uint32_t* result, const double* key) {
+    const uint32_t value = *result;
+    *result = f[0];
+    ps_adjust_heap(f, 0, len, value, key);
+}
+
+static void partial_sort_by_key(uint32_t* f, size_t mid, size_t last, const double* key) {
+    const ptrdiff_t len = (ptrdiff_t)mid;
+    if (len >= 2) { /* __make_heap */
+        for (ptrdiff_t parent = (len - 2) / 2;; --parent) {
+            ps_adjust_heap(f, parent, len, f[parent], key);
+            if (parent == 0) break;
+        }
+    }
+    for (size_t i = mid; i < last; ++i) /* __heap_select */
+        if (ps_less(key, f[i], f[0])) ps_pop_heap(f, len, &f[i], key);
+    for (ptrdiff_t l = len; l > 1;) { /* __sort_heap */
+        --l;
+        ps_pop_heap(f, l, &f[l], key);
+    }
 }
 
 /* src/coresg.cpp:143-169 (recompute_missing_edges, AntiHub) and :171-258
- * (repair_connectivity). The >max_reps partial_sort is restated with a stable
- * sort; libstdc++'s partial_sort order among equal scores is not reproduced
- * (only reachable with more than max_reps / 1 = 4096 components). */
+ * (repair_connectivity), including the representative cut above max_reps by
+ * the reference's partial_sort (restated above). */
 static void recompute_bridges(orc_sg* sg) {
     free(sg->bridges);
     const size_t cap = sg->nselected + 1;
@@ -969,17 +1020,7 @@ static void recompute_bridges(orc_sg* sg) {
     for (uint32_t c = 0; c < ncomp; ++c)
         for (uint32_t t = 0; t < bsize[c]; ++t) reps[nreps++] = buckets[(size_t)c * per + t];
     if (sg->p.max_reps && nreps > sg->p.max_reps) {
-        g_rep_score = sg->score;
-        /* stable restatement of the partial_sort (see header comment) */
-        for (size_t i = 1; i < nreps; ++i) {
-            const uint32_t x = reps[i];
-            size_t j = i;
-            while (j > 0 && rep_cmp(&x, &reps[j - 1]) < 0) {
-                reps[j] = reps[j - 1];
-                --j;
-            }
-            reps[j] = x;
-        }
+        partial_sort_by_key(reps, sg->p.max_reps, nreps, sg->score);
         nreps = sg->p.max_reps;
     }
     if (nreps >= 2) {

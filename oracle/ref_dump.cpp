@@ -19,6 +19,10 @@
 //   exact  CORESG MPTS OUT           CoreSG::exact_mst(mpts), Prim acceptance order
 //   mrd    DATA DIM IDS CORE OUT     CPU mrd_block over a gathered subset
 //   recall CORESG QUERIES SEED OUT   sampled_recall against brute force
+//   reps   DATA DIM K MAXREPS MINPC BFLIMIT OUT.coresg
+//                                    CoreSG::build with max_reps / the
+//                                    anti-hub quota / brute-force limit set,
+//                                    saved (the repair's representative cut)
 
 #include <cstdio>
 #include <cstdlib>
@@ -251,6 +255,19 @@ int run(int argc, char** argv) {
         Tagged t(argv[5]);
         std::vector<double> v = {r.recall, r.distance_ratio, double(r.nodes)};
         t.vec("recall", 10, v);
+        return 0;
+    }
+    if (cmd == "reps" && argc == 9) {
+        const std::uint32_t dim = std::strtoul(argv[3], nullptr, 10);
+        CoreSGParams p = single_thread_params();
+        p.k_max = std::strtoul(argv[4], nullptr, 10);
+        p.max_reps = std::strtoull(argv[5], nullptr, 10);
+        p.hub.min_per_component = std::strtoull(argv[6], nullptr, 10);
+        p.incr.brute_force_limit = std::strtoull(argv[7], nullptr, 10);
+        std::vector<float> raw = read_f32(argv[2]);
+        CoreSG sg(p);
+        sg.build(raw.data(), raw.size() / dim, dim);
+        sg.save(argv[8]);
         return 0;
     }
     std::fprintf(stderr, "ref_dump: bad command line\n");

@@ -34,10 +34,22 @@ def load_pkg_module(name: str):
 
 
 def golden_cases():
-    """Per-input fixture files; exact_mst.npz holds extra outputs for several of them."""
+    """Per-input fixture files; exact_mst.npz holds extra outputs for several of
+    them and maxreps.npz its own set of builds (the repair's representative cut)."""
     names = (os.path.splitext(os.path.basename(p))[0]
              for p in glob.glob(os.path.join(GOLDEN_DIR, "*.npz")))
-    return sorted(x for x in names if x != "exact_mst")
+    return sorted(x for x in names if x not in ("exact_mst", "maxreps"))
+
+
+def maxreps_cases():
+    """(tag, data, max_reps, .coresg bytes) of the reference builds whose
+    connectivity repair cuts more than max_reps representatives."""
+    g = golden("maxreps")
+    out = []
+    for seed, jitter, max_reps in g["cases"]:
+        tag = f"s{seed}j{jitter}r{max_reps}"
+        out.append((tag, g[f"{tag}_data"], int(max_reps), g[f"{tag}_coresg_bytes"]))
+    return out
 
 
 def golden(name: str) -> dict:

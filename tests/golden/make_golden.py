@@ -6,7 +6,7 @@ deterministic serial one) on small seeded inputs, and stores inputs plus the
 reference's outputs as compressed .npz data fixtures next to this script. The
 reference is only needed to *write* fixtures; tests read the committed files.
 
-    python tests/golden/make_golden.py [--exact-only]
+    python tests/golden/make_golden.py [--exact-only | --maxreps-only]
 """
 from __future__ import annotations
 
@@ -166,13 +166,58 @@ def make_exact(tmp):
     np.savez_compressed(os.path.join(HERE, "exact_mst.npz"), **out)
 
 
+def translated_copies(per, d, clusters, levels, seed, jitter):
+    """`clusters` far-apart integer translations of one integer template of
+    `per` points: every island has the same k-NN structure, so their scores
+    tie exactly. With `jitter`, a quarter of the islands get one point moved,
+    which breaks some of the ties."""
+    rng = np.random.default_rng(seed)
+    tmpl = rng.integers(0, levels, size=(per, d)).astype(np.float64)
+    centres = np.arange(clusters)[:, None] * 1000.0 + np.zeros((1, d))
+    centres[:, 1:] = rng.integers(0, 50, size=(clusters, d - 1)) * 1000.0
+    pts = (centres[:, None, :] + tmpl[None]).reshape(-1, d)
+    if jitter:
+        for c in rng.choice(clusters, size=clusters // 4, replace=False):
+            pts[c * per + rng.integers(0, per)] += rng.integers(1, 3, size=d)
+    return pts[rng.permutation(pts.shape[0])].astype(np.float32)
+
+
+# (seed, jitter, max_reps): 40 components after the bridges and one
+# representative each, more than max_reps, with tied scores -- the repair's
+# partial_sort cut (coresg.cpp:221-227) decides which survive. A stable sort in
+# its place changes the repair edges of every case listed (checked when the
+# cases were chosen).
+MAXREPS_CASES = [(0, 0, 8), (0, 0, 13), (0, 0, 20), (1, 0, 13), (1, 1, 20), (2, 1, 13)]
+
+
+def make_maxreps(tmp):
+    out = {}
+    for seed, jitter, max_reps in MAXREPS_CASES:
+        x = translated_copies(10, 2, 40, 7, seed, jitter)
+        tag = f"s{seed}j{jitter}r{max_reps}"
+        data_path = os.path.join(tmp, f"{tag}.bin")
+        x.tofile(data_path)
+        sg_path = os.path.join(tmp, f"{tag}.coresg")
+        run([os.path.join(REF, "ref_dump"), "reps", data_path, str(x.shape[1]), "5",
+             str(max_reps), "0", "100000", sg_path])
+        out[f"{tag}_data"] = x
+        out[f"{tag}_coresg_bytes"] = np.frombuffer(open(sg_path, "rb").read(), np.uint8)
+        print(f"max_reps cut: {tag}")
+    out["cases"] = np.array(MAXREPS_CASES, np.uint32)
+    np.savez_compressed(os.path.join(HERE, "maxreps.npz"), **out)
+
+
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle"), "ref"], check=True)
     with tempfile.TemporaryDirectory() as tmp:
+        if "--maxreps-only" in sys.argv:
+            make_maxreps(tmp)
+            return
         if "--exact-only" not in sys.argv:
             for name, spec in CASES.items():
                 make_case(name, spec, tmp)
         make_exact(tmp)
+        make_maxreps(tmp)
 
 
 if __name__ == "__main__":

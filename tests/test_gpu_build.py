@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import GOLDEN_DIR, REPO, golden, golden_cases, load_pkg_module
+from conftest import GOLDEN_DIR, REPO, golden, golden_cases, load_pkg_module, maxreps_cases
 
 sys.path.insert(0, GOLDEN_DIR)
 import refdump  # noqa: E402  (tagged-array reader for ref_dump output)
@@ -83,6 +83,28 @@ def test_hubness_selection_bridges_from_reference_graph(name):
     # the build's own parameters: bridges and repair exactly as in the file
     st = g.build_bridges(chip.build_params(**{k: v for k, v in BUILD_FLAGS.get(name, {}).items()
                                               if k != "brute_force_limit"}))
+    u, v, w, rc = g.get_bridges()
+    ru, rv, rw = f.bridge_uvw()
+    assert np.array_equal(u, ru) and np.array_equal(v, rv)
+    assert np.array_equal(w.view(np.uint32), rw.view(np.uint32))
+    assert rc == f.repair_count and st["repair_edges"] == f.repair_count
+    g.close()
+
+
+@pytest.mark.parametrize("case", maxreps_cases(), ids=lambda c: c[0])
+def test_repair_cut_above_max_reps_bitexact(case):
+    """More components than max_reps, tied scores: the device build keeps the
+    representatives the reference's partial_sort keeps (coresg.cpp:221-227),
+    so its bridges and repair count equal the reference-written file's."""
+    tag, x, max_reps, raw = case
+    print(f"[gpu] repair cut above max_reps: {tag}", flush=True)
+    f = coresg_io.loads(raw.tobytes())
+    g = chip.Graph(x.shape[0], x.shape[1], 5)
+    g.set_points(x)
+    st = g.build(chip.build_params(brute_force_limit=100000, min_per_component=0,
+                                   max_reps=max_reps))
+    ids, dist, flags, count = g.get_knn()
+    assert np.array_equal(ids, f.ids) and np.array_equal(count, f.count)
     u, v, w, rc = g.get_bridges()
     ru, rv, rw = f.bridge_uvw()
     assert np.array_equal(u, ru) and np.array_equal(v, rv)

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import golden, golden_cases, load_pkg_module
+from conftest import golden, golden_cases, load_pkg_module, maxreps_cases
 
 coresg_io = load_pkg_module("coresg_io")
 CASES = golden_cases()
@@ -69,6 +69,24 @@ def test_build_matches_reference_file(name):
     assert np.array_equal(bw.view(np.uint32), rw.view(np.uint32))
     assert sg.repair_count() == ref.repair_count
     assert np.array_equal(sg.points(), ref.points)
+
+
+@pytest.mark.parametrize("case", maxreps_cases(), ids=lambda c: c[0])
+def test_repair_cut_above_max_reps_matches_reference_file(case):
+    """More components than max_reps with tied scores: the repair keeps the
+    representatives the reference's partial_sort keeps (coresg.cpp:221-227),
+    so bridges, repair count and the whole file are the reference's. A stable
+    sort in the cut's place fails every case here."""
+    tag, x, max_reps, raw = case
+    ref = coresg_io.loads(raw.tobytes())
+    assert ref.repair_count > 0
+    sg = oracle.CoreSG.build(x, k_max=5, brute_force_limit=100000, min_per_component=0,
+                             max_reps=max_reps)
+    bu, bv, bw = sg.bridges()
+    ru, rv, rw = ref.bridge_uvw()
+    assert np.array_equal(bu, ru) and np.array_equal(bv, rv)
+    assert np.array_equal(bw.view(np.uint32), rw.view(np.uint32))
+    assert sg.repair_count() == ref.repair_count
 
 
 @pytest.mark.parametrize("name", CASES)
