@@ -98,6 +98,21 @@ int coresg_hip_device_memory(int device, size_t* free_bytes, size_t* total_bytes
  * operands. */
 void coresg_hip_release_thread_cache(void);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc): coresg_hip_mrd_block's result, with host operands staged
+ * through one fixed 32 MiB array per GPU that the library holds until the
+ * process ends, instead of device buffers it allocates. Points (m * stride),
+ * core (m) and the rows ((r1 - r0) * m), each rounded up to 64 floats, must
+ * fit coresg_hip_mrd_block_noalloc_capacity() floats; larger calls fail with a
+ * message naming the limit. Calls on one GPU are serialised by a host lock
+ * (they share the array); the null stream is used. */
+int coresg_hip_mrd_block_noalloc(int device, const float* host_pts, size_t m, uint32_t dim,
+                                 uint32_t stride, int metric, const float* host_core, size_t r0,
+                                 size_t r1, float* host_out);
+/* Floats of the fixed array behind coresg_hip_mrd_block_noalloc. */
+size_t coresg_hip_mrd_block_noalloc_capacity(void);
+
 /* ---- 2. device-resident CORE-SG ------------------------------------------ */
 
 typedef struct coresg_hip_graph coresg_hip_graph;

@@ -93,6 +93,10 @@ SIGNATURES = [
                                        C.POINTER(C.c_uint64)]),
     ("coresg_hip_mrd_block", C.c_int, [C.c_int, _vp, C.c_size_t, C.c_uint32, C.c_uint32,
                                        C.c_int, _vp, C.c_size_t, C.c_size_t, _vp]),
+    ("coresg_hip_mrd_block_noalloc", C.c_int, [C.c_int, _vp, C.c_size_t, C.c_uint32,
+                                               C.c_uint32, C.c_int, _vp, C.c_size_t, C.c_size_t,
+                                               _vp]),
+    ("coresg_hip_mrd_block_noalloc_capacity", C.c_size_t, []),
     ("coresg_hip_mrd_row_min", C.c_int, [C.c_int, _vp, C.c_size_t, C.c_uint32, C.c_uint32,
                                          C.c_int, _vp, _vp, C.c_size_t, C.c_size_t, _vp, _vp]),
     ("coresg_hip_enable_peer_access", C.c_int, [_vp, C.c_size_t]),
@@ -434,6 +438,19 @@ def mrd_block(pts_padded: np.ndarray, dim: int, core: np.ndarray, r0: int = 0,
     core = _f32(core)
     check(lib().coresg_hip_mrd_block(device, ptr(pts), m, dim, stride, 0, ptr(core), r0, r1,
                                      ptr(out)))
+    return out.reshape(r1 - r0, m)
+
+
+def mrd_block_noalloc(pts_padded: np.ndarray, dim: int, core: np.ndarray, r0: int = 0,
+                      r1: int | None = None, device: int = 0) -> np.ndarray:
+    """mrd_block through the allocation-free entry point (fixed device scratch)."""
+    pts = _f32(pts_padded)
+    m, stride = pts.shape
+    r1 = m if r1 is None else r1
+    out = np.empty((r1 - r0) * m, np.float32)
+    core = _f32(core)
+    check(lib().coresg_hip_mrd_block_noalloc(device, ptr(pts), m, dim, stride, 0, ptr(core), r0,
+                                             r1, ptr(out)))
     return out.reshape(r1 - r0, m)
 
 
