@@ -9,12 +9,14 @@ n = 1,000,000 points, d = 24, a 32-component Gaussian mixture (std 0.2-3.0,
 mpts 1..32.
 
 Prints one JSON line. `value` is the build rate in points/s; the sweep rate
-(MSTs/s), the same sweep as one call per mpts (the plain form), and the
-latency percentiles of single-mpts extract queries sit beside it, with the
-dominant kernel's roofline fraction (device time from HIP events on the
-library's stream) and the reference CPU path timed on this host.
+(MSTs/s), the same sweep as one call per mpts value, the latency percentiles
+of single-mpts extract queries, build + a 16-value sweep in the plain
+one-launch-per-step form against the fused form, the dominant kernel's
+roofline fraction (device time from HIP events on the library's stream) and
+the reference CPU path on the same 1M points, timed on this host, sit beside
+it.
 
-    python bench.py [--steps K] [--warmup W] [--n N] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--n N] [--no-cpu-baseline] [--cpu-threads T]
 """
 from __future__ import annotations
 
@@ -68,35 +70,48 @@ def measured_traffic(kernel):
         return None
 
 
-def cpu_baseline(x_full, k, sample_n, mpts, threads):
-    """The reference CPU path (oracle/_ref/coresg_cli, built from the
-    reference sources) on a bounded sample of the same workload."""
+def cpu_baseline(x, k, extract_mpts, threads, recall_queries):
+    """The reference CPU path (oracle/_ref/coresg_cli, built from the reference
+    sources) on the whole workload: the build with its sampled recall on the
+    same queries as the device's (sampled_recall, seed 42), then extract on one
+    mpts value and on `extract_mpts`; the difference of the two extract runs is
+    the per-mpts cost (edges(mpts) + MST) without the file load."""
     cli = os.path.join(REPO, "oracle", "_ref", "coresg_cli")
     if not os.path.exists(cli):
         return None
-    x = x_full[:sample_n]
-    d = x.shape[1]
+    n, d = x.shape
     with tempfile.TemporaryDirectory() as tmp:
-        data = os.path.join(tmp, "sample.bin")
+        data = os.path.join(tmp, "x.bin")
         x.tofile(data)
-        sg = os.path.join(tmp, "sample.coresg")
+        sg = os.path.join(tmp, "x.coresg")
         t0 = time.perf_counter()
         out = subprocess.run([cli, "build", "--data", data, "--dim", str(d), "--kmax", str(k),
                               "--threads", str(threads), "--instances", "1", "--gpu", "0",
-                              "--sample", "16", "--iters", "10", "--verbose", "0", "--out", sg],
+                              "--sample", "16", "--iters", "10", "--verbose", "0",
+                              "--recall-sample", str(recall_queries), "--out", sg],
                              check=True, capture_output=True, text=True).stdout
         build_wall = time.perf_counter() - t0
         m = re.search(r"total ([0-9.]+)", out)
         build_s = float(m.group(1)) if m else build_wall
-        t0 = time.perf_counter()
-        out2 = subprocess.run([cli, "extract", "--in", sg, "--threads", str(threads), "--mpts",
-                               ",".join(str(v) for v in mpts), "--verbose", "0"],
-                              check=True, capture_output=True, text=True).stdout
-        extract_wall = time.perf_counter() - t0
-        mst_s = sum(float(v) for v in re.findall(r"\(([0-9.]+)s\)", out2))
-    return dict(build_s=build_s, build_pts_per_s=sample_n / build_s,
-                extract_wall_s=extract_wall, mst_only_s=mst_s,
-                msts_per_s=len(mpts) / extract_wall)
+        m = re.search(r"recall\s*:\s*([0-9.]+) over", out)
+        recall = float(m.group(1)) if m else None
+        print(f"[bench] cpu baseline: build {build_s:.2f} s on {threads} threads, recall {recall}",
+              file=sys.stderr, flush=True)
+
+        def extract(mp):
+            t = time.perf_counter()
+            o = subprocess.run([cli, "extract", "--in", sg, "--threads", str(threads), "--mpts",
+                                ",".join(str(v) for v in mp), "--verbose", "0"],
+                               check=True, capture_output=True, text=True).stdout
+            return time.perf_counter() - t, sum(float(v) for v in re.findall(r"\(([0-9.]+)s\)", o))
+
+        one_wall, _ = extract(extract_mpts[-1:])
+        many_wall, mst_only = extract(extract_mpts)
+    per_mpts = (many_wall - one_wall) / (len(extract_mpts) - 1)
+    return dict(build_s=build_s, build_pts_per_s=n / build_s, recall_sampled=recall,
+                extract_per_mpts_s=per_mpts, extract_load_s=one_wall - per_mpts,
+                mst_only_per_mpts_s=mst_only / len(extract_mpts),
+                msts_per_s=1.0 / per_mpts, extract_mpts=list(extract_mpts))
 
 
 def serving_run(chip, datagen, k, seed_n, batches, insert_rate, extract_rate, budget, seed=3):
@@ -363,7 +378,8 @@ def main():
     ap.add_argument("--n", type=int, default=None, help="override the point count")
     ap.add_argument("--mpts-max", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="host threads of the reference CPU baseline (the box's CPU share)")
     ap.add_argument("--recall-queries", type=int, default=300)
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched uneven instances (256 mixtures, d in {8,16,32,64})")
@@ -409,7 +425,7 @@ def main():
 
     def single_queries():
         """extract(mpts) latency of one-mpts queries against the resident
-        CORE-SG, and the plain form of the sweep (one call per mpts)."""
+        CORE-SG, and the sweep as one call per mpts value."""
         lat = []
         t0 = time.perf_counter()
         for m in mpts:
@@ -489,6 +505,36 @@ def main():
 
     recall, ratio = g.sampled_recall(args.recall_queries, 42)
 
+    # Fused against plain on the north-star unit of work: build + a 16-value
+    # mpts sweep (2, 4, ..., 32). Plain = the same kernels one launch per step
+    # with a host synchronisation after each, the descent's convergence read
+    # between rounds, one mpts value per sweep pass with host-driven Boruvka
+    # rounds and no captured graphs.
+    sweep16 = [m for m in range(2, k + 1, 2)][:16]
+
+    def build_and_sweep16():
+        t0 = time.perf_counter()
+        g.build(params)
+        t1 = time.perf_counter()
+        g.prepare_edges()
+        res16 = g.extract(sweep16)
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1, [r["weight"] for r in res16]
+
+    g.set_plain(True)
+    build_and_sweep16()  # warm the plain path
+    pb, ps, pw = build_and_sweep16()
+    g.set_plain(False)
+    fb, fs, fw = build_and_sweep16()
+    plain_vs_fused = dict(
+        unit="CORE-SG build + 16-value mpts sweep (2, 4, ..., 32)",
+        plain_build_ms=pb * 1e3, plain_sweep16_ms=ps * 1e3, plain_total_ms=(pb + ps) * 1e3,
+        fused_build_ms=fb * 1e3, fused_sweep16_ms=fs * 1e3, fused_total_ms=(fb + fs) * 1e3,
+        fused_over_plain=(pb + ps) / (fb + fs),
+        sweep16_fused_over_plain=ps / fs)
+    print(f"[bench] plain build+sweep16 {(pb + ps) * 1e3:.1f} ms, fused {(fb + fs) * 1e3:.1f} ms",
+          file=sys.stderr, flush=True)
+
     serving = None
     if not args.no_serving:
         g.close()
@@ -512,16 +558,27 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        sample_n = min(args.cpu_sample, n)
-        cb = cpu_baseline(x, k, sample_n, [1, 2, 4, 8, 16, 32][: len(mpts)], threads)
+        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        ext = [2, 8, 16, 24, 32] if k >= 32 else sorted({2, max(2, k // 2), k})
+        cb = cpu_baseline(x, k, ext, threads, args.recall_queries)
         if cb:
+            cpu_total16 = cb["build_s"] + 16 * cb["extract_per_mpts_s"]
+            gpu_total16 = plain_vs_fused["fused_total_ms"] / 1e3
             cpu = dict(value=cb["build_pts_per_s"], unit="pts/s", cores=threads,
                        kind="reference",
-                       sample=f"first {sample_n} points of {workload}; reference coresg_cli "
-                              f"build --threads {threads} --instances 1 --sample 16 --iters 10",
-                       build_s=cb["build_s"], msts_per_s=cb["msts_per_s"],
-                       extract_mpts="1,2,4,8,16,32")
+                       sample=f"the whole {workload} workload (n={n}); reference coresg_cli build "
+                              f"--threads {threads} --instances 1 --sample 16 --iters 10 "
+                              f"--recall-sample {args.recall_queries}, then extract of mpts "
+                              f"{ext[-1]} and of {','.join(map(str, ext))}; per-mpts cost = "
+                              f"difference of the two extract runs over {len(ext) - 1} values",
+                       build_s=cb["build_s"], recall_sampled=cb["recall_sampled"],
+                       extract_per_mpts_s=cb["extract_per_mpts_s"],
+                       extract_load_s=cb["extract_load_s"],
+                       mst_only_per_mpts_s=cb["mst_only_per_mpts_s"],
+                       msts_per_s=cb["msts_per_s"],
+                       build_plus_sweep16_s=cpu_total16,
+                       fused_gpu_over_cpu_build_plus_sweep16=cpu_total16 / gpu_total16,
+                       fused_gpu_over_cpu_build=(n / build_s) / cb["build_pts_per_s"])
 
     out = {
         "metric": "CORE-SG build pts/s + mpts-sweep MSTs/s at n=1M; p50/p99 extract latency",
@@ -541,8 +598,9 @@ def main():
         "build_ms": build_s * 1e3,
         "sweep_ms": sweep_s * 1e3,
         "sweep_msts_per_s": len(mpts) / sweep_s,
-        "sweep_plain_ms": plain_s * 1e3,
-        "sweep_fused_over_plain": plain_s / sweep_s,
+        "sweep_one_call_per_mpts_ms": plain_s * 1e3,
+        "sweep_batched_over_one_call_per_mpts": plain_s / sweep_s,
+        "plain_vs_fused": plain_vs_fused,
         "extract_p50_ms": percentile(lat, 50),
         "extract_p99_ms": percentile(lat, 99),
         "knn_rounds": st["rounds"],

@@ -346,6 +346,16 @@ int coresg_hip_extract_instances(coresg_hip_graph* g, const uint32_t* mpts, uint
 int coresg_hip_sampled_recall(coresg_hip_graph* g, size_t queries, uint64_t seed,
                               double* recall, double* distance_ratio);
 
+/* The plain one-launch-per-step form of every graph operation, the shape of
+ * the reference's CUDA driver loop (nndescent_kernels.cu:407-418: launch,
+ * synchronise, read back, decide): with on != 0 every kernel launch is
+ * followed by a host synchronisation, the descent's convergence is read after
+ * each round before the next is enqueued, and the sweep runs one mpts value
+ * per pass with each Boruvka round launched and checked from the host (no
+ * captured graphs). Results are identical to the default (fused) form; only
+ * the scheduling differs. Off by default. */
+int coresg_hip_graph_set_plain(coresg_hip_graph* g, int on);
+
 /* Per-kernel device timing with HIP events recorded on the graph's own
  * stream around each launch of the named kernels (off by default). */
 int coresg_hip_graph_set_profiling(coresg_hip_graph* g, int on);

@@ -128,6 +128,7 @@ SIGNATURES = [
                                        _vp]),
     ("coresg_hip_sampled_recall", C.c_int, [_vp, C.c_size_t, C.c_uint64,
                                             C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ("coresg_hip_graph_set_plain", C.c_int, [_vp, C.c_int]),
     ("coresg_hip_graph_set_profiling", C.c_int, [_vp, C.c_int]),
     ("coresg_hip_graph_kernel_times", C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double),
                                                 C.POINTER(C.c_uint64)]),
@@ -388,6 +389,11 @@ class Graph:
         return r.value, ratio.value
 
     # ---- profiling ------------------------------------------------------------
+    def set_plain(self, on: bool):
+        """The plain one-launch-per-step form (host sync after every launch,
+        one mpts per sweep pass, no captured graphs); same results."""
+        check(lib().coresg_hip_graph_set_plain(self.h, int(on)))
+
     def set_profiling(self, on: bool):
         check(lib().coresg_hip_graph_set_profiling(self.h, int(on)))
 

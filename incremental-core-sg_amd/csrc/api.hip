@@ -84,7 +84,12 @@ using namespace coresg_hip;
 
 namespace {
 
-void bind(const Graph& g) { CSG_CHECK(hipSetDevice(g.device)); }
+// The graph's device current; in the plain form, launch checks of this entry
+// point also wait for the graph's stream (reset when the entry point returns).
+void bind(const Graph& g) {
+    CSG_CHECK(hipSetDevice(g.device));
+    t_plain_stream = g.plain ? g.stream : nullptr;
+}
 
 }  // namespace
 
@@ -332,6 +337,10 @@ int coresg_hip_exact_mst(coresg_hip_graph* h, uint32_t mpts, coresg_hip_mst_info
         bind(h->g);
         exact_mst(h->g, mpts, info, out_u, out_v, out_w, out_labels);
     });
+}
+
+int coresg_hip_graph_set_plain(coresg_hip_graph* h, int on) {
+    return guarded([&] { h->g.plain = on != 0; });
 }
 
 int coresg_hip_graph_set_profiling(coresg_hip_graph* h, int on) {

@@ -35,7 +35,18 @@ struct HipError : std::runtime_error {
         }                                                                                 \
     } while (0)
 
-#define CSG_LAUNCH_CHECK() CSG_CHECK(hipGetLastError())
+// The plain form (one launch per step with a host synchronisation after each,
+// the shape of the reference's CUDA driver loop, nndescent_kernels.cu:407-418):
+// while an entry point serves a graph in that mode, this thread's launch checks
+// also wait for the graph's stream.
+inline thread_local hipStream_t t_plain_stream = nullptr;
+
+#define CSG_LAUNCH_CHECK()                                                 \
+    do {                                                                  \
+        CSG_CHECK(hipGetLastError());                                     \
+        if (::coresg_hip::t_plain_stream)                                 \
+            CSG_CHECK(hipStreamSynchronize(::coresg_hip::t_plain_stream)); \
+    } while (0)
 
 inline unsigned blocks_for(std::size_t total, unsigned threads) {
     std::size_t b = (total + threads - 1) / threads;

@@ -253,6 +253,10 @@ struct Graph {
         return inst_off.empty() ? 1u : static_cast<std::uint32_t>(inst_off.size() - 1);
     }
 
+    // The plain form: every launch followed by a host synchronisation, the
+    // sweep one mpts value per pass with every Boruvka round launched and
+    // checked from the host, the descent's convergence read after each round.
+    bool plain = false;
     // Profiling (audit: FP64 check of the Gram-filter join's distances).
     bool profiling = false;
     bool audit = false;

@@ -24,6 +24,10 @@ const char* get_last_error();
 template <typename F>
 int guarded(F&& f) {
     (void)hipGetLastError();
+    struct PlainReset {
+        PlainReset() { t_plain_stream = nullptr; }
+        ~PlainReset() { t_plain_stream = nullptr; }
+    } plain_reset;
     try {
         f();
         return 1;

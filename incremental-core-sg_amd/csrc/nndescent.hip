@@ -1645,6 +1645,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     };
     std::uint32_t last = 0;
     bool converged = false;
+    const bool plain = gp && gp->plain;
     for (std::uint32_t round = 0; round < iters; ++round) {
         last = round;
         {
@@ -1730,12 +1731,21 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
             CSG_CHECK(hipMemcpyAsync(rb->gram_audit, b.gram_audit.p, 8, hipMemcpyDeviceToHost, s));
         }
         CSG_CHECK(hipEventRecord(b.round_done.e[round & 1u], s));
+        if (plain) {
+            // the plain form: the host reads the round's count before it
+            // enqueues the next round
+            if (consume(round)) {
+                converged = true;
+                break;
+            }
+            continue;
+        }
         if (round > 0 && consume(round - 1)) {
             converged = true;  // this round's kernels returned at once
             break;
         }
     }
-    if (!converged) consume(last);
+    if (!converged && !plain) consume(last);
     // the skipped round's copies land in `readback` too: wait for them
     CSG_CHECK(hipEventSynchronize(b.round_done.e[last & 1u]));
     return res;

@@ -704,10 +704,26 @@ void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_
         launch_round(sw, true, vblk, s);
     }
     round_tail();
+    if (g.plain) {
+        // the plain form: every later round launched on its own and its live
+        // mask read back before the next
+        for (std::uint32_t round = 2;; ++round) {
+            std::uint32_t st[2] = {0, 0};  // error bits, live mask
+            CSG_CHECK(hipMemcpyAsync(st, g.counters.p, 8, hipMemcpyDeviceToHost, s));
+            CSG_CHECK(hipStreamSynchronize(s));
+            if (st[0])
+                throw std::runtime_error("extract: Boruvka state corrupted (error bits " +
+                                         std::to_string(st[0]) + ")");
+            if (st[1] == 0) break;
+            if (round > 64) throw std::runtime_error("extract: Boruvka did not converge");
+            launch_round(sw, false, vblk, s);
+            round_tail();
+        }
+    }
     // later rounds: kGraphRounds of them per replay of a captured graph (the
     // same launches every round; kernels of retired slots return at once), one
     // host read of the live mask per replay
-    {
+    if (!g.plain) {
         std::vector<unsigned char> key(sizeof(Sweep) + 4 * sizeof(unsigned));
         const unsigned dims[4] = {vblk, eblk, B, kGraphRounds};
         std::memcpy(key.data(), &sw, sizeof(Sweep));
@@ -730,7 +746,7 @@ void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_
             g.sweep_key = key;
         }
     }
-    for (std::uint32_t replay = 0;; ++replay) {
+    for (std::uint32_t replay = 0; !g.plain; ++replay) {
         std::uint32_t st[2] = {0, 0};  // error bits, live mask
         CSG_CHECK(hipMemcpyAsync(st, g.counters.p, 8, hipMemcpyDeviceToHost, s));
         CSG_CHECK(hipStreamSynchronize(s));
@@ -875,8 +891,10 @@ void extract_msts(Graph& g, const std::uint32_t* mpts_list, std::uint32_t count,
     auto* info = static_cast<coresg_hip_mst_info*>(info_v);
     prepare_edges(g);
     const std::size_t n = g.n, estride = n ? n - 1 : 0;
-    for (std::uint32_t q0 = 0; q0 < count; q0 += kMaxSlots) {
-        const std::uint32_t B = std::min<std::uint32_t>(kMaxSlots, count - q0);
+    // the plain form sweeps one mpts value per pass
+    const std::uint32_t per_pass = g.plain ? 1u : kMaxSlots;
+    for (std::uint32_t q0 = 0; q0 < count; q0 += per_pass) {
+        const std::uint32_t B = std::min<std::uint32_t>(per_pass, count - q0);
         run_batch(g, mpts_list + q0, B, info + q0, out_u ? out_u + q0 * estride : nullptr,
                   out_v ? out_v + q0 * estride : nullptr, out_w ? out_w + q0 * estride : nullptr,
                   out_labels ? out_labels + q0 * n : nullptr);
