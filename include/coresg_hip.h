@@ -254,6 +254,21 @@ int coresg_hip_extract(coresg_hip_graph* g, const uint32_t* mpts, uint32_t count
 int coresg_hip_exact_mst(coresg_hip_graph* g, uint32_t mpts, coresg_hip_mst_info* info,
                          uint32_t* out_u, uint32_t* out_v, float* out_w, uint32_t* out_labels);
 
+/* CoreSG::save (io.cpp:232-260): the resident graph in the reference's
+ * .coresg format, byte for byte (metric L2, the bridges' missing-edge mode). */
+int coresg_hip_graph_save(coresg_hip_graph* g, const char* path);
+/* CoreSG::load (io.cpp:262-330): a graph on `device` from a .coresg file
+ * (L2 only); the derived hubness state -- components, reverse degrees, scores
+ * and the anti-hub selection -- is recomputed with the default parameters, as
+ * the reference's load does. NULL on failure (coresg_hip_last_error). */
+coresg_hip_graph* coresg_hip_graph_load(int device, const char* path);
+/* write_edges_tsv (io.cpp:131-138): "u\tv\tweight" then one line per edge,
+ * the weight as %.9e. Host-only; no device needed. */
+int coresg_hip_write_edges_tsv(const char* path, const uint32_t* u, const uint32_t* v,
+                               const float* w, size_t m);
+/* write_labels_tsv (io.cpp:140-145): "id\tlabel" then one line per point. */
+int coresg_hip_write_labels_tsv(const char* path, const uint32_t* labels, size_t n);
+
 /* ---- 3. serving session -------------------------------------------------- */
 
 /* One device-resident CORE-SG behind a request queue served by a worker

@@ -129,6 +129,10 @@ SIGNATURES = [
     ("coresg_hip_sampled_recall", C.c_int, [_vp, C.c_size_t, C.c_uint64,
                                             C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("coresg_hip_graph_set_plain", C.c_int, [_vp, C.c_int]),
+    ("coresg_hip_graph_save", C.c_int, [_vp, C.c_char_p]),
+    ("coresg_hip_graph_load", _vp, [C.c_int, C.c_char_p]),
+    ("coresg_hip_write_edges_tsv", C.c_int, [C.c_char_p, _vp, _vp, _vp, C.c_size_t]),
+    ("coresg_hip_write_labels_tsv", C.c_int, [C.c_char_p, _vp, C.c_size_t]),
     ("coresg_hip_graph_set_profiling", C.c_int, [_vp, C.c_int]),
     ("coresg_hip_graph_kernel_times", C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double),
                                                 C.POINTER(C.c_uint64)]),
@@ -389,6 +393,24 @@ class Graph:
         return r.value, ratio.value
 
     # ---- profiling ------------------------------------------------------------
+    @classmethod
+    def load(cls, path: str, device: int = 0) -> "Graph":
+        """CoreSG::load (io.cpp:262-330) of a .coresg file onto the device."""
+        h = lib().coresg_hip_graph_load(device, os.fsencode(path))
+        if not h:
+            raise CoresgHipError(last_error())
+        g = cls.__new__(cls)
+        g.h = C.c_void_p(h)
+        g.n = int(lib().coresg_hip_graph_size(g.h))
+        with open(path, "rb") as f:
+            head = f.read(20)
+        g.k, g.dim, g.stride = (int(x) for x in np.frombuffer(head[8:20], np.uint32))
+        return g
+
+    def save(self, path: str) -> None:
+        """CoreSG::save (io.cpp:232-260) in the reference's .coresg format."""
+        check(lib().coresg_hip_graph_save(self.h, os.fsencode(path)))
+
     def set_plain(self, on: bool):
         """The plain one-launch-per-step form (host sync after every launch,
         one mpts per sweep pass, no captured graphs); same results."""
@@ -445,6 +467,18 @@ def mrd_block(pts_padded: np.ndarray, dim: int, core: np.ndarray, r0: int = 0,
     check(lib().coresg_hip_mrd_block(device, ptr(pts), m, dim, stride, 0, ptr(core), r0, r1,
                                      ptr(out)))
     return out.reshape(r1 - r0, m)
+
+
+def write_edges_tsv(path: str, u, v, w) -> None:
+    """write_edges_tsv (io.cpp:131-138), host-only."""
+    u, v, w = _u32(u), _u32(v), _f32(w)
+    check(lib().coresg_hip_write_edges_tsv(os.fsencode(path), ptr(u), ptr(v), ptr(w), u.size))
+
+
+def write_labels_tsv(path: str, labels) -> None:
+    """write_labels_tsv (io.cpp:140-145), host-only."""
+    lab = _u32(labels)
+    check(lib().coresg_hip_write_labels_tsv(os.fsencode(path), ptr(lab), lab.size))
 
 
 def mrd_block_noalloc(pts_padded: np.ndarray, dim: int, core: np.ndarray, r0: int = 0,

@@ -184,6 +184,18 @@ void repair_connectivity(Graph& g, const coresg_hip_build_params& p, coresg_hip_
 
 std::uint64_t count_knn_edges(Graph& g);  // extract.hip
 
+void derive_hubness(Graph& g, const coresg_hip_build_params& p) {
+    if (!g.knn_set || !g.points_set)
+        throw std::runtime_error("derive_hubness: points and k-NN lists must be set");
+    std::uint32_t tagged = 0;
+    hubness_compute(g, &tagged);
+    if (g.inst_off.empty()) {
+        const std::uint64_t knn_edges = count_knn_edges(g);
+        g.selected = select_antihubs(g, antihub_budget(p, g.dim, knn_edges, g.n),
+                                     p.min_per_component);
+    }
+}
+
 void build_bridges(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_stats* st) {
     if (!g.knn_set || !g.points_set)
         throw std::runtime_error("build_bridges: points and k-NN lists must be set");
@@ -197,6 +209,7 @@ void build_bridges(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_
     std::uint64_t knn_edges = 0;
     double bridge_s = 0.0;
     if (p.missing_mode > 2) throw std::runtime_error("build_bridges: unknown missing_mode");
+    g.missing_mode = p.missing_mode;
     if (p.missing_mode != 0 && !g.inst_off.empty())
         throw std::runtime_error("build_bridges: missing_mode 1/2 needs a single-instance graph");
     if (g.inst_off.empty()) {

@@ -9,6 +9,9 @@ namespace coresg_hip {
 
 void build_knn(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_stats* st);
 void build_bridges(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_stats* st);
+// The derived state CoreSG::load recomputes (io.cpp:316-328): components,
+// reverse degrees, scores and the anti-hub selection, bridges untouched.
+void derive_hubness(Graph& g, const coresg_hip_build_params& p);
 // CoreSG::insert (coresg.cpp:325-389): append rows, exact k-NN of the batch
 // against every point with the batch offered to the existing lists, then
 // hubness, selection, bridges and repair over the grown graph.

@@ -194,6 +194,7 @@ struct Graph {
     std::vector<std::uint32_t> bu, bv;
     std::vector<float> bw;
     std::size_t repair_count = 0;
+    std::uint32_t missing_mode = 0;  // MissingEdgeMode of the bridges (coresg.hpp:54-58)
 
     // Ownership of every slot (knn_graph.cpp:143-166), as computed for the
     // build's edge count; valid while the k-NN lists are unchanged, and reused

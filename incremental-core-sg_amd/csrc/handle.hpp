@@ -43,4 +43,19 @@ int guarded(F&& f) {
     return 0;
 }
 
+// guarded() for host-only entry points (no HIP call at all, so they also run
+// on a machine without a GPU).
+template <typename F>
+int guarded_host(F&& f) {
+    try {
+        f();
+        return 1;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed (out of memory)");
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+    }
+    return 0;
+}
+
 }  // namespace coresg_hip

@@ -35,10 +35,11 @@ def load_pkg_module(name: str):
 
 def golden_cases():
     """Per-input fixture files; exact_mst.npz holds extra outputs for several of
-    them and maxreps.npz its own set of builds (the repair's representative cut)."""
+    them, maxreps.npz its own set of builds (the repair's representative cut) and
+    tsv.npz the reference CLI's MST output files."""
     names = (os.path.splitext(os.path.basename(p))[0]
              for p in glob.glob(os.path.join(GOLDEN_DIR, "*.npz")))
-    return sorted(x for x in names if x not in ("exact_mst", "maxreps"))
+    return sorted(x for x in names if x not in ("exact_mst", "maxreps", "tsv"))
 
 
 def maxreps_cases():

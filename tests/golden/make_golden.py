@@ -6,7 +6,7 @@ deterministic serial one) on small seeded inputs, and stores inputs plus the
 reference's outputs as compressed .npz data fixtures next to this script. The
 reference is only needed to *write* fixtures; tests read the committed files.
 
-    python tests/golden/make_golden.py [--exact-only | --maxreps-only]
+    python tests/golden/make_golden.py [--exact-only | --maxreps-only | --tsv-only]
 """
 from __future__ import annotations
 
@@ -207,17 +207,43 @@ def make_maxreps(tmp):
     np.savez_compressed(os.path.join(HERE, "maxreps.npz"), **out)
 
 
+TSV_CASES = [("mix_d16", 4), ("ties", 3), ("islands_repair", 5)]
+
+
+def make_tsv(tmp):
+    """The reference CLI's MST output files (extract --out-prefix, written by
+    write_edges_tsv, io.cpp:131-138) for some cases, from the .coresg bytes
+    already stored in each case's fixture."""
+    out = {}
+    for name, m in TSV_CASES:
+        with np.load(os.path.join(HERE, f"{name}.npz")) as z:
+            raw = z["coresg_bytes"]
+        sg_path = os.path.join(tmp, f"{name}.coresg")
+        raw.tofile(sg_path)
+        prefix = os.path.join(tmp, name)
+        run([os.path.join(REF, "coresg_cli"), "extract", "--in", sg_path, "--threads", "1",
+             "--mpts", str(m), "--out-prefix", prefix])
+        out[f"{name}_mpts{m}_tsv"] = np.frombuffer(open(f"{prefix}_mpts{m}.tsv", "rb").read(),
+                                                   np.uint8)
+        print(f"edges tsv: {name} mpts={m}")
+    np.savez_compressed(os.path.join(HERE, "tsv.npz"), **out)
+
+
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle"), "ref"], check=True)
     with tempfile.TemporaryDirectory() as tmp:
         if "--maxreps-only" in sys.argv:
             make_maxreps(tmp)
             return
+        if "--tsv-only" in sys.argv:
+            make_tsv(tmp)
+            return
         if "--exact-only" not in sys.argv:
             for name, spec in CASES.items():
                 make_case(name, spec, tmp)
         make_exact(tmp)
         make_maxreps(tmp)
+        make_tsv(tmp)
 
 
 if __name__ == "__main__":
