@@ -77,6 +77,14 @@ const char* coresg_hip_last_error(void);
 int coresg_hip_nndescent(int device, const coresg_hip_nnd_config* cfg, const float* host_vectors,
                          uint32_t* host_ids, float* host_dists, uint8_t* host_flags,
                          uint32_t* host_counts, uint64_t* out_updates);
+/* A device-memory budget for coresg_hip_nndescent on `device` (0 = none):
+ * every device allocation the call makes is checked against it first, and a
+ * call that would exceed it fails with an "out of memory" message, which the
+ * Backend treats as the reference treats a device OOM (record it, fall back
+ * to the host unless oom_fallback is off; cuda_backend.cpp:88-101). No
+ * counterpart in cuda_api.hpp; the reference plans device memory per instance
+ * (resource.cpp:72-84). */
+int coresg_hip_set_device_budget(int device, size_t bytes);
 /* cuda_api.hpp:68-70 (mrd_block_gpu): rows [r0,r1) of the m x m
  * mutual-reachability matrix into host_out ((r1-r0)*m floats). */
 int coresg_hip_mrd_block(int device, const float* host_pts, size_t m, uint32_t dim,

@@ -91,6 +91,7 @@ SIGNATURES = [
     ("coresg_hip_get_device_props", C.c_int, [C.c_int, C.POINTER(DeviceProps)]),
     ("coresg_hip_nndescent", C.c_int, [C.c_int, C.POINTER(NndConfig), _vp, _vp, _vp, _vp, _vp,
                                        C.POINTER(C.c_uint64)]),
+    ("coresg_hip_set_device_budget", C.c_int, [C.c_int, C.c_size_t]),
     ("coresg_hip_mrd_block", C.c_int, [C.c_int, _vp, C.c_size_t, C.c_uint32, C.c_uint32,
                                        C.c_int, _vp, C.c_size_t, C.c_size_t, _vp]),
     ("coresg_hip_mrd_block_noalloc", C.c_int, [C.c_int, _vp, C.c_size_t, C.c_uint32,
@@ -458,13 +459,13 @@ class Graph:
 # ---- reference-shaped host-pointer calls -------------------------------------
 
 def mrd_block(pts_padded: np.ndarray, dim: int, core: np.ndarray, r0: int = 0,
-              r1: int | None = None, device: int = 0) -> np.ndarray:
+              r1: int | None = None, device: int = 0, metric: int = 0) -> np.ndarray:
     pts = _f32(pts_padded)
     m, stride = pts.shape
     r1 = m if r1 is None else r1
     out = np.empty((r1 - r0) * m, np.float32)
     core = _f32(core)
-    check(lib().coresg_hip_mrd_block(device, ptr(pts), m, dim, stride, 0, ptr(core), r0, r1,
+    check(lib().coresg_hip_mrd_block(device, ptr(pts), m, dim, stride, metric, ptr(core), r0, r1,
                                      ptr(out)))
     return out.reshape(r1 - r0, m)
 
@@ -482,24 +483,28 @@ def write_labels_tsv(path: str, labels) -> None:
 
 
 def mrd_block_noalloc(pts_padded: np.ndarray, dim: int, core: np.ndarray, r0: int = 0,
-                      r1: int | None = None, device: int = 0) -> np.ndarray:
+                      r1: int | None = None, device: int = 0, metric: int = 0) -> np.ndarray:
     """mrd_block through the allocation-free entry point (fixed device scratch)."""
     pts = _f32(pts_padded)
     m, stride = pts.shape
     r1 = m if r1 is None else r1
     out = np.empty((r1 - r0) * m, np.float32)
     core = _f32(core)
-    check(lib().coresg_hip_mrd_block_noalloc(device, ptr(pts), m, dim, stride, 0, ptr(core), r0,
-                                             r1, ptr(out)))
+    check(lib().coresg_hip_mrd_block_noalloc(device, ptr(pts), m, dim, stride, metric, ptr(core),
+                                             r0, r1, ptr(out)))
     return out.reshape(r1 - r0, m)
 
 
+METRIC_L2, METRIC_COSINE = 0, 1
+
+
 def nndescent(vectors_padded: np.ndarray, dim: int, k: int, sample: int = 16, iters: int = 10,
-              seed: int = 42, delta: float = 0.001, graph=None, device: int = 0):
+              seed: int = 42, delta: float = 0.001, graph=None, device: int = 0,
+              metric: int = METRIC_L2):
     vec = _f32(vectors_padded)
     n, stride = vec.shape
     cfg = NndConfig(n=n, dim=dim, stride=stride, k=k, sample=sample, iters=iters, seed=seed,
-                    metric=0, delta=delta, random_fill=1)
+                    metric=metric, delta=delta, random_fill=1)
     if graph is None:
         ids = np.full(n * k, 0xFFFFFFFF, np.uint32)
         dist = np.full(n * k, np.inf, np.float32)

@@ -1,6 +1,7 @@
 // C ABI of the library: error convention, device queries, the resident graph.
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -86,6 +87,23 @@ namespace {
 
 // The graph's device current; in the plain form, launch checks of this entry
 // point also wait for the graph's stream (reset when the entry point returns).
+// Per-device budgets of the reference-shaped calls (coresg_hip_set_device_budget).
+constexpr int kMaxBudgetDevices = 64;
+std::mutex g_budget_mu;
+std::size_t g_budget[kMaxBudgetDevices] = {};
+
+// The calling thread's allocations are charged to `device`'s budget while in
+// scope.
+struct BudgetScope {
+    AllocBudget b;
+    explicit BudgetScope(int device) {
+        std::lock_guard<std::mutex> lk(g_budget_mu);
+        b.limit = (device >= 0 && device < kMaxBudgetDevices) ? g_budget[device] : 0;
+        t_alloc_budget = b.limit ? &b : nullptr;
+    }
+    ~BudgetScope() { t_alloc_budget = nullptr; }
+};
+
 void bind(const Graph& g) {
     CSG_CHECK(hipSetDevice(g.device));
     t_plain_stream = g.plain ? g.stream : nullptr;
@@ -504,11 +522,21 @@ int coresg_hip_sampled_recall(coresg_hip_graph* h, size_t queries, uint64_t seed
     });
 }
 
+int coresg_hip_set_device_budget(int device, size_t bytes) {
+    return guarded([&] {
+        if (device < 0 || device >= kMaxBudgetDevices)
+            throw std::runtime_error("set_device_budget: device id out of range");
+        std::lock_guard<std::mutex> lk(g_budget_mu);
+        g_budget[device] = bytes;
+    });
+}
+
 int coresg_hip_nndescent(int device, const coresg_hip_nnd_config* cfg, const float* host_vectors,
                          uint32_t* host_ids, float* host_dists, uint8_t* host_flags,
                          uint32_t* host_counts, uint64_t* out_updates) {
     return guarded([&] {
         CSG_CHECK(hipSetDevice(device));
+        BudgetScope budget(device);
         nndescent_host(device, *cfg, host_vectors, host_ids, host_dists, host_flags, host_counts,
                        out_updates);
     });

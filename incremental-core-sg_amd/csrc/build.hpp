@@ -32,8 +32,10 @@ void mrd_row_min_host(int device, const float* pts, std::size_t m, std::uint32_t
                       float* out_min, std::uint32_t* out_argmin);
 void release_thread_cache();
 // Rows [r0, r0 + rows) of the m x m MRD matrix of device-resident operands,
-// launched on s (mrd.hip).
+// launched on s (mrd.hip); `cosine`: the metric value is r / 2 instead of
+// sqrt(r).
 void launch_mrd(const float* pts, std::uint32_t m, std::uint32_t stride, const float* core,
-                std::uint32_t r0, std::uint32_t rows, float* out, hipStream_t s);
+                std::uint32_t r0, std::uint32_t rows, float* out, hipStream_t s,
+                bool cosine = false);
 
 }  // namespace coresg_hip

@@ -117,6 +117,17 @@ __device__ __forceinline__ float l2sqr_exact(const float* __restrict__ a,
 // given, which this library never uses.
 __device__ __forceinline__ float sqrt_exact(float x) { return __builtin_sqrtf(x); }
 
+// MetricOp<L2> / MetricOp<Cosine> (distance.hpp:196-218): both rank by the
+// squared Euclidean distance; L2 stores sqrt(r), cosine (rows L2-normalised on
+// load, so the cosine distance is exactly r / 2) stores 0.5f * r. to_rank maps
+// a stored value back: v * v and 2.0f * v.
+__device__ __forceinline__ float metric_finalize(float r, bool cosine) {
+    return cosine ? 0.5f * r : sqrt_exact(r);
+}
+__device__ __forceinline__ float metric_to_rank(float v, bool cosine) {
+    return cosine ? 2.0f * v : v * v;
+}
+
 // std::max(a, b) for floats: b if a < b, else a (no NaNs occur here).
 __device__ __forceinline__ float fmax_ref(float a, float b) { return a < b ? b : a; }
 

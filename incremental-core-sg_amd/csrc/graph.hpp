@@ -19,6 +19,28 @@ struct AdjEntry {
     float w;
 };
 
+// A device-memory budget for the allocations one entry point makes on the
+// calling thread (coresg_hip_set_device_budget): every DBuf allocation is
+// checked against it before hipMalloc, and exceeding it fails the way a
+// device out-of-memory does.
+struct AllocBudget {
+    std::size_t limit = 0;  // bytes; 0 = none
+    std::size_t used = 0;
+};
+inline thread_local AllocBudget* t_alloc_budget = nullptr;
+
+inline void budget_charge(std::size_t bytes) {
+    AllocBudget* b = t_alloc_budget;
+    if (!b || !b->limit) return;
+    if (b->used + bytes > b->limit)
+        throw HipError("hipMalloc(" + std::to_string(bytes) +
+                           " B) -> hipErrorOutOfMemory: the device budget of " +
+                           std::to_string(b->limit) + " B is exhausted (" +
+                           std::to_string(b->used) + " B in use)",
+                       hipErrorOutOfMemory);
+    b->used += bytes;
+}
+
 // Owning device allocation (hipMalloc / hipFree) with its element count.
 template <typename T>
 struct DBuf {
@@ -38,6 +60,7 @@ struct DBuf {
         if (count <= n && p) return;
         release();
         const std::size_t bytes = (count ? count : 1) * sizeof(T);
+        budget_charge(bytes);
         CSG_CHECK(hipMalloc(reinterpret_cast<void**>(&p), bytes));
         n = count;
     }
@@ -48,6 +71,7 @@ struct DBuf {
         if (count <= n && p) return;
         const std::size_t cap = std::max(count, n + n / 2);
         T* q = nullptr;
+        budget_charge((cap ? cap : 1) * sizeof(T));
         CSG_CHECK(hipMalloc(reinterpret_cast<void**>(&q), (cap ? cap : 1) * sizeof(T)));
         if (p && keep)
             CSG_CHECK(hipMemcpyAsync(q, p, keep * sizeof(T), hipMemcpyDeviceToDevice, s));

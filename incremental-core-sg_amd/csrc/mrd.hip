@@ -28,7 +28,7 @@ constexpr std::uint32_t kMaxLdsStride = 1024;
 __global__ void __launch_bounds__(256)
     mrd_tile_kernel(const float* __restrict__ pts, std::uint32_t m, std::uint32_t stride,
                     const float* __restrict__ core, std::uint32_t r0, std::uint32_t rows,
-                    float* __restrict__ out) {
+                    float* __restrict__ out, bool cosine) {
     extern __shared__ float lds[];
     float* A = lds;                    // kTile rows
     float* B = lds + kTile * stride;   // kTile rows
@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(256)
     const std::uint32_t gr = r0 + lr;
     float value = 0.0f;
     if (gr != gc) {
-        const float d = sqrt_exact(l2sqr_exact(A + ty * stride, B + tx * stride, stride));
+        const float d = metric_finalize(l2sqr_exact(A + ty * stride, B + tx * stride, stride), cosine);
         value = fmax_ref(d, fmax_ref(core[gr], core[gc]));
     }
     out[std::size_t(lr) * m + gc] = value;
@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256)
     mrd_plain_kernel(const float* __restrict__ pts, std::uint32_t m, std::uint32_t stride,
                      const float* __restrict__ core, std::uint32_t r0, std::uint32_t rows,
-                     float* __restrict__ out) {
+                     float* __restrict__ out, bool cosine) {
     const std::size_t total = std::size_t(rows) * m;
     for (std::size_t t = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
          t += std::size_t(gridDim.x) * blockDim.x) {
@@ -65,8 +65,9 @@ __global__ void __launch_bounds__(256)
         const std::uint32_t gr = r0 + lr;
         float value = 0.0f;
         if (gr != gc) {
-            const float d = sqrt_exact(l2sqr_exact(pts + std::size_t(gr) * stride,
-                                                   pts + std::size_t(gc) * stride, stride));
+            const float d = metric_finalize(l2sqr_exact(pts + std::size_t(gr) * stride,
+                                                        pts + std::size_t(gc) * stride, stride),
+                                            cosine);
             value = fmax_ref(d, fmax_ref(core[gr], core[gc]));
         }
         out[t] = value;
@@ -77,15 +78,15 @@ __global__ void __launch_bounds__(256)
 
 // Rows [r0, r0 + rows) of the m x m MRD matrix of resident operands, on s.
 void launch_mrd(const float* pts, std::uint32_t m, std::uint32_t stride, const float* core,
-                std::uint32_t r0, std::uint32_t rows, float* out, hipStream_t s) {
+                std::uint32_t r0, std::uint32_t rows, float* out, hipStream_t s, bool cosine) {
     if (rows == 0 || m == 0) return;
     if (stride <= kMaxLdsStride) {
         dim3 grid((m + kTile - 1) / kTile, (rows + kTile - 1) / kTile);
         const std::size_t lds = std::size_t(2) * kTile * stride * sizeof(float);
-        mrd_tile_kernel<<<grid, 256, lds, s>>>(pts, m, stride, core, r0, rows, out);
+        mrd_tile_kernel<<<grid, 256, lds, s>>>(pts, m, stride, core, r0, rows, out, cosine);
     } else {
         mrd_plain_kernel<<<capped_blocks(std::size_t(rows) * m, 256), 256, 0, s>>>(
-            pts, m, stride, core, r0, rows, out);
+            pts, m, stride, core, r0, rows, out, cosine);
     }
     CSG_LAUNCH_CHECK();
 }
@@ -309,7 +310,7 @@ __global__ void __launch_bounds__(256)
     row_min_kernel(const float* __restrict__ pts, std::uint32_t m, std::uint32_t stride,
                    const float* __restrict__ core, const std::uint32_t* __restrict__ comp,
                    std::uint32_t r0, std::uint32_t rows, float* __restrict__ out_min,
-                   std::uint32_t* __restrict__ out_arg) {
+                   std::uint32_t* __restrict__ out_arg, bool cosine) {
     __shared__ float sv[4];
     __shared__ std::uint32_t si[4];
     for (std::uint32_t lr = blockIdx.x; lr < rows; lr += gridDim.x) {
@@ -320,7 +321,8 @@ __global__ void __launch_bounds__(256)
         std::uint32_t li = kNone;
         for (std::uint32_t j = threadIdx.x; j < m; j += 256) {
             if (j == gr || (comp && comp[j] == my)) continue;
-            const float d = sqrt_exact(l2sqr_exact(a, pts + std::size_t(j) * stride, stride));
+            const float d =
+                metric_finalize(l2sqr_exact(a, pts + std::size_t(j) * stride, stride), cosine);
             const float w = fmax_ref(d, fmax_ref(core[gr], core[j]));
             if (w < lv || (w == lv && j < li)) {
                 lv = w;
@@ -406,8 +408,8 @@ void ensure_resident(int device, const float* host_pts, std::size_t m, std::uint
 }
 
 void check_metric(int metric) {
-    if (metric != CORESG_HIP_METRIC_L2)
-        throw std::runtime_error("unsupported metric for the device path (L2 only)");
+    if (metric != CORESG_HIP_METRIC_L2 && metric != CORESG_HIP_METRIC_COSINE)
+        throw std::runtime_error("unsupported metric for the device path (L2 and cosine only)");
 }
 
 }  // namespace
@@ -560,7 +562,7 @@ void mrd_block_host(int device, const float* pts, std::size_t m, std::uint32_t d
     t_cache.out.reserve(rows * m);
     launch_mrd(t_cache.pts.p, static_cast<std::uint32_t>(m), stride, t_cache.core.p,
                static_cast<std::uint32_t>(r0), static_cast<std::uint32_t>(rows), t_cache.out.p,
-               t_cache.stream);
+               t_cache.stream, metric == CORESG_HIP_METRIC_COSINE);
     CSG_CHECK(hipMemcpyAsync(out, t_cache.out.p, rows * m * 4, hipMemcpyDeviceToHost,
                              t_cache.stream));
     CSG_CHECK(hipStreamSynchronize(t_cache.stream));
@@ -593,7 +595,8 @@ void mrd_row_min_host(int device, const float* pts, std::size_t m, std::uint32_t
     }
     row_min_kernel<<<static_cast<unsigned>(rows < 65536 ? rows : 65536), 256, 0, s>>>(
         d_pts.p, static_cast<std::uint32_t>(m), stride, d_core.p, component ? d_comp.p : nullptr,
-        static_cast<std::uint32_t>(r0), static_cast<std::uint32_t>(rows), d_min.p, d_arg.p);
+        static_cast<std::uint32_t>(r0), static_cast<std::uint32_t>(rows), d_min.p, d_arg.p,
+        metric == CORESG_HIP_METRIC_COSINE);
     CSG_LAUNCH_CHECK();
     CSG_CHECK(hipMemcpyAsync(out_min, d_min.p, rows * 4, hipMemcpyDeviceToHost, s));
     CSG_CHECK(hipMemcpyAsync(out_argmin, d_arg.p, rows * 4, hipMemcpyDeviceToHost, s));

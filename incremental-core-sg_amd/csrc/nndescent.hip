@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(256)
                        std::uint32_t k, std::uint64_t seed, std::uint32_t* __restrict__ ids,
                        float* __restrict__ dist, std::uint8_t* __restrict__ flags,
                        std::uint32_t* __restrict__ count, const std::uint32_t* __restrict__ inst_of,
-                       const std::uint32_t* __restrict__ inst_off) {
+                       const std::uint32_t* __restrict__ inst_off, bool cosine) {
     __shared__ std::uint32_t s_stage_id[4][64];
     __shared__ float s_stage_d[4][64];
     const std::uint32_t lane = threadIdx.x & 63u, wib = threadIdx.x >> 6;
@@ -137,8 +137,9 @@ __global__ void __launch_bounds__(256)
             const bool take = ok && rank < needed;
             float d = __builtin_inff();
             if (take)
-                d = sqrt_exact(l2sqr_exact(pts + std::size_t(node) * stride,
-                                           pts + std::size_t(cand) * stride, stride));
+                d = metric_finalize(l2sqr_exact(pts + std::size_t(node) * stride,
+                                                pts + std::size_t(cand) * stride, stride),
+                                    cosine);
             // move taken candidates into lanes [accepted, accepted + taken):
             // each writes its slot (its rank) in the wave's staging area
             const std::uint64_t tmask = __ballot(take);
@@ -210,7 +211,8 @@ __global__ void __launch_bounds__(256)
                   std::uint32_t* __restrict__ rev_new_cnt, std::uint32_t* __restrict__ rev_old,
                   std::uint32_t* __restrict__ rev_old_cnt, std::uint32_t* __restrict__ hist,
                   std::uint32_t nb, const float* __restrict__ dist,
-                  float* __restrict__ thr_node, const std::uint32_t* __restrict__ halt) {
+                  float* __restrict__ thr_node, const std::uint32_t* __restrict__ halt,
+                  bool cosine) {
     extern __shared__ std::uint32_t s_hist[];  // nb buckets (BUCKETS only)
     if (*halt) return;
     constexpr std::uint32_t P = 64 / SEG;  // nodes per wavefront
@@ -263,8 +265,7 @@ __global__ void __launch_bounds__(256)
             old_cnt[v] = to < max_old ? to : max_old;
             // the node's join threshold D[k-1]^2 for this round (to_rank of
             // its current k-th distance, nndescent.cpp:41-42)
-            const float t = dist[base + k - 1];
-            thr_node[v] = t * t;
+            thr_node[v] = metric_to_rank(dist[base + k - 1], cosine);
         }
     }
     if (BUCKETS) {
@@ -481,6 +482,7 @@ struct JoinArgs {
     unsigned long long* tally;
     unsigned int* audit;
     const std::uint32_t* halt;  // nonzero: the descent has converged, skip
+    bool cosine;                // stored values are r / 2 instead of sqrt(r)
 };
 
 // Exact squared distance with `a` held in registers (compile-time stride),
@@ -613,7 +615,8 @@ __device__ __forceinline__ void join_emit(const JoinArgs& a, const std::uint32_t
         const float d = s_D[pitch ? __umul24(i, pitch) + j : p];
         const bool to_i = d < s_thr[i], to_j = d < s_thr[j];
         if (!(to_i || to_j)) continue;
-        const unsigned long long hb = static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32;
+        const unsigned long long hb =
+            static_cast<unsigned long long>(fbits(metric_finalize(d, a.cosine))) << 32;
         if (to_i) {
             const std::uint32_t pos = s_base[i] + atomicAdd(&s_cnt[i], 1u);
             if (pos < a.cap) a.cand[std::size_t(s_id[i]) * a.cap + pos] = hb | s_id[j];
@@ -779,7 +782,7 @@ __global__ void __launch_bounds__(kJoinThreads) join_reg_kernel(JoinArgs a) {
                 if (wi || wj) {
                     const float d = l2sqr_reg<STRIDE>(rj, s_rows + i * STRIDE);
                     const unsigned long long hb =
-                        static_cast<unsigned long long>(fbits(sqrt_exact(d))) << 32;
+                        static_cast<unsigned long long>(fbits(metric_finalize(d, a.cosine))) << 32;
                     if (wi) {
                         const std::uint32_t pos = s_base[i] + atomicAdd(&s_cnt[i], 1u);
                         if (pos < a.cap) a.cand[std::size_t(s_id[i]) * a.cap + pos] = hb | id_j;
@@ -1539,7 +1542,8 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
                           std::uint32_t k, std::uint32_t* ids, float* dist, std::uint8_t* flags,
                           std::uint32_t* count, std::uint32_t smp, std::uint32_t iters,
                           double delta, std::uint64_t seed, bool random_fill,
-                          std::uint32_t cap, std::uint32_t gram, hipStream_t s) {
+                          std::uint32_t cap, std::uint32_t gram, hipStream_t s,
+                          bool cosine = false) {
     DescentResult res;
     if (n == 0) return res;
     if (k > kMaxK) throw std::runtime_error("nndescent: k above 64 is not supported on device");
@@ -1552,7 +1556,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         const bool inst = gp && !gp->inst_off.empty();
         init_random_kernel<<<wave_blocks, 256, 0, s>>>(pts, n, stride, k, seed, ids, dist, flags,
                                                        count, inst ? gp->inst_of.p : nullptr,
-                                                       inst ? gp->inst_off_d.p : nullptr);
+                                                       inst ? gp->inst_off_d.p : nullptr, cosine);
         CSG_LAUNCH_CHECK();
     }
     if (iters == 0) return res;
@@ -1618,7 +1622,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     b.readback.reserve(iters);
     b.round_done.create();
     JoinArgs ja{pts, n, stride, smp, b.gids.p, b.gthr.p, b.gcnt.p, b.cand.p, b.fill.p, cap,
-                nullptr, nullptr, b.halt.p};
+                nullptr, nullptr, b.halt.p, cosine};
     const unsigned merge_blocks = static_cast<unsigned>(std::min<std::size_t>(n, kMergeBlocks));
     const bool tallies = gp && gp->profiling;
     const bool gram_audit = tallies && gram != kGramExact;
@@ -1659,7 +1663,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
                 sample_fn<<<kRevBlocks, 256, hl, s>>>(
                     n, k, ids, flags, count, smp, b.new_list.p, b.new_cnt.p, b.old_list.p,
                     b.old_cnt.p, b.rev_new.p, b.rev_new_cnt.p, b.rev_old.p, b.rev_old_cnt.p,
-                    b.rev_hist.p, nb, dist, b.thr_node.p, b.halt.p);
+                    b.rev_hist.p, nb, dist, b.thr_node.p, b.halt.p, cosine);
                 CSG_LAUNCH_CHECK();
                 exclusive_scan_u32(b.rev_hist.p, std::size_t(nb) * kRevBlocks, b.rev_total.p,
                                    b.rev_tmp.p, s);
@@ -1683,7 +1687,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
                 sample_fn<<<wave_blocks, 256, 0, s>>>(
                     n, k, ids, flags, count, smp, b.new_list.p, b.new_cnt.p, b.old_list.p,
                     b.old_cnt.p, b.rev_new.p, b.rev_new_cnt.p, b.rev_old.p, b.rev_old_cnt.p,
-                    nullptr, 0, dist, b.thr_node.p, b.halt.p);
+                    nullptr, 0, dist, b.thr_node.p, b.halt.p, cosine);
                 CSG_LAUNCH_CHECK();
             }
         }
@@ -2335,8 +2339,9 @@ void nndescent_host(int device, const coresg_hip_nnd_config& cfg, const float* v
                     std::uint64_t* updates) {
     (void)device;
     if (cfg.n == 0 || cfg.k == 0) return;
-    if (cfg.metric != CORESG_HIP_METRIC_L2)
-        throw std::runtime_error("nndescent: unsupported metric for the device path (L2 only)");
+    if (cfg.metric != CORESG_HIP_METRIC_L2 && cfg.metric != CORESG_HIP_METRIC_COSINE)
+        throw std::runtime_error("nndescent: unsupported metric for the device path "
+                                 "(L2 and cosine only)");
     if (cfg.stride % 8 != 0 || cfg.stride < cfg.dim)
         throw std::runtime_error("nndescent: stride must be round_up(dim, 8)");
     hipStream_t s;
@@ -2358,7 +2363,7 @@ void nndescent_host(int device, const coresg_hip_nnd_config& cfg, const float* v
     const DescentResult r =
         run_descent(nullptr, d_pts.p, cfg.n, cfg.stride, cfg.k, d_ids.p, d_dist.p, d_flags.p,
                     d_cnt.p, cfg.sample, cfg.iters, cfg.delta, cfg.seed, cfg.random_fill != 0,
-                    kDefaultCap, kGramAuto, s);
+                    kDefaultCap, kGramAuto, s, cfg.metric == CORESG_HIP_METRIC_COSINE);
     CSG_CHECK(hipMemcpyAsync(ids, d_ids.p, nk * 4, hipMemcpyDeviceToHost, s));
     CSG_CHECK(hipMemcpyAsync(dists, d_dist.p, nk * 4, hipMemcpyDeviceToHost, s));
     CSG_CHECK(hipMemcpyAsync(flags, d_flags.p, nk, hipMemcpyDeviceToHost, s));

@@ -69,8 +69,8 @@ void mrd_block_noalloc_host(int device, const float* pts, std::size_t m, std::ui
     if (device < 0 || device >= kMaxDevices)
         throw std::runtime_error("mrd_block_noalloc: device id out of range");
     if (m == 0 || r1 <= r0) return;
-    if (metric != CORESG_HIP_METRIC_L2)
-        throw std::runtime_error("mrd_block_noalloc: unsupported metric (L2 only)");
+    if (metric != CORESG_HIP_METRIC_L2 && metric != CORESG_HIP_METRIC_COSINE)
+        throw std::runtime_error("mrd_block_noalloc: unsupported metric (L2 and cosine only)");
     if (stride % 8 != 0) throw std::runtime_error("mrd_block_noalloc: stride must be a multiple of 8");
     if (r1 > m) throw std::runtime_error("mrd_block_noalloc: row range exceeds m");
     if (m > 0xFFFFFFFFull) throw std::runtime_error("mrd_block_noalloc: m above 2^32 - 1");
@@ -89,7 +89,8 @@ void mrd_block_noalloc_host(int device, const float* pts, std::size_t m, std::ui
     CSG_CHECK(hipMemcpyAsync(base, pts, m * stride * 4, hipMemcpyHostToDevice, s));
     CSG_CHECK(hipMemcpyAsync(base + o_core, core, m * 4, hipMemcpyHostToDevice, s));
     launch_mrd(base, static_cast<std::uint32_t>(m), stride, base + o_core,
-               static_cast<std::uint32_t>(r0), static_cast<std::uint32_t>(rows), base + o_out, s);
+               static_cast<std::uint32_t>(r0), static_cast<std::uint32_t>(rows), base + o_out, s,
+               metric == CORESG_HIP_METRIC_COSINE);
     CSG_CHECK(hipMemcpyAsync(out, base + o_out, rows * m * 4, hipMemcpyDeviceToHost, s));
     CSG_CHECK(hipStreamSynchronize(s));
 }

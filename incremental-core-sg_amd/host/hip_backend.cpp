@@ -13,9 +13,13 @@
 //     bit-identical to the host backend's (same distance summation order).
 //   * trim: releases the calling thread's resident MRD operands.
 // It also offers DeviceExtract (device_extract.hpp): CoreSG's extraction half
-// (all_msts, exact_mst) on the device, over the CORE-SG the caller holds.
+// (all_msts, exact_mst) on the device, over the CORE-SG the caller holds; the
+// device copy of that CORE-SG (points, lists, bridges, edge list) stays
+// resident across calls while the CoreSG is unchanged.
 #include <algorithm>
+#include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -27,14 +31,15 @@
 namespace coresg {
 namespace {
 
+// Metrics the device kernels cover (cuda_backend.cpp:15-21): L2 and cosine
+// (rows normalised by the host Dataset, so the cosine distance is half the
+// squared Euclidean one); L1 and inner product run on the host.
 bool metric_to_hip(MetricKind m, int* code) {
-    // The device kernels compute the true L2 metric; cosine, L1 and inner
-    // product stay on the host (the reference gates L1/IP the same way).
-    if (m == MetricKind::L2) {
-        *code = CORESG_HIP_METRIC_L2;
-        return true;
+    switch (m) {
+        case MetricKind::L2: *code = CORESG_HIP_METRIC_L2; return true;
+        case MetricKind::Cosine: *code = CORESG_HIP_METRIC_COSINE; return true;
+        default: return false;
     }
-    return false;
 }
 
 void hip_check(int ok, const char* what) {
@@ -169,13 +174,18 @@ public:
         }
     }
 
-    void trim() override { coresg_hip_release_thread_cache(); }
+    void trim() override {
+        coresg_hip_release_thread_cache();
+        std::lock_guard<std::mutex> lk(resident_mu_);
+        resident_.reset();
+    }
 
     std::vector<MstResult> extract_msts(const CoreSG& sg,
                                         const std::vector<std::uint32_t>& mpts) override {
         const std::size_t n = sg.knn().size();
         if (n < 2 || mpts.empty()) return sg.all_msts(mpts);
-        DeviceGraph dg(device_for(0), sg);
+        std::lock_guard<std::mutex> lk(resident_mu_);
+        DeviceGraph& dg = resident(sg);
         const std::size_t q = mpts.size(), n1 = n - 1;
         std::vector<coresg_hip_mst_info> info(q);
         std::vector<std::uint32_t> u(q * n1), v(q * n1), lab(q * n);
@@ -214,10 +224,76 @@ public:
 private:
     int device_for(unsigned slot) const { return ids_[slot < ids_.size() ? slot : 0]; }
 
+    // What identifies the CoreSG a resident device copy was made from: the
+    // object, its sizes and storage, and a digest of the bridges and of a
+    // stride of list entries (an insert changes n; a rebuilt bridge set
+    // changes the digest).
+    struct ResidentKey {
+        const CoreSG* sg = nullptr;
+        std::size_t n = 0, k = 0, dim = 0, bridges = 0, repair = 0;
+        const void *pts = nullptr, *ids = nullptr, *dist = nullptr;
+        std::uint64_t digest = 0;
+        bool operator==(const ResidentKey& o) const {
+            return sg == o.sg && n == o.n && k == o.k && dim == o.dim && bridges == o.bridges &&
+                   repair == o.repair && pts == o.pts && ids == o.ids && dist == o.dist &&
+                   digest == o.digest;
+        }
+    };
+
+    static std::uint64_t mix(std::uint64_t h, std::uint64_t x) {
+        h ^= x + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+        return h;
+    }
+
+    static ResidentKey key_of(const CoreSG& sg) {
+        ResidentKey key;
+        const KnnGraph& knn = sg.knn();
+        key.sg = &sg;
+        key.n = knn.size();
+        key.k = knn.k();
+        key.dim = sg.data().dim();
+        key.bridges = sg.missing_edges().size();
+        key.repair = sg.repair_edge_count();
+        key.pts = sg.data().data();
+        key.ids = knn.ids_storage().data();
+        key.dist = knn.dist_storage().data();
+        std::uint64_t h = 0;
+        for (const Edge& e : sg.missing_edges()) {
+            std::uint32_t wb;
+            std::memcpy(&wb, &e.w, 4);
+            h = mix(h, (std::uint64_t(e.u) << 32) | e.v);
+            h = mix(h, wb);
+        }
+        const std::vector<idx_t>& ids = knn.ids_storage();
+        const std::vector<dist_t>& dist = knn.dist_storage();
+        for (std::size_t i = 0; i < ids.size(); i += 4099) {
+            std::uint32_t db;
+            std::memcpy(&db, &dist[i], 4);
+            h = mix(h, (std::uint64_t(ids[i]) << 32) | db);
+        }
+        key.digest = h;
+        return key;
+    }
+
+    // The device copy of `sg`, made on first use and kept across calls (its
+    // edge list stays prepared) until the CoreSG changes or trim() is called.
+    DeviceGraph& resident(const CoreSG& sg) {
+        const ResidentKey key = key_of(sg);
+        if (!resident_ || !(resident_key_ == key)) {
+            resident_.reset();
+            resident_.reset(new DeviceGraph(device_for(0), sg));
+            resident_key_ = key;
+        }
+        return *resident_;
+    }
+
     std::vector<int> ids_;
     bool oom_fallback_ = true;
     std::vector<DeviceInfo> devices_;
     std::unique_ptr<Backend> fallback_;
+    std::mutex resident_mu_;
+    std::unique_ptr<DeviceGraph> resident_;
+    ResidentKey resident_key_;
 };
 
 }  // namespace

@@ -71,6 +71,11 @@ int main(int argc, char** argv) {
     bool ok = a.size() == b.size();
     for (std::size_t i = 0; ok && i < a.size(); ++i)
         ok = same(a[i], b[i], "mst(" + std::to_string(mpts[i]) + ")");
+    // again, in reverse order: served by the device copy kept from the first call
+    std::vector<std::uint32_t> rev(mpts.rbegin(), mpts.rend());
+    const std::vector<MstResult> c = all_msts_routed(dev, rev);
+    for (std::size_t i = 0; ok && i < a.size(); ++i)
+        ok = same(a[i], c[a.size() - 1 - i], "mst(" + std::to_string(mpts[i]) + ") again");
     for (std::size_t i = 0; with_exact && ok && i < mpts.size(); ++i)
         ok = same(host.exact_mst(mpts[i]), exact_mst_routed(dev, mpts[i]),
                   "exact_mst(" + std::to_string(mpts[i]) + ")");

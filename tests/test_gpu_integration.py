@@ -98,3 +98,22 @@ def test_reference_coresg_routes_exact_mst_to_device(tmp_path):
     r = subprocess.run([EXTRACT_CHECK, "--exact", str(path), "1", "4", "16"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+BACKEND_CHECK = os.path.join(REPO, "oracle", "_ref", "backend_check")
+
+
+@pytest.mark.skipif(not os.path.exists(BACKEND_CHECK), reason="integration build not present")
+@pytest.mark.parametrize("mode", ["small", "oom", "oom-stop", "cosine"])
+def test_reference_coresg_backend_semantics(mode):
+    """The reference's CoreSG::build through HipBackend keeps the CUDA backend's
+    plugin semantics (cuda_backend.cpp:15-133; tests/cpp/backend_check.cpp):
+    instances below 512 points on the host, a device out-of-memory (forced by
+    a device budget) recorded and finished on the host -- or not run with
+    oom_fallback off -- and the cosine metric on the device with the host's
+    distances."""
+    print(f"[gpu] backend semantics: {mode}", flush=True)
+    r = subprocess.run([BACKEND_CHECK, mode], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert f"backend_check {mode}: passed" in r.stdout
