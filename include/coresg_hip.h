@@ -279,16 +279,24 @@ int coresg_hip_write_labels_tsv(const char* path, const uint32_t* labels, size_t
 
 /* ---- 3. serving session -------------------------------------------------- */
 
-/* One device-resident CORE-SG behind a request queue served by a worker
- * thread that owns the device. The reference serves inserts and MST queries
- * as separate CLI runs that reload the CORE-SG each time
- * (coresg_cli.cpp:507-570); here the graph stays in HBM. Requests take effect
- * in submission order: an extract sees exactly the inserts submitted before
- * it. Extract requests waiting together are admitted into one fused sweep
- * (up to 32 mpts values); every mpts is computed independently, so a result
- * is the same alone and in a mix. An insert is admitted only while the
- * graph's projected device footprint stays within hbm_budget_bytes
- * (0 = no limit); otherwise it fails with a message naming the budget. */
+/* One device-resident CORE-SG served by continuous batching. The reference
+ * serves inserts and MST queries as separate CLI runs that reload the CORE-SG
+ * each time (coresg_cli.cpp:507-570); here the graph stays in HBM and two
+ * device workers run concurrently:
+ *   - inserts are applied one at a time in submission order to a working
+ *     graph (CoreSG::insert); each completed insert publishes a new version
+ *     (graph_version counts the inserts applied);
+ *   - extracts run on the latest published version while the next one is
+ *     built; every extract waiting when the extract worker becomes free is
+ *     admitted into the next fused sweep (up to 32 mpts values). Every mpts
+ *     is computed independently, so a result is the same alone and in a mix,
+ *     and equals CoreSG::mst on the version it reports (graph_version,
+ *     graph_n) -- an extract does not wait for inserts submitted before it.
+ * An insert is admitted while the analytic device footprint of the next
+ * version, plus older versions still held by in-flight extracts, stays within
+ * hbm_budget_bytes (0 = no limit); it waits while those older versions are
+ * released and fails, with a message naming the budget, only when it cannot
+ * fit even then. */
 typedef struct coresg_hip_session coresg_hip_session;
 
 enum { CORESG_HIP_REQ_INSERT = 1, CORESG_HIP_REQ_EXTRACT = 2 };
@@ -303,8 +311,8 @@ typedef struct coresg_hip_request_status {
     uint64_t id;
     int kind;
     int state;
-    uint32_t graph_n;       /* points in the graph when the request completed */
-    uint64_t graph_version; /* inserts applied before it completed */
+    uint32_t graph_n;       /* points in the version it produced (insert) or read (extract) */
+    uint64_t graph_version; /* inserts applied in that version */
     uint32_t num_msts;      /* extract: mpts values requested */
     uint32_t batch_requests; /* requests served by the same device pass */
     double queued_ms;       /* submit -> start */
@@ -341,7 +349,10 @@ int coresg_hip_session_release(coresg_hip_session* s, uint64_t id);
 /* The served graph, once the queue has drained (for inspection only; submit
  * nothing while it is in use). */
 coresg_hip_graph* coresg_hip_session_graph(coresg_hip_session* s);
-/* Finish the queued requests, stop the worker, free the graph. */
+/* Device bytes the session holds now: the working graph plus every published
+ * version still alive. */
+size_t coresg_hip_session_device_bytes(coresg_hip_session* s);
+/* Finish the queued requests, stop the workers, free the graph. */
 void coresg_hip_session_destroy(coresg_hip_session* s);
 
 /* ---- 4. batched independent instances ------------------------------------ */

@@ -156,6 +156,7 @@ SIGNATURES = [
     ("coresg_hip_session_extract_result", C.c_int, [_vp, C.c_uint64, _vp, _vp, _vp, _vp, _vp]),
     ("coresg_hip_session_release", C.c_int, [_vp, C.c_uint64]),
     ("coresg_hip_session_graph", _vp, [_vp]),
+    ("coresg_hip_session_device_bytes", C.c_size_t, [_vp]),
     ("coresg_hip_session_destroy", None, [_vp]),
 ]
 
@@ -519,9 +520,10 @@ def nndescent(vectors_padded: np.ndarray, dim: int, k: int, sample: int = 16, it
 
 
 class Session:
-    """A serving session over one device-resident CORE-SG (insert and
-    extract requests, executed in submission order by the library's worker
-    thread; waiting extracts share one fused sweep)."""
+    """A serving session over one device-resident CORE-SG: inserts applied in
+    submission order by one device worker, extracts served concurrently on the
+    latest published version by another (waiting extracts share one fused
+    sweep); every result reports the version it read."""
 
     def __init__(self, rows: np.ndarray, k_max: int, params: BuildParams | None = None,
                  hbm_budget: int = 0, device: int = 0, graph: "Graph | None" = None):
@@ -607,7 +609,7 @@ class Session:
             raise CoresgHipError(last_error())
         gh = C.c_void_p(gh)
         return dict(n=int(lib().coresg_hip_graph_size(gh)),
-                    device_bytes=int(lib().coresg_hip_graph_device_bytes(gh)))
+                    device_bytes=int(lib().coresg_hip_session_device_bytes(self.h)))
 
     def close(self):
         if getattr(self, "h", None):

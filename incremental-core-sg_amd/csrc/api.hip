@@ -22,11 +22,15 @@ const char* get_last_error() { return t_last_error.c_str(); }
 
 Graph::~Graph() {
     resolve_timers();
-    if (sweep_exec) (void)hipGraphExecDestroy(sweep_exec);
     if (stream) {
         (void)hipSetDevice(device);
-        (void)hipStreamDestroy(stream);
+        (void)hipStreamSynchronize(stream);
     }
+    if (sweep_exec) {
+        std::lock_guard<std::mutex> lk(device_api_mutex());
+        (void)hipGraphExecDestroy(sweep_exec);
+    }
+    destroy_stream(stream);
 }
 
 void Graph::ensure_scan_tmp(std::size_t count) { scan_tmp.reserve(scan_tmp_words(count)); }
@@ -189,7 +193,7 @@ coresg_hip_graph* coresg_hip_graph_create(int device, uint32_t n, uint32_t dim, 
         g.stride = (dim + 7u) & ~7u;
         g.k = k_max;
         CSG_CHECK(hipSetDevice(device));
-        CSG_CHECK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+        create_stream(&g.stream);
         g.pts.reserve(std::size_t(n) * g.stride);
         g.ids.reserve(std::size_t(n) * k_max);
         g.dist.reserve(std::size_t(n) * k_max);

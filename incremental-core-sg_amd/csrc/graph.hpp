@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <initializer_list>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -18,6 +19,15 @@ struct AdjEntry {
     std::uint32_t nbr, rank;
     float w;
 };
+
+// Device allocations, frees and stream creation/destruction on one thread and
+// a stream capture on another are kept apart (a serving session builds the
+// next graph version while the previous one is swept, and the sweep captures
+// its Boruvka rounds): the runtime may synchronise the device inside a free.
+inline std::mutex& device_api_mutex() {
+    static std::mutex mu;
+    return mu;
+}
 
 // A device-memory budget for the allocations one entry point makes on the
 // calling thread (coresg_hip_set_device_budget): every DBuf allocation is
@@ -51,7 +61,10 @@ struct DBuf {
     DBuf& operator=(const DBuf&) = delete;
     ~DBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            std::lock_guard<std::mutex> lk(device_api_mutex());
+            (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
     }
@@ -61,8 +74,17 @@ struct DBuf {
         release();
         const std::size_t bytes = (count ? count : 1) * sizeof(T);
         budget_charge(bytes);
+        std::lock_guard<std::mutex> lk(device_api_mutex());
         CSG_CHECK(hipMalloc(reinterpret_cast<void**>(&p), bytes));
         n = count;
+    }
+    // Take over another buffer's allocation (this one's is freed).
+    void adopt(DBuf& o) {
+        release();
+        p = o.p;
+        n = o.n;
+        o.p = nullptr;
+        o.n = 0;
     }
     // Grow to at least `count` elements keeping the first `keep` (copied on
     // `s`); capacity grows by half again at least, so a stream of inserts
@@ -72,7 +94,10 @@ struct DBuf {
         const std::size_t cap = std::max(count, n + n / 2);
         T* q = nullptr;
         budget_charge((cap ? cap : 1) * sizeof(T));
-        CSG_CHECK(hipMalloc(reinterpret_cast<void**>(&q), (cap ? cap : 1) * sizeof(T)));
+        {
+            std::lock_guard<std::mutex> lk(device_api_mutex());
+            CSG_CHECK(hipMalloc(reinterpret_cast<void**>(&q), (cap ? cap : 1) * sizeof(T)));
+        }
         if (p && keep)
             CSG_CHECK(hipMemcpyAsync(q, p, keep * sizeof(T), hipMemcpyDeviceToDevice, s));
         CSG_CHECK(hipStreamSynchronize(s));
@@ -94,7 +119,10 @@ struct HostBuf {
     HostBuf& operator=(const HostBuf&) = delete;
     ~HostBuf() { release(); }
     void release() {
-        if (p) (void)hipHostFree(p);
+        if (p) {
+            std::lock_guard<std::mutex> lk(device_api_mutex());
+            (void)hipHostFree(p);
+        }
         p = nullptr;
         n = 0;
     }
@@ -102,6 +130,7 @@ struct HostBuf {
     void reserve(std::size_t count) {
         if (count <= n && p) return;
         release();
+        std::lock_guard<std::mutex> lk(device_api_mutex());
         CSG_CHECK(hipHostMalloc(reinterpret_cast<void**>(&p), (count ? count : 1) * sizeof(T),
                                 hipHostMallocDefault));
         n = count;
@@ -109,17 +138,23 @@ struct HostBuf {
 };
 
 // A non-blocking stream destroyed with its owner (also on an exception path).
+inline void create_stream(hipStream_t* s) {
+    std::lock_guard<std::mutex> lk(device_api_mutex());
+    CSG_CHECK(hipStreamCreateWithFlags(s, hipStreamNonBlocking));
+}
+inline void destroy_stream(hipStream_t s) {
+    if (!s) return;
+    (void)hipStreamSynchronize(s);
+    std::lock_guard<std::mutex> lk(device_api_mutex());
+    (void)hipStreamDestroy(s);
+}
+
 struct OwnedStream {
     hipStream_t s = nullptr;
-    OwnedStream() { CSG_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+    OwnedStream() { create_stream(&s); }
     OwnedStream(const OwnedStream&) = delete;
     OwnedStream& operator=(const OwnedStream&) = delete;
-    ~OwnedStream() {
-        if (s) {
-            (void)hipStreamSynchronize(s);
-            (void)hipStreamDestroy(s);
-        }
-    }
+    ~OwnedStream() { destroy_stream(s); }
 };
 
 // Stream capture bracket: ends the capture and discards its graph when left by
@@ -127,7 +162,8 @@ struct OwnedStream {
 struct CaptureGuard {
     hipStream_t s;
     bool open = false;
-    explicit CaptureGuard(hipStream_t stream) : s(stream) {
+    std::unique_lock<std::mutex> api;  // no allocation or free elsewhere meanwhile
+    explicit CaptureGuard(hipStream_t stream) : s(stream), api(device_api_mutex()) {
         CSG_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         open = true;
     }

@@ -155,7 +155,7 @@ coresg_hip_graph* load_graph(int device, const char* path) {
     g.stride = stride;
     g.k = k;
     CSG_CHECK(hipSetDevice(device));
-    CSG_CHECK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    create_stream(&g.stream);
     g.pts.reserve(n * stride);
     g.ids.reserve(nk);
     g.dist.reserve(nk);

@@ -366,7 +366,7 @@ struct MrdCache {
         pts_key = core_key = nullptr;
         m = 0;
         stride = 0;
-        if (stream) (void)hipStreamDestroy(stream);
+        destroy_stream(stream);
         stream = nullptr;
         device = -1;
     }
@@ -392,7 +392,7 @@ void ensure_resident(int device, const float* host_pts, std::size_t m, std::uint
     }
     t_cache.release();
     t_cache.device = device;
-    CSG_CHECK(hipStreamCreateWithFlags(&t_cache.stream, hipStreamNonBlocking));
+    create_stream(&t_cache.stream);
     t_cache.pts.reserve(np);
     t_cache.core.reserve(m);
     t_cache.host_pts.assign(host_pts, host_pts + np);

@@ -2344,8 +2344,8 @@ void nndescent_host(int device, const coresg_hip_nnd_config& cfg, const float* v
                                  "(L2 and cosine only)");
     if (cfg.stride % 8 != 0 || cfg.stride < cfg.dim)
         throw std::runtime_error("nndescent: stride must be round_up(dim, 8)");
-    hipStream_t s;
-    CSG_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    OwnedStream os;
+    hipStream_t s = os.s;
     const std::size_t n = cfg.n, nk = n * cfg.k;
     DBuf<float> d_pts, d_dist;
     DBuf<std::uint32_t> d_ids, d_cnt;
@@ -2369,7 +2369,6 @@ void nndescent_host(int device, const coresg_hip_nnd_config& cfg, const float* v
     CSG_CHECK(hipMemcpyAsync(flags, d_flags.p, nk, hipMemcpyDeviceToHost, s));
     CSG_CHECK(hipMemcpyAsync(counts, d_cnt.p, n * 4, hipMemcpyDeviceToHost, s));
     CSG_CHECK(hipStreamSynchronize(s));
-    CSG_CHECK(hipStreamDestroy(s));
     if (updates) *updates = r.updates;
 }
 

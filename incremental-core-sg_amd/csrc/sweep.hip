@@ -729,7 +729,10 @@ void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_
         std::memcpy(key.data(), &sw, sizeof(Sweep));
         std::memcpy(key.data() + sizeof(Sweep), dims, sizeof(dims));
         if (!g.sweep_exec || g.sweep_key != key) {
-            if (g.sweep_exec) CSG_CHECK(hipGraphExecDestroy(g.sweep_exec));
+            if (g.sweep_exec) {
+                std::lock_guard<std::mutex> lk(device_api_mutex());
+                CSG_CHECK(hipGraphExecDestroy(g.sweep_exec));
+            }
             g.sweep_exec = nullptr;
             hipGraph_t graph = nullptr;
             {
@@ -740,8 +743,12 @@ void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_
                 }
                 graph = cap.end();
             }
-            const hipError_t e = hipGraphInstantiate(&g.sweep_exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
+            hipError_t e;
+            {
+                std::lock_guard<std::mutex> lk(device_api_mutex());
+                e = hipGraphInstantiate(&g.sweep_exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+            }
             CSG_CHECK(e);
             g.sweep_key = key;
         }

@@ -1,8 +1,8 @@
-"""The serving session on the GPU: a mixed stream of inserts and extract
-queries submitted concurrently from several threads gives, request by
-request, exactly the results of the same requests served one at a time in the
-same order; waiting extracts share fused sweeps; the HBM budget admits or
-rejects inserts."""
+"""The serving session on the GPU: inserts and extract queries submitted
+concurrently from several threads; every extract equals the same extract
+served alone on the version it reports; extracts run while an insert builds
+the next version; waiting extracts share fused sweeps; the HBM budget admits
+or rejects inserts."""
 from __future__ import annotations
 
 import threading
@@ -27,8 +27,11 @@ def clone(g, x):
     return h
 
 
-def test_session_mixed_stream_equals_sequential():
-    print("[gpu] session: concurrent mixed stream vs one-at-a-time replay", flush=True)
+def test_session_mixed_stream_equals_alone():
+    """Inserts and extracts submitted concurrently from several threads: every
+    extract equals the same extract served alone on the version it reports
+    (replayed in a second session that applies the same inserts one by one)."""
+    print("[gpu] session: concurrent mixed stream vs the same requests alone", flush=True)
     x, k = datagen.workload("mixture-20k-d16")
     n0 = 12000
     g = chip.Graph(n0, x.shape[1], k)
@@ -36,82 +39,108 @@ def test_session_mixed_stream_equals_sequential():
     g.build(chip.build_params())
     g2 = clone(g, x)
     rng = np.random.default_rng(5)
-    script = []
     cuts = [n0, 12500, 14000, 14001, 17000, 20000]
-    for a, b in zip(cuts[:-1], cuts[1:]):
-        for _ in range(3):
-            m = rng.choice(np.arange(1, k + 1), size=int(rng.integers(1, 9)), replace=False)
-            script.append(("extract", [int(v) for v in m]))
-        script.append(("insert", (a, b)))
-    script.append(("extract", list(range(1, k + 1))))
+    inserts = [(a, b) for a, b in zip(cuts[:-1], cuts[1:])]
+    extracts = []
+    for _ in range(24):
+        m = rng.choice(np.arange(1, k + 1), size=int(rng.integers(1, 9)), replace=False)
+        extracts.append([int(v) for v in m])
+    extracts.append(list(range(1, k + 1)))
 
     a_sess = chip.Session(None, k, graph=g)
-    order = []  # (id, request) in the session's order
+    ins_ids, ext_ids = [], []
     lock = threading.Lock()
 
-    def submit(items):
-        for kind, arg in items:
-            with lock:  # the id order is the session order; keep the script's per-thread order
-                if kind == "insert":
-                    rid = a_sess.submit_insert(x[arg[0]:arg[1]])
-                else:
-                    rid = a_sess.submit_extract(arg, edges=True, labels=True)
-                order.append((rid, kind, arg))
+    def submit_inserts():
+        for a, b in inserts:
+            with lock:
+                ins_ids.append(a_sess.submit_insert(x[a:b]))
 
-    # inserts stay in script order (one thread); extracts interleave from others
-    inserts = [s for s in script if s[0] == "insert"]
-    extracts = [s for s in script if s[0] == "extract"]
-    threads = [threading.Thread(target=submit, args=(inserts,))]
-    threads += [threading.Thread(target=submit, args=(extracts[i::3],)) for i in range(3)]
+    def submit_extracts(items):
+        for m in items:
+            with lock:
+                ext_ids.append((a_sess.submit_extract(m, edges=True, labels=True), m))
+
+    threads = [threading.Thread(target=submit_inserts)]
+    threads += [threading.Thread(target=submit_extracts, args=(extracts[i::3],)) for i in range(3)]
     for t in threads:
         t.start()
     for t in threads:
         t.join()
-    order.sort()
-    res_a, batched = {}, 0
-    for rid, kind, arg in order:
+    versions = []
+    for rid in ins_ids:
+        st = a_sess.wait(rid)
+        assert st["state"] == chip.REQ_DONE, st["error"]
+        versions.append((st["graph_version"], st["graph_n"]))
+    assert versions == [(v + 1, b) for v, (_, b) in enumerate(inserts)]
+    by_version, batched = {}, 0
+    for rid, m in ext_ids:
         st = a_sess.wait(rid)
         assert st["state"] == chip.REQ_DONE, st["error"]
         batched = max(batched, st["batch_requests"])
-        if kind == "extract":
-            res_a[rid] = (st["graph_version"], a_sess.result(rid, edges=True, labels=True))
+        by_version.setdefault(st["graph_version"], []).append(
+            (m, st["graph_n"], a_sess.result(rid, edges=True, labels=True)))
 
     b_sess = chip.Session(None, k, graph=g2)
     checked = 0
-    for rid, kind, arg in order:
-        if kind == "insert":
-            st = b_sess.wait(b_sess.submit_insert(x[arg[0]:arg[1]]))
-            assert st["state"] == chip.REQ_DONE, st["error"]
-            continue
-        bid = b_sess.submit_extract(arg, edges=True, labels=True)
-        st = b_sess.wait(bid)
-        assert st["batch_requests"] == 1
-        version, ra = res_a[rid]
-        assert st["graph_version"] == version
-        rb = b_sess.result(bid, edges=True, labels=True)
-        for p, q in zip(ra, rb):
-            for key in ("mpts", "num_edges", "num_components", "rounds"):
-                assert p[key] == q[key]
-            assert np.float64(p["weight"]).view(np.uint64) == np.float64(q["weight"]).view(np.uint64)
-            assert np.array_equal(p["u"], q["u"]) and np.array_equal(p["v"], q["v"])
-            assert np.array_equal(p["w"].view(np.uint32), q["w"].view(np.uint32))
-            assert np.array_equal(p["labels"], q["labels"])
-        checked += 1
-    print(f"  {checked} extract requests identical; largest fused batch {batched} requests",
-          flush=True)
+    for v in range(len(inserts) + 1):
+        if v > 0:
+            a, b = inserts[v - 1]
+            st = b_sess.wait(b_sess.submit_insert(x[a:b]))
+            assert st["state"] == chip.REQ_DONE and st["graph_version"] == v, st["error"]
+        for m, n_seen, ra in by_version.get(v, []):
+            bid = b_sess.submit_extract(m, edges=True, labels=True)
+            st = b_sess.wait(bid)
+            assert st["batch_requests"] == 1
+            assert st["graph_version"] == v and st["graph_n"] == n_seen
+            rb = b_sess.result(bid, edges=True, labels=True)
+            for p, q in zip(ra, rb):
+                for key in ("mpts", "num_edges", "num_components", "rounds"):
+                    assert p[key] == q[key]
+                assert np.float64(p["weight"]).view(np.uint64) == \
+                    np.float64(q["weight"]).view(np.uint64)
+                assert np.array_equal(p["u"], q["u"]) and np.array_equal(p["v"], q["v"])
+                assert np.array_equal(p["w"].view(np.uint32), q["w"].view(np.uint32))
+                assert np.array_equal(p["labels"], q["labels"])
+            checked += 1
+    print(f"  {checked} extract requests identical over versions {sorted(by_version)}; largest "
+          f"fused batch {batched} requests", flush=True)
     assert checked == len(extracts)
     a_sess.close()
     b_sess.close()
 
 
+def test_session_extract_overlaps_insert():
+    """An extract submitted while an insert is building the next version is
+    served from the published one and completes first."""
+    print("[gpu] session: extract served while an insert runs", flush=True)
+    x = datagen.gaussian_mixture(130_000, 16, 12, 0.3, 2.0, 0.05, seed=9)
+    k = 16
+    s = chip.Session(x[:100_000], k)
+    ins = s.submit_insert(x[100_000:])
+    ext = s.submit_extract([4, 8])
+    se, si = s.wait(ext), s.wait(ins)
+    assert si["state"] == chip.REQ_DONE and se["state"] == chip.REQ_DONE
+    assert se["graph_version"] == 0 and se["graph_n"] == 100_000
+    assert si["graph_version"] == 1 and si["graph_n"] == 130_000
+    assert se["latency_ms"] < si["latency_ms"]
+    st = s.wait(s.submit_extract([4]))
+    assert st["graph_version"] == 1 and st["graph_n"] == 130_000
+    print(f"  extract {se['latency_ms']:.1f} ms (version 0) during insert "
+          f"{si['latency_ms']:.1f} ms", flush=True)
+    s.close()
+
+
 def test_session_fuses_waiting_extracts():
-    x, k = datagen.workload("mixture-20k-d16")
+    """Extracts submitted while a sweep runs join the next sweep together."""
+    x = datagen.gaussian_mixture(150_000, 16, 12, 0.3, 2.0, 0.05, seed=10)
+    k = 16
     s = chip.Session(x, k)
+    first = s.submit_extract(list(range(1, k + 1)))  # keeps the worker busy
     ids = [s.submit_extract([m]) for m in range(1, k + 1)]
-    sts = [s.wait(i) for i in ids]
+    sts = [s.wait(i) for i in [first] + ids]
     assert all(st["state"] == chip.REQ_DONE for st in sts)
-    # the worker takes everything that is waiting when it becomes free
-    assert max(st["batch_requests"] for st in sts) > 1
+    assert max(st["batch_requests"] for st in sts[1:]) > 1
     s.close()
 
 
