@@ -139,7 +139,9 @@ def serving_run(chip, datagen, k, seed_n, batches, insert_rate, extract_rate, bu
                 for _ in range(n_ext)]
     events = sorted([(float(t), "insert", i) for i, t in enumerate(t_ins)] +
                     [(float(t), "extract", i) for i, t in enumerate(t_ext)])
-    params = chip.build_params(sample=16, iters=10)
+    # inserts by graph search (beam search over the existing lists + restricted
+    # refinement); insert_search=0 would scan every point exactly
+    params = chip.build_params(sample=16, iters=10, insert_search=1)
     t0 = time.perf_counter()
     sess = chip.Session(seed_pts, k, params, hbm_budget=budget)
     seed_build_s = time.perf_counter() - t0
@@ -179,6 +181,7 @@ def serving_run(chip, datagen, k, seed_n, batches, insert_rate, extract_rate, bu
     return dict(
         workload=f"serving: seed {seed_n} (d=24) + {batches} log-uniform insert batches "
                  f"[1k,50k] + Poisson extract queries (1..8 mpts from 2..32)",
+        insert_mode="graph search (beam over the existing lists, restricted refinement)",
         seed_build_ms=seed_build_s * 1e3, inserted_points=int(sizes.sum()), final_points=final_n,
         inserts=len(lat["insert"]), extracts=len(lat["extract"]), failed=failed,
         insert_rate_per_s=insert_rate, extract_rate_per_s=extract_rate, wall_s=wall,

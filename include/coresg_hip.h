@@ -159,6 +159,15 @@ typedef struct coresg_hip_build_params {
      * with base distances), 2 = None. Modes 1 and 2 apply to single-instance
      * graphs. */
     uint32_t missing_mode;
+    /* insert: where a new point's neighbours among the existing points come
+     * from. 0 = an exact scan of every point (the lists are the exact k
+     * nearest); 1 = a beam search over the existing lists, entered at the
+     * closest of up to 256 selected anti-hubs and 8 random points, with a
+     * 64-entry beam (the reference's anchored seeding,
+     * incremental_knn.cpp:118-216); the batch's own pairs are scanned
+     * exactly in both modes, and every visited existing point is offered the
+     * new point (try_insert semantics). */
+    uint32_t insert_search;
 } coresg_hip_build_params;
 
 typedef struct coresg_hip_build_stats {

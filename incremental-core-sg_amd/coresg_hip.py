@@ -42,7 +42,7 @@ class BuildParams(C.Structure):
                 ("min_per_component", C.c_size_t), ("ensure_connected", C.c_int),
                 ("reps_per_component", C.c_size_t), ("max_reps", C.c_size_t),
                 ("join_capacity", C.c_uint32), ("gram", C.c_uint32),
-                ("missing_mode", C.c_uint32)]
+                ("missing_mode", C.c_uint32), ("insert_search", C.c_uint32)]
 
 
 class BuildStats(C.Structure):

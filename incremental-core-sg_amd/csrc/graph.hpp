@@ -296,6 +296,7 @@ struct Graph {
 
     DescentBuffers nnd;
     DBuf<unsigned long long> ins_partial;  // insert: per-chunk best lists
+    DBuf<std::uint32_t> ins_hints;         // insert search: entry hints
 
     // Hubness / selection state (filled by the build).
     DBuf<std::int32_t> rdeg;

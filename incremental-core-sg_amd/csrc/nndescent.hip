@@ -1781,6 +1781,9 @@ struct InsertArgs {
     std::uint32_t chunks, chunk_cols;
     unsigned long long* partial;
     unsigned long long* tally;  // Gram-filter scan, profiling: {pairs, exact}
+    // the scanned columns start at col_base; a new point's key lists in
+    // `partial` are `slots` (>= chunks) lists apart
+    std::uint32_t col_base, slots;
 };
 
 // (d, id) lexicographic order on metric distances: the reference's brute
@@ -1824,7 +1827,7 @@ __global__ void __launch_bounds__(kInsThreads) insert_scan_kernel(InsertArgs a) 
     std::uint32_t* s_li = reinterpret_cast<std::uint32_t*>(lds + 2 * kInsRows * KP);
     const std::uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     const std::uint32_t chunk = blockIdx.x % a.chunks;
-    const std::uint32_t col0 = chunk * a.chunk_cols;
+    const std::uint32_t col0 = a.col_base + chunk * a.chunk_cols;
     const std::uint32_t col1 = min(a.n1, col0 + a.chunk_cols);
     const std::uint32_t qstep = (gridDim.x / a.chunks) * kInsRows;
     for (std::uint32_t rb = a.n0 + (blockIdx.x / a.chunks) * kInsRows; rb < a.n1; rb += qstep) {
@@ -1926,7 +1929,7 @@ __global__ void __launch_bounds__(kInsThreads) insert_scan_kernel(InsertArgs a) 
             }
             if (has && a.partial) {
                 unsigned long long* out =
-                    a.partial + (std::size_t(p - a.n0) * a.chunks + chunk) * KP;
+                    a.partial + (std::size_t(p - a.n0) * a.slots + chunk) * KP;
 #pragma unroll
                 for (std::uint32_t i = 0; i < KP; ++i)
                     out[i] = ti[i] == kNone
@@ -1957,7 +1960,7 @@ __device__ __forceinline__ void insert_write_list(const InsertArgs& a, std::uint
                                                   std::uint32_t chunk, const float (&td)[KP],
                                                   const std::uint32_t (&ti)[KP]) {
     if (a.partial) {
-        unsigned long long* out = a.partial + (std::size_t(p - a.n0) * a.chunks + chunk) * KP;
+        unsigned long long* out = a.partial + (std::size_t(p - a.n0) * a.slots + chunk) * KP;
 #pragma unroll
         for (std::uint32_t i = 0; i < KP; ++i)
             out[i] = ti[i] == kNone ? ~0ull
@@ -2012,7 +2015,7 @@ __global__ void __launch_bounds__(kInsThreads) insert_scan_gram_kernel(InsertArg
     const float eta = float(STRIDE + 24) * 0x1p-24f + 0x1p-18f;
     const float nscale = 1.0f + float(KP16 + 4) * 0x1p-24f;
     const std::uint32_t chunk = blockIdx.x % a.chunks;
-    const std::uint32_t col0 = chunk * a.chunk_cols;
+    const std::uint32_t col0 = a.col_base + chunk * a.chunk_cols;
     const std::uint32_t col1 = min(a.n1, col0 + a.chunk_cols);
     const std::uint32_t qstep = (gridDim.x / a.chunks) * kInsRows;
     unsigned long long n_pairs = 0, n_exact = 0;
@@ -2239,6 +2242,199 @@ __global__ void __launch_bounds__(64)
     }
 }
 
+// ---------------------------------------------------------------------------
+// graph-search insertion: each new point finds its neighbours among the
+// existing points by a beam search over the k-NN lists (the reference's
+// anchored seeding, incremental_knn.cpp:118-216) instead of a scan of every
+// point; one wavefront per new point
+// ---------------------------------------------------------------------------
+
+constexpr unsigned kSearchWaves = 2;         // wavefronts per block
+constexpr unsigned kSearchHash = 2048;       // visited-set slots per wavefront (LDS)
+constexpr unsigned kSearchVisitCap = 1536;   // no expansion past this many visits
+constexpr unsigned kSearchHintScan = 256;    // hints scanned for entry points (kHintScan)
+constexpr unsigned kSearchEntries = 8;       // random entry points (search_entries)
+constexpr std::uint32_t kRefineRounds = 5;   // refinement rounds (the CLI's --refine-iters)
+
+struct SearchArgs {
+    const float* pts;
+    std::uint32_t n0, n1, k, stride;  // existing points [0, n0), new points [n0, n1)
+    const std::uint32_t* ids;         // existing lists
+    const float* dist;
+    const std::uint32_t* count;
+    const std::uint32_t* hints;  // the selected anti-hubs: a one-level coarse index
+    std::uint32_t nhints;
+    unsigned long long* cand;  // proposal buffers of the existing points
+    std::uint32_t* fill;
+    std::uint32_t cap;
+    unsigned long long* partial;  // per new point: `slots` key lists of kp keys
+    std::uint32_t slots, slot, kp;
+    std::uint64_t seed;
+    unsigned long long* evals;  // distance evaluations (profiling), or nullptr
+};
+
+__device__ __forceinline__ std::uint64_t xorshift64(std::uint64_t* s) {
+    std::uint64_t x = *s;
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    *s = x;
+    return x;
+}
+
+// Wave-wide ascending bitonic sort of one 64-bit key per lane.
+__device__ __forceinline__ unsigned long long wave_sort_u64(unsigned long long key,
+                                                            std::uint32_t lane) {
+#pragma unroll
+    for (std::uint32_t k2 = 2; k2 <= 64; k2 <<= 1) {
+#pragma unroll
+        for (std::uint32_t j = k2 >> 1; j > 0; j >>= 1) {
+            const unsigned long long o = __shfl_xor(key, static_cast<int>(j), 64);
+            const bool take_min = ((lane & j) == 0) == ((lane & k2) == 0);
+            key = take_min ? (o < key ? o : key) : (o > key ? o : key);
+        }
+    }
+    return key;
+}
+
+// Number of entries of the ascending 64-entry list `l` below key (strictly, or
+// at most key with `le`).
+__device__ __forceinline__ std::uint32_t rank_in(const unsigned long long* l,
+                                                 unsigned long long key, bool le) {
+    std::uint32_t lo = 0, hi = 64;
+    while (lo < hi) {
+        const std::uint32_t mid = (lo + hi) >> 1;
+        const bool below = le ? (l[mid] <= key) : (l[mid] < key);
+        if (below)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(64 * kSearchWaves) insert_search_kernel(SearchArgs a) {
+    __shared__ std::uint32_t s_hash[kSearchWaves][kSearchHash];
+    __shared__ unsigned long long s_beam[kSearchWaves][64], s_new[kSearchWaves][64];
+    __shared__ unsigned long long s_out[kSearchWaves][128];
+    __shared__ std::uint8_t s_outx[kSearchWaves][128];
+    const std::uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const std::uint32_t stride = a.stride;
+    unsigned long long evals = 0;
+    for (std::uint32_t q = a.n0 + blockIdx.x * kSearchWaves + w; q < a.n1;
+         q += gridDim.x * kSearchWaves) {
+        const float* vq = a.pts + std::size_t(q) * stride;
+        for (std::uint32_t i = lane; i < kSearchHash; i += 64) s_hash[w][i] = kNone;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // the beam: lane i holds the i-th best key (metric distance bits <<
+        // 32 | id) and whether it was expanded
+        unsigned long long beam = ~0ull;
+        bool expanded = false;
+        std::uint32_t visited = 0;
+        std::uint64_t rs = a.seed ^ (0x9E3779B97F4A7C15ull * (std::uint64_t(q) + 1));
+
+        // Offers one candidate id per lane (kNone: none): first visits get
+        // their exact distance, are proposed to their own list when they beat
+        // its k-th (the merge makes the exact test), and join the beam.
+        auto offer = [&](std::uint32_t id) {
+            bool fresh = false;
+            if (id < a.n0) {
+                std::uint32_t h = (id * 2654435761u) >> 21;  // 11 bits: kSearchHash slots
+                for (int t = 0; t < 16; ++t) {
+                    const std::uint32_t old = atomicCAS(&s_hash[w][h], kNone, id);
+                    if (old == kNone) {
+                        fresh = true;
+                        break;
+                    }
+                    if (old == id) break;
+                    h = (h + 1) & (kSearchHash - 1);
+                }
+            }
+            unsigned long long key = ~0ull;
+            if (fresh) {
+                const float r = l2sqr_exact(vq, a.pts + std::size_t(id) * stride, stride);
+                const float d = sqrt_exact(r);
+                key = (static_cast<unsigned long long>(fbits(d)) << 32) | id;
+                ++evals;
+                bool propose = a.count[id] < a.k;
+                if (!propose) {
+                    const float dk = a.dist[std::size_t(id) * a.k + a.k - 1];
+                    propose = r < dk * dk * (1.0f + 0x1p-20f);
+                }
+                if (propose) {
+                    const std::uint32_t pos = atomicAdd(&a.fill[id], 1u);
+                    if (pos < a.cap)
+                        a.cand[std::size_t(id) * a.cap + pos] =
+                            (static_cast<unsigned long long>(fbits(d)) << 32) | q;
+                }
+            }
+            visited += static_cast<std::uint32_t>(__popcll(__ballot(fresh)));
+            if (!__ballot(fresh)) return;
+            // merge the sorted offers with the beam, keep the 64 smallest
+            const unsigned long long nk = wave_sort_u64(key, lane);
+            s_beam[w][lane] = beam;
+            s_new[w][lane] = nk;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            const std::uint32_t pb = lane + rank_in(s_new[w], beam, false);
+            const std::uint32_t pn = lane + rank_in(s_beam[w], nk, true);
+            s_out[w][pb] = beam;
+            s_outx[w][pb] = expanded ? 1 : 0;
+            s_out[w][pn] = nk;
+            s_outx[w][pn] = 0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            beam = s_out[w][lane];
+            expanded = s_outx[w][lane] != 0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        };
+
+        // entry points: the closest of up to kSearchHintScan hints, then
+        // kSearchEntries uniform draws
+        if (a.nhints) {
+            const std::uint32_t scan = a.nhints < kSearchHintScan ? a.nhints : kSearchHintScan;
+            const std::uint32_t hstep = a.nhints / scan > 1 ? a.nhints / scan : 1;
+            const std::uint32_t off = static_cast<std::uint32_t>(xorshift64(&rs) % a.nhints);
+            for (std::uint32_t t0 = 0; t0 < scan; t0 += 64) {
+                const std::uint32_t t = t0 + lane;
+                offer(t < scan ? a.hints[(off + std::size_t(t) * hstep) % a.nhints] : kNone);
+            }
+        }
+        {
+            std::uint32_t id = kNone;
+            for (std::uint32_t e = 0; e < kSearchEntries; ++e) {
+                const std::uint32_t r = static_cast<std::uint32_t>(xorshift64(&rs) % a.n0);
+                if (lane == e) id = r;
+            }
+            offer(id);
+        }
+        // expand the closest unexpanded entry until every entry of the beam is
+        // expanded (nothing outside it can improve it) or the visit cap
+        while (visited < kSearchVisitCap) {
+            const unsigned long long open = __ballot(beam != ~0ull && !expanded);
+            if (!open) break;
+            const std::uint32_t b = static_cast<std::uint32_t>(__ffsll(static_cast<long long>(open)) - 1);
+            const std::uint32_t c = static_cast<std::uint32_t>(
+                __shfl(static_cast<unsigned int>(beam & 0xFFFFFFFFull), static_cast<int>(b), 64));
+            if (lane == b) expanded = true;
+            const std::uint32_t deg = a.count[c];
+            offer(lane < deg && lane < a.k ? a.ids[std::size_t(c) * a.k + lane] : kNone);
+        }
+        // the kp best as this point's key list in its search slot
+        if (lane < a.kp) {
+            unsigned long long* out =
+                a.partial + (std::size_t(q - a.n0) * a.slots + a.slot) * a.kp;
+            out[lane] = beam;
+        }
+    }
+    if (a.evals) {
+        evals = wave_sum(evals);
+        if (lane == 0) atomicAdd(a.evals, evals);
+    }
+}
+
 using InsertFn = void (*)(InsertArgs);
 
 template <std::uint32_t KP>
@@ -2428,6 +2624,7 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
     g.hub_ready = false;
 
     std::uint64_t applied = 0;
+    std::uint32_t refine_rounds = 0;
     if (brute) {
         // exact lists for every point (incremental_knn.cpp:1169-1180)
         brute_force_kernel<<<capped_blocks(n1, 64), 64, 0, s>>>(g.pts.p, n1, stride, k, g.ids.p,
@@ -2441,17 +2638,27 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
         b.partial.reserve(kMergeBlocks);
         b.updates.reserve(1);
         fill_u32(b.fill.p, n0 ? n0 : 1, 0u, s);
+        // Graph search (insert_search = 1): the batch's neighbours among the
+        // existing points come from a beam search over their lists, and the
+        // scan below covers only the batch itself (its own points' pairs);
+        // otherwise the scan covers every point.
+        const bool search = p.insert_search != 0 && n0 > 0;
+        const std::uint32_t col_base = search ? n0 : 0u;
+        const std::uint32_t ncols = n1 - col_base;
         // Column chunks so that small batches still fill the chip: about 2048
-        // blocks, at most 16 chunks (16 * KP <= 512 keys per point to merge)
+        // blocks, at most 16 chunks (16 * KP <= 512 keys per point to merge,
+        // one list fewer with the search's)
         const std::uint32_t qblocks = (count + kInsRows - 1) / kInsRows;
         std::uint32_t chunks = (2048u + qblocks - 1) / qblocks;
-        chunks = std::max(1u, std::min({chunks, 16u, 512u / kp, (n1 + kInsTile - 1) / kInsTile}));
+        chunks = std::max(1u, std::min({chunks, 16u, 512u / kp - (search ? 1u : 0u),
+                                        (ncols + kInsTile - 1) / kInsTile}));
         const std::uint32_t chunk_cols =
-            ((n1 + chunks - 1) / chunks + kInsTile - 1) / kInsTile * kInsTile;
-        chunks = (n1 + chunk_cols - 1) / chunk_cols;
+            ((ncols + chunks - 1) / chunks + kInsTile - 1) / kInsTile * kInsTile;
+        chunks = (ncols + chunk_cols - 1) / chunk_cols;
+        const std::uint32_t slots = chunks + (search ? 1u : 0u);
         unsigned long long* partial = nullptr;
-        if (chunks > 1) {
-            g.ins_partial.reserve(std::size_t(count) * chunks * kp);
+        if (slots > 1) {
+            g.ins_partial.reserve(std::size_t(count) * slots * kp);
             partial = g.ins_partial.p;
         }
         unsigned long long* tally = nullptr;
@@ -2461,7 +2668,37 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
             tally = b.gram_tally.p;
         }
         InsertArgs ia{g.pts.p, n0, n1, k, g.ids.p, g.dist.p, g.flags.p, g.count.p,
-                      b.cand.p, b.fill.p, cap, chunks, chunk_cols, partial, tally};
+                      b.cand.p, b.fill.p, cap, chunks, chunk_cols, partial, tally,
+                      col_base, slots};
+        if (search) {
+            // entry hints: the current anti-hub selection (over the existing
+            // points), as the reference passes it (incremental_knn.cpp:135-160)
+            const std::uint32_t nh = static_cast<std::uint32_t>(g.selected.size());
+            g.ins_hints.reserve(nh ? nh : 1);
+            if (nh)
+                CSG_CHECK(hipMemcpyAsync(g.ins_hints.p, g.selected.data(), std::size_t(nh) * 4,
+                                         hipMemcpyHostToDevice, s));
+            unsigned long long* evals = nullptr;
+            if (g.profiling) {
+                b.tally.reserve(3);
+                fill_u64(reinterpret_cast<std::uint64_t*>(b.tally.p), 1, 0ull, s);
+                evals = b.tally.p;
+            }
+            SearchArgs sa{g.pts.p, n0, n1, k, stride, g.ids.p, g.dist.p, g.count.p,
+                          g.ins_hints.p, nh, b.cand.p, b.fill.p, cap, partial, slots, chunks,
+                          kp, p.seed ^ (std::uint64_t(n0) << 20), evals};
+            const unsigned sblocks = static_cast<unsigned>(
+                std::min<std::size_t>((count + kSearchWaves - 1) / kSearchWaves, 16384));
+            ScopedKernelTimer t(g, "insert_search");
+            insert_search_kernel<<<sblocks, 64 * kSearchWaves, 0, s>>>(sa);
+            CSG_LAUNCH_CHECK();
+            if (evals) {
+                unsigned long long ev = 0;
+                CSG_CHECK(hipMemcpyAsync(&ev, evals, 8, hipMemcpyDeviceToHost, s));
+                CSG_CHECK(hipStreamSynchronize(s));
+                g.add_work("insert_search", {count, ev, 0, 0});
+            }
+        }
         const std::size_t lds_tile =
             gram_scan ? (std::size_t(kGramTile) * stride + 3 * kGramTile + std::size_t(kInsRows) * stride +
                          kInsRows + std::size_t(kInsRows) * (kGramTile + 1)) *
@@ -2480,7 +2717,7 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
             hipLaunchKernelGGL(fn, dim3(blocks), dim3(kInsThreads), lds, s, ia);
             CSG_LAUNCH_CHECK();
             if (partial) {
-                const std::uint32_t keys = chunks * kp;
+                const std::uint32_t keys = slots * kp;
                 const unsigned mblocks = static_cast<unsigned>(std::min<std::size_t>(count, 65536));
                 if (keys <= 64)
                     insert_merge_kernel<1><<<mblocks, 64, 0, s>>>(partial, n0, n1, k, keys, g.ids.p,
@@ -2511,6 +2748,19 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
             CSG_CHECK(hipStreamSynchronize(s));
             applied = upd;
         }
+        if (search) {
+            // restricted refinement (the reference's refine_nnd after the
+            // merge, incremental_knn.cpp:873-987): descent rounds over the
+            // grown graph, which only nodes holding entries not yet joined --
+            // the batch's lists and the entries it just placed -- take part in;
+            // up to kRefineRounds, stopping at the usual update threshold
+            const DescentResult r =
+                run_descent(&g, g.pts.p, n1, stride, k, g.ids.p, g.dist.p, g.flags.p, g.count.p,
+                            p.sample, kRefineRounds, p.delta, p.seed ^ 0x5851F42D4C957F2Dull,
+                            false, kDefaultCap, p.gram, s);
+            applied += r.updates;
+            refine_rounds = r.rounds;
+        }
         if (tally) {
             unsigned long long gt[2] = {0, 0};
             CSG_CHECK(hipMemcpyAsync(gt, tally, sizeof(gt), hipMemcpyDeviceToHost, s));
@@ -2523,7 +2773,7 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
     build_bridges(g, p, st);
     if (st) {
         st->knn_seconds = knn_s;
-        st->rounds = 0;
+        st->rounds = refine_rounds;
         st->updates = applied;
         st->total_seconds =
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
