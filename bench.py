@@ -333,6 +333,8 @@ def batched_run(chip, datagen, n_sets=256, k=24, seed=5):
         g.set_points(x)
         g.set_instances(off)
         g.build(params)  # warm (allocations)
+        g.prepare_edges()
+        tune = g.autotune_sweep(mpts)  # this (n, d) bucket's sweep shape
         t0 = time.perf_counter()
         st = g.build(params)
         t1 = time.perf_counter()
@@ -344,6 +346,8 @@ def batched_run(chip, datagen, n_sets=256, k=24, seed=5):
         for j, i in enumerate(idx):
             weights[i] = [r["weight"] for r in per[j]]
         out["buckets"][str(d)] = dict(instances=len(idx), points=int(off[-1]),
+                                      sweep_shape=dict(lanes=tune["lanes"],
+                                                       max_blocks=tune["max_blocks"]),
                                       build_ms=(t1 - t0) * 1e3, sweep_ms=(t2 - t1) * 1e3,
                                       knn_ms=st["knn_seconds"] * 1e3,
                                       hubness_select_ms=st["hubness_seconds"] * 1e3,
@@ -440,6 +444,16 @@ def main():
     for w in range(args.warmup):
         step(None)
         print(f"[bench] warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    # the sweep's launch shapes, autotuned on this graph for the batched sweep
+    # (32 slots), the 16-value sweep and single-mpts queries
+    g.prepare_edges()
+    tuned = {}
+    for name, mp in (("sweep32", mpts), ("sweep16", list(range(2, k + 1, 2))[:16]),
+                     ("single", [mpts[len(mpts) // 2]])):
+        tuned[name] = g.autotune_sweep(mp)
+        print(f"[bench] autotune {name}: {tuned[name]['lanes']} lanes, <= "
+              f"{tuned[name]['max_blocks']} blocks, {tuned[name]['best_ms']:.2f} ms "
+              f"(default {tuned[name]['default_ms']:.2f} ms)", file=sys.stderr, flush=True)
     g.reset_timers()
     g.set_profiling(True)
     rec = []
@@ -604,6 +618,7 @@ def main():
         "sweep_one_call_per_mpts_ms": plain_s * 1e3,
         "sweep_batched_over_one_call_per_mpts": plain_s / sweep_s,
         "plain_vs_fused": plain_vs_fused,
+        "autotune": tuned,
         "extract_p50_ms": percentile(lat, 50),
         "extract_p99_ms": percentile(lat, 99),
         "knn_rounds": st["rounds"],

@@ -286,6 +286,32 @@ int coresg_hip_write_edges_tsv(const char* path, const uint32_t* u, const uint32
 /* write_labels_tsv (io.cpp:140-145): "id\tlabel" then one line per point. */
 int coresg_hip_write_labels_tsv(const char* path, const uint32_t* labels, size_t n);
 
+/* ---- autotuning ----------------------------------------------------------- */
+
+/* The launch shape chosen for the mpts sweep's Borůvka round kernel. */
+typedef struct coresg_hip_tune_result {
+    uint32_t slots;      /* mpts values per pass the shape was tuned for */
+    uint32_t lanes;      /* lanes serving one vertex (4..64, >= slots rounded up) */
+    uint32_t max_blocks; /* grid cap of the round kernel */
+    uint32_t candidates; /* shapes timed */
+    double best_ms;      /* one extract of the slots with the chosen shape */
+    double default_ms;   /* the same with the untuned shape */
+} coresg_hip_tune_result;
+
+/* Times every candidate launch shape of the sweep's round kernel (lanes per
+ * vertex x grid cap) on this graph with the first min(count, 32) values of
+ * `mpts` and keeps the fastest, for this graph and for every later graph of
+ * the same size bucket (floor(log2 n), k_max) and slot count. Results never
+ * depend on the shape. */
+int coresg_hip_autotune_sweep(coresg_hip_graph* g, const uint32_t* mpts, uint32_t count,
+                              coresg_hip_tune_result* out);
+/* The process-wide table of tuned shapes as text, one line per bucket:
+ * "log2n k_max slots lanes max_blocks". Writes up to cap bytes (NUL
+ * terminated when room) and returns the length of the whole table. */
+size_t coresg_hip_tune_table(char* buf, size_t cap);
+/* Adds the lines of such a table (a table saved by an earlier process). */
+int coresg_hip_tune_table_load(const char* text);
+
 /* ---- 3. serving session -------------------------------------------------- */
 
 /* One device-resident CORE-SG served by continuous batching. The reference

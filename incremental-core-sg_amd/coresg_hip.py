@@ -62,6 +62,14 @@ class MstInfo(C.Structure):
                 ("num_components", C.c_uint32), ("rounds", C.c_uint32), ("weight", C.c_double)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("slots", C.c_uint32), ("lanes", C.c_uint32), ("max_blocks", C.c_uint32),
+                ("candidates", C.c_uint32), ("best_ms", C.c_double), ("default_ms", C.c_double)]
+
+    def asdict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class RequestStatus(C.Structure):
     _fields_ = [("id", C.c_uint64), ("kind", C.c_int), ("state", C.c_int),
                 ("graph_n", C.c_uint32), ("graph_version", C.c_uint64), ("num_msts", C.c_uint32),
@@ -130,6 +138,9 @@ SIGNATURES = [
     ("coresg_hip_sampled_recall", C.c_int, [_vp, C.c_size_t, C.c_uint64,
                                             C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("coresg_hip_graph_set_plain", C.c_int, [_vp, C.c_int]),
+    ("coresg_hip_autotune_sweep", C.c_int, [_vp, _vp, C.c_uint32, C.POINTER(TuneResult)]),
+    ("coresg_hip_tune_table", C.c_size_t, [C.c_char_p, C.c_size_t]),
+    ("coresg_hip_tune_table_load", C.c_int, [C.c_char_p]),
     ("coresg_hip_graph_save", C.c_int, [_vp, C.c_char_p]),
     ("coresg_hip_graph_load", _vp, [C.c_int, C.c_char_p]),
     ("coresg_hip_write_edges_tsv", C.c_int, [C.c_char_p, _vp, _vp, _vp, C.c_size_t]),
@@ -413,6 +424,14 @@ class Graph:
         """CoreSG::save (io.cpp:232-260) in the reference's .coresg format."""
         check(lib().coresg_hip_graph_save(self.h, os.fsencode(path)))
 
+    def autotune_sweep(self, mpts) -> dict:
+        """Time the sweep's launch shapes with these mpts values (<= 32 used)
+        and keep the fastest for this graph and its size bucket."""
+        mp = _u32(list(mpts))
+        r = TuneResult()
+        check(lib().coresg_hip_autotune_sweep(self.h, ptr(mp), mp.size, C.byref(r)))
+        return r.asdict()
+
     def set_plain(self, on: bool):
         """The plain one-launch-per-step form (host sync after every launch,
         one mpts per sweep pass, no captured graphs); same results."""
@@ -469,6 +488,19 @@ def mrd_block(pts_padded: np.ndarray, dim: int, core: np.ndarray, r0: int = 0,
     check(lib().coresg_hip_mrd_block(device, ptr(pts), m, dim, stride, metric, ptr(core), r0, r1,
                                      ptr(out)))
     return out.reshape(r1 - r0, m)
+
+
+def tune_table() -> str:
+    """The process-wide table of tuned sweep shapes ("log2n k slots lanes
+    max_blocks" per line)."""
+    m = lib().coresg_hip_tune_table(None, 0)
+    buf = C.create_string_buffer(m + 1)
+    lib().coresg_hip_tune_table(buf, m + 1)
+    return buf.value.decode()
+
+
+def tune_table_load(text: str) -> None:
+    check(lib().coresg_hip_tune_table_load(text.encode()))
 
 
 def write_edges_tsv(path: str, u, v, w) -> None:

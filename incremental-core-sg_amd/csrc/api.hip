@@ -361,6 +361,29 @@ int coresg_hip_exact_mst(coresg_hip_graph* h, uint32_t mpts, coresg_hip_mst_info
     });
 }
 
+int coresg_hip_autotune_sweep(coresg_hip_graph* h, const uint32_t* mpts, uint32_t count,
+                              coresg_hip_tune_result* out) {
+    return guarded([&] {
+        bind(h->g);
+        autotune_sweep(h->g, mpts, count, out);
+    });
+}
+
+size_t coresg_hip_tune_table(char* buf, size_t cap) {
+    std::string t;
+    guarded_host([&] { t = tune_table_text(); });
+    if (buf && cap) {
+        const std::size_t m = t.size() < cap - 1 ? t.size() : cap - 1;
+        std::memcpy(buf, t.data(), m);
+        buf[m] = '\0';
+    }
+    return t.size();
+}
+
+int coresg_hip_tune_table_load(const char* text) {
+    return guarded_host([&] { tune_table_load(text); });
+}
+
 int coresg_hip_graph_set_plain(coresg_hip_graph* h, int on) {
     return guarded([&] { h->g.plain = on != 0; });
 }

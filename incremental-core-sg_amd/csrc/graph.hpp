@@ -229,6 +229,12 @@ struct DescentBuffers {
     EventPair round_done;
 };
 
+// Launch shape of the sweep's Boruvka round kernel: lanes per vertex group
+// (at least the slot count rounded up to a power of two) and the grid cap.
+struct SweepShape {
+    std::uint32_t lanes = 0, max_blocks = 0;
+};
+
 // Accumulated device time of one kernel (HIP events on the graph's stream).
 struct KernelTimer {
     std::vector<hipEvent_t> pending;  // start/stop pairs not yet resolved
@@ -293,6 +299,8 @@ struct Graph {
     hipGraphExec_t sweep_exec = nullptr;
     std::vector<unsigned char> sweep_key;
     DBuf<float> out_w;
+    // autotuned sweep launch shapes of this graph, by slots (power of two)
+    std::map<std::uint32_t, SweepShape> sweep_tune;
 
     DescentBuffers nnd;
     DBuf<unsigned long long> ins_partial;  // insert: per-chunk best lists
@@ -354,6 +362,15 @@ void extract_msts(Graph& g, const std::uint32_t* mpts, std::uint32_t count, void
                   std::uint32_t* out_labels);
 void reweighted_edges(Graph& g, std::uint32_t mpts, std::uint32_t* u, std::uint32_t* v,
                       float* w);
+// The sweep's round-kernel launch shape for Bp slots on g: tuned on g, else
+// tuned on a graph of the same size bucket (log2 n, k), else the default.
+SweepShape sweep_shape(const Graph& g, std::uint32_t Bp);
+// Times the candidate launch shapes on g with the first min(count, 32) of
+// `mpts` and keeps the fastest for g and its size bucket; `res` is a
+// coresg_hip_tune_result.
+void autotune_sweep(Graph& g, const std::uint32_t* mpts, std::uint32_t count, void* res);
+std::string tune_table_text();
+void tune_table_load(const char* text);
 // Per-instance MST summaries of a batch of mpts values (info[i * count + q]).
 void extract_instances(Graph& g, const std::uint32_t* mpts, std::uint32_t count, void* info);
 // CoreSG::exact_mst over the resident points (exact.hip); info is a

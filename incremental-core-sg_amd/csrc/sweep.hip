@@ -27,9 +27,14 @@
 // on the host in Kruskal order.
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/coresg_hip.h"
@@ -574,18 +579,23 @@ std::uint32_t pow2_at_least(std::uint32_t x) {
     return p;
 }
 
-void launch_round(const Sweep& sw, bool first, unsigned blocks, hipStream_t s) {
-    const bool wide = sw.Bp > 16;
-    if (first) {
-        if (wide)
-            sweep_round_kernel<32, true><<<blocks, kThreads, 0, s>>>(sw);
-        else
-            sweep_round_kernel<16, true><<<blocks, kThreads, 0, s>>>(sw);
-    } else {
-        if (wide)
-            sweep_round_kernel<32, false><<<blocks, kThreads, 0, s>>>(sw);
-        else
-            sweep_round_kernel<16, false><<<blocks, kThreads, 0, s>>>(sw);
+// Lanes per vertex group (W >= Bp: a group holds every slot of its vertex).
+template <std::uint32_t W>
+void launch_round_w(const Sweep& sw, bool first, unsigned blocks, hipStream_t s) {
+    if (first)
+        sweep_round_kernel<W, true><<<blocks, kThreads, 0, s>>>(sw);
+    else
+        sweep_round_kernel<W, false><<<blocks, kThreads, 0, s>>>(sw);
+}
+
+void launch_round(const Sweep& sw, bool first, std::uint32_t W, unsigned blocks, hipStream_t s) {
+    switch (W) {
+        case 4: launch_round_w<4>(sw, first, blocks, s); break;
+        case 8: launch_round_w<8>(sw, first, blocks, s); break;
+        case 16: launch_round_w<16>(sw, first, blocks, s); break;
+        case 32: launch_round_w<32>(sw, first, blocks, s); break;
+        case 64: launch_round_w<64>(sw, first, blocks, s); break;
+        default: throw std::runtime_error("sweep: lanes per vertex must be 4..64");
     }
     CSG_LAUNCH_CHECK();
 }
@@ -676,8 +686,12 @@ void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_
     sw.err = g.counters.p;
 
     const unsigned eblk = capped_blocks(nb, kThreads);
-    const unsigned vlanes = sw.Bp > 16 ? 32u : 16u;
-    const unsigned vblk = capped_blocks(std::size_t(n) * vlanes, kThreads, 16384);
+    // launch shape of the round kernel: the autotuned one for this slot
+    // count, else 16 lanes per vertex (32 above 16 slots) and at most 16384
+    // blocks
+    SweepShape shape = sweep_shape(g, sw.Bp);
+    const unsigned vlanes = shape.lanes;
+    const unsigned vblk = capped_blocks(std::size_t(n) * vlanes, kThreads, shape.max_blocks);
     sweep_init_kernel<<<eblk, kThreads, 0, s>>>(sw, g.dist.p, g.count.p, g.slot_mpts.p);
     CSG_LAUNCH_CHECK();
 
@@ -701,7 +715,7 @@ void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_
     {
         if (g.profiling) g.add_work("boruvka_round1", {2 * m, n, B});
         ScopedKernelTimer t(g, "boruvka_round1");
-        launch_round(sw, true, vblk, s);
+        launch_round(sw, true, vlanes, vblk, s);
     }
     round_tail();
     if (g.plain) {
@@ -716,7 +730,7 @@ void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_
                                          std::to_string(st[0]) + ")");
             if (st[1] == 0) break;
             if (round > 64) throw std::runtime_error("extract: Boruvka did not converge");
-            launch_round(sw, false, vblk, s);
+            launch_round(sw, false, vlanes, vblk, s);
             round_tail();
         }
     }
@@ -725,7 +739,7 @@ void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_
     // host read of the live mask per replay
     if (!g.plain) {
         std::vector<unsigned char> key(sizeof(Sweep) + 4 * sizeof(unsigned));
-        const unsigned dims[4] = {vblk, eblk, B, kGraphRounds};
+        const unsigned dims[4] = {vblk, eblk, B, kGraphRounds * 256u + vlanes};
         std::memcpy(key.data(), &sw, sizeof(Sweep));
         std::memcpy(key.data() + sizeof(Sweep), dims, sizeof(dims));
         if (!g.sweep_exec || g.sweep_key != key) {
@@ -738,7 +752,7 @@ void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_
             {
                 CaptureGuard cap(s);
                 for (std::uint32_t r = 0; r < kGraphRounds; ++r) {
-                    launch_round(sw, false, vblk, s);
+                    launch_round(sw, false, vlanes, vblk, s);
                     round_tail();
                 }
                 graph = cap.end();
@@ -891,6 +905,122 @@ void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_
 }
 
 }  // namespace
+
+// ---- the sweep's autotuner ---------------------------------------------------
+
+namespace {
+
+struct TuneKey {
+    std::uint32_t log2n, k, Bp;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(log2n, k, Bp) < std::tie(o.log2n, o.k, o.Bp);
+    }
+};
+
+std::mutex g_tune_mu;
+std::map<TuneKey, SweepShape> g_tune;  // process-wide, by size bucket
+
+TuneKey tune_key(const Graph& g, std::uint32_t Bp) {
+    std::uint32_t l = 0;
+    while ((std::uint64_t(1) << (l + 1)) <= g.n) ++l;
+    return TuneKey{l, g.k, Bp};
+}
+
+}  // namespace
+
+SweepShape sweep_shape(const Graph& g, std::uint32_t Bp) {
+    auto it = g.sweep_tune.find(Bp);
+    if (it != g.sweep_tune.end()) return it->second;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto jt = g_tune.find(tune_key(g, Bp));
+        if (jt != g_tune.end()) return jt->second;
+    }
+    return SweepShape{Bp > 16 ? 32u : 16u, 16384u};
+}
+
+void autotune_sweep(Graph& g, const std::uint32_t* mpts, std::uint32_t count, void* res_v) {
+    auto* res = static_cast<coresg_hip_tune_result*>(res_v);
+    if (count == 0) throw std::runtime_error("autotune: no mpts values");
+    const std::uint32_t B = std::min<std::uint32_t>(count, kMaxSlots);
+    const std::uint32_t Bp = pow2_at_least(B);
+    prepare_edges(g);
+    std::vector<coresg_hip_mst_info> info(B);
+    auto time_one = [&]() {
+        double best = 1e300;
+        for (int rep = 0; rep < 3; ++rep) {  // the first also warms up
+            const auto t0 = std::chrono::steady_clock::now();
+            extract_msts(g, mpts, B, info.data(), nullptr, nullptr, nullptr, nullptr);
+            const double ms =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                    .count();
+            if (rep > 0) best = std::min(best, ms);
+        }
+        return best;
+    };
+    g.sweep_tune[Bp] = SweepShape{Bp > 16 ? 32u : 16u, 16384u};
+    const double default_ms = time_one();
+    SweepShape best = g.sweep_tune[Bp];
+    double best_ms = default_ms;
+    std::uint32_t tried = 1;
+    for (std::uint32_t lanes = 4; lanes <= 64; lanes <<= 1) {
+        if (lanes < Bp) continue;
+        for (std::uint32_t cap : {4096u, 8192u, 16384u, 32768u}) {
+            const SweepShape c{lanes, cap};
+            if (c.lanes == (Bp > 16 ? 32u : 16u) && c.max_blocks == 16384u) continue;
+            g.sweep_tune[Bp] = c;
+            const double ms = time_one();
+            ++tried;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+    }
+    g.sweep_tune[Bp] = best;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[tune_key(g, Bp)] = best;
+    }
+    if (res) {
+        res->slots = B;
+        res->lanes = best.lanes;
+        res->max_blocks = best.max_blocks;
+        res->candidates = tried;
+        res->best_ms = best_ms;
+        res->default_ms = default_ms;
+    }
+}
+
+std::string tune_table_text() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::string out;
+    for (const auto& kv : g_tune) {
+        out += std::to_string(kv.first.log2n) + " " + std::to_string(kv.first.k) + " " +
+               std::to_string(kv.first.Bp) + " " + std::to_string(kv.second.lanes) + " " +
+               std::to_string(kv.second.max_blocks) + "\n";
+    }
+    return out;
+}
+
+void tune_table_load(const char* text) {
+    std::map<TuneKey, SweepShape> add;
+    const char* p = text;
+    while (p && *p) {
+        unsigned a, b, c, d, e;
+        int used = 0;
+        if (std::sscanf(p, "%u %u %u %u %u%n", &a, &b, &c, &d, &e, &used) != 5)
+            throw std::runtime_error("tune table: expected 'log2n k slots lanes max_blocks' lines");
+        if (d < 4 || d > 64 || (d & (d - 1)) || d < c || c == 0 || c > 32 || (c & (c - 1)) ||
+            e == 0)
+            throw std::runtime_error("tune table: invalid entry");
+        add[TuneKey{a, b, c}] = SweepShape{d, e};
+        p += used;
+        while (*p == '\n' || *p == ' ' || *p == '\r' || *p == '\t') ++p;
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (const auto& kv : add) g_tune[kv.first] = kv.second;
+}
 
 void extract_msts(Graph& g, const std::uint32_t* mpts_list, std::uint32_t count, void* info_v,
                   std::uint32_t* out_u, std::uint32_t* out_v, float* out_w,
