@@ -168,6 +168,9 @@ typedef struct coresg_hip_build_params {
      * exactly in both modes, and every visited existing point is offered the
      * new point (try_insert semantics). */
     uint32_t insert_search;
+    /* the k of the score's degenerate case (HubnessParams::k_for_score,
+     * hubness.hpp:83; 0 = the graph's k_max, which CoreSG::build sets) */
+    uint32_t score_k;
 } coresg_hip_build_params;
 
 typedef struct coresg_hip_build_stats {
@@ -244,6 +247,14 @@ int coresg_hip_insert(coresg_hip_graph* g, const float* rows, uint32_t count, in
 /* Current number of points (grows with coresg_hip_insert). */
 uint32_t coresg_hip_graph_size(const coresg_hip_graph* g);
 
+/* The derived hubness state alone, from the k-NN lists (points need not be
+ * set; the graph's dim is the budget dimension): ConnectivityTracker::rebuild
+ * (connectivity.cpp:24-139), HubnessIndex::build (hubness.cpp:44-158) and
+ * rescore_all with the anti-hub budget and quota (hubness.cpp:16-40,
+ * 424-521) under p's antihub_multiplier, max_pair_ops, min/max_selected and
+ * min_per_component. Read back with coresg_hip_graph_get_hubness /
+ * _get_selected. */
+int coresg_hip_graph_derive_hubness(coresg_hip_graph* g, const coresg_hip_build_params* p);
 size_t coresg_hip_graph_get_selected(coresg_hip_graph* g, uint32_t* out, size_t cap);
 int coresg_hip_graph_get_hubness(coresg_hip_graph* g, int32_t* rdeg, double* score,
                                  uint32_t* conn_labels, uint32_t* num_components);

@@ -42,7 +42,8 @@ class BuildParams(C.Structure):
                 ("min_per_component", C.c_size_t), ("ensure_connected", C.c_int),
                 ("reps_per_component", C.c_size_t), ("max_reps", C.c_size_t),
                 ("join_capacity", C.c_uint32), ("gram", C.c_uint32),
-                ("missing_mode", C.c_uint32), ("insert_search", C.c_uint32)]
+                ("missing_mode", C.c_uint32), ("insert_search", C.c_uint32),
+                ("score_k", C.c_uint32)]
 
 
 class BuildStats(C.Structure):
@@ -138,6 +139,7 @@ SIGNATURES = [
     ("coresg_hip_sampled_recall", C.c_int, [_vp, C.c_size_t, C.c_uint64,
                                             C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("coresg_hip_graph_set_plain", C.c_int, [_vp, C.c_int]),
+    ("coresg_hip_graph_derive_hubness", C.c_int, [_vp, C.POINTER(BuildParams)]),
     ("coresg_hip_autotune_sweep", C.c_int, [_vp, _vp, C.c_uint32, C.POINTER(TuneResult)]),
     ("coresg_hip_tune_table", C.c_size_t, [C.c_char_p, C.c_size_t]),
     ("coresg_hip_tune_table_load", C.c_int, [C.c_char_p]),

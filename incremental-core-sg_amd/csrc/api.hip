@@ -361,6 +361,13 @@ int coresg_hip_exact_mst(coresg_hip_graph* h, uint32_t mpts, coresg_hip_mst_info
     });
 }
 
+int coresg_hip_graph_derive_hubness(coresg_hip_graph* h, const coresg_hip_build_params* p) {
+    return guarded([&] {
+        bind(h->g);
+        derive_hubness(h->g, *p);
+    });
+}
+
 int coresg_hip_autotune_sweep(coresg_hip_graph* h, const uint32_t* mpts, uint32_t count,
                               coresg_hip_tune_result* out) {
     return guarded([&] {

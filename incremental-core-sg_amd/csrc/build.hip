@@ -185,8 +185,8 @@ void repair_connectivity(Graph& g, const coresg_hip_build_params& p, coresg_hip_
 std::uint64_t count_knn_edges(Graph& g);  // extract.hip
 
 void derive_hubness(Graph& g, const coresg_hip_build_params& p) {
-    if (!g.knn_set || !g.points_set)
-        throw std::runtime_error("derive_hubness: points and k-NN lists must be set");
+    if (!g.knn_set) throw std::runtime_error("derive_hubness: k-NN lists must be set");
+    g.score_k = p.score_k;
     std::uint32_t tagged = 0;
     hubness_compute(g, &tagged);
     if (g.inst_off.empty()) {
@@ -201,6 +201,7 @@ void build_bridges(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_
         throw std::runtime_error("build_bridges: points and k-NN lists must be set");
     const auto t0 = Clock::now();
     std::uint32_t tagged = 0;
+    g.score_k = p.score_k;
     hubness_compute(g, &tagged);
     g.bu.clear();
     g.bv.clear();

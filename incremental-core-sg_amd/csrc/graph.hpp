@@ -311,6 +311,7 @@ struct Graph {
     DBuf<double> score;
     DBuf<std::uint32_t> conn;
     std::uint32_t num_components = 0;
+    std::uint32_t score_k = 0;  // k of the score's degenerate case (0 = k)
     std::vector<std::uint32_t> selected;
     bool hub_ready = false;
 

@@ -475,7 +475,8 @@ void hubness_compute(Graph& g, std::uint32_t* tagged_out) {
     {
         ScopedKernelTimer t(g, "hub_score");
         score_kernel<<<capped_blocks(std::size_t(n) * lanes, kThreads), kThreads, 0, s>>>(
-            g.ids.p, g.count.p, n, k, k, g.rdeg.p, lanes, g.score.p, d_tagged);
+            g.ids.p, g.count.p, n, k, g.score_k ? g.score_k : k, g.rdeg.p, lanes, g.score.p,
+            d_tagged);
         CSG_LAUNCH_CHECK();
     }
     std::uint32_t h[2] = {0, 0};

@@ -175,6 +175,9 @@ coresg_hip_graph* load_graph(int device, const char* path) {
     g.missing_mode = mode;
     coresg_hip_build_params p;
     coresg_hip_default_build_params(&p);
+    // a CoreSG constructed with default parameters sets k_for_score to their
+    // k_max (32, coresg.cpp:51) and load() keeps it whatever the file's k_max
+    p.score_k = 32;
     derive_hubness(g, p);
     return h.release();
 }

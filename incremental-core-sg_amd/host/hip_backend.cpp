@@ -221,6 +221,46 @@ public:
         return sg.exact_mst(mpts);
     }
 
+    Hubness hubness(const KnnGraph& knn, std::uint32_t budget_dim,
+                    const HubnessParams& params) override {
+        Hubness out;
+        const auto n = static_cast<std::uint32_t>(knn.size());
+        if (n == 0) return out;
+        // the device scores with the arithmetic mean (the default rule)
+        if (params.normalization != ScoreNormalization::ArithmeticMean)
+            throw std::runtime_error("hip hubness: only the arithmetic-mean score is on the device");
+        coresg_hip_graph* g = coresg_hip_graph_create(device_for(0), n, budget_dim ? budget_dim : 1,
+                                                      knn.k());
+        hip_check(g != nullptr, "coresg_hip_graph_create");
+        std::unique_ptr<coresg_hip_graph, void (*)(coresg_hip_graph*)> guard(
+            g, coresg_hip_graph_destroy);
+        hip_check(coresg_hip_graph_set_knn(g, knn.ids_storage().data(), knn.dist_storage().data(),
+                                           knn.flag_storage().data(),
+                                           knn.count_storage().data(), CORESG_HIP_HOST),
+                  "set_knn");
+        coresg_hip_build_params p;
+        coresg_hip_default_build_params(&p);
+        p.antihub_multiplier = params.antihub_multiplier;
+        p.max_pair_ops = params.max_pair_ops;
+        p.min_selected = params.min_selected;
+        p.max_selected = params.max_selected;
+        p.min_per_component = params.min_per_component;
+        p.score_k = params.k_for_score;
+        hip_check(coresg_hip_graph_derive_hubness(g, &p), "derive_hubness");
+        std::vector<std::int32_t> rd(n);
+        out.score.resize(n);
+        out.component.resize(n);
+        std::uint32_t nc = 0;
+        hip_check(coresg_hip_graph_get_hubness(g, rd.data(), out.score.data(),
+                                               out.component.data(), &nc),
+                  "get_hubness");
+        out.reverse_degree.assign(rd.begin(), rd.end());
+        out.components = nc;
+        out.selected.resize(coresg_hip_graph_get_selected(g, nullptr, 0));
+        coresg_hip_graph_get_selected(g, out.selected.data(), out.selected.size());
+        return out;
+    }
+
 private:
     int device_for(unsigned slot) const { return ids_[slot < ids_.size() ? slot : 0]; }
 

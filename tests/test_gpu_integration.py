@@ -117,3 +117,22 @@ def test_reference_coresg_backend_semantics(mode):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert f"backend_check {mode}: passed" in r.stdout
+
+
+HUBNESS_CHECK = os.path.join(REPO, "oracle", "_ref", "hubness_check")
+
+
+@pytest.mark.skipif(not os.path.exists(HUBNESS_CHECK), reason="integration build not present")
+@pytest.mark.parametrize("name,quota", [("mix_d16", 2), ("mix_d24", 2), ("islands", 2),
+                                        ("islands_q0", 0), ("ties", 2)])
+def test_reference_coresg_hubness_capability(name, quota, tmp_path):
+    """The hubness half of CoreSG::build through HipBackend's DeviceExtract
+    capability equals what the reference's HubnessIndex and
+    ConnectivityTracker hold after load (tests/cpp/hubness_check.cpp)."""
+    print(f"[gpu] hubness capability: {name} quota {quota}", flush=True)
+    path = tmp_path / "g.coresg"
+    golden(name)["coresg_bytes"].tofile(path)
+    r = subprocess.run([HUBNESS_CHECK, str(path), str(quota)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "-> identical (backend hip)" in r.stdout
