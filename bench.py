@@ -399,7 +399,8 @@ def main():
                          "+ Poisson extract queries)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the run, copy 1 GiB device-to-device three times (a plain "
-                         "copy kernel of known size, the scale for HBM byte counters)")
+                         "copy kernel of known size, the scale for HBM byte counters) and "
+                         "run three random-row gather probes of known bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -445,7 +446,10 @@ def main():
         step(None)
         print(f"[bench] warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
     # the sweep's launch shapes, autotuned on this graph for the batched sweep
-    # (32 slots), the 16-value sweep and single-mpts queries
+    # (32 slots), the 16-value sweep and single-mpts queries (an untimed build
+    # first when there was no warmup step)
+    if args.warmup == 0:
+        g.build(params)
     g.prepare_edges()
     tuned = {}
     for name, mp in (("sweep32", mpts), ("sweep16", list(range(2, k + 1, 2))[:16]),
@@ -654,6 +658,12 @@ def main():
         hip.hipFree(a_)
         hip.hipFree(b_)
         print("[bench] copy probe: 3 x 1 GiB device-to-device", file=sys.stderr, flush=True)
+        # random whole-row gathers of a 1M x 24 array (96 MB, the join's point
+        # rows): 3 x 32M rows of 96 B = 3.07 GB read, 128 MB written per run
+        for _ in range(3):
+            ms = chip.probe_gather(1_000_000, 24, 32 << 20)
+        print(f"[bench] gather probe: 3 x 32M random 96-byte rows of a 96 MB array, "
+              f"{ms:.2f} ms each", file=sys.stderr, flush=True)
 
 
 if __name__ == "__main__":

@@ -323,6 +323,16 @@ size_t coresg_hip_tune_table(char* buf, size_t cap);
 /* Adds the lines of such a table (a table saved by an earlier process). */
 int coresg_hip_tune_table_load(const char* text);
 
+/* ---- measurement ---------------------------------------------------------- */
+
+/* A probe of known algorithmic bytes for calibrating hardware byte counters
+ * against the path's access pattern: `gathers` reads of whole random rows of a
+ * rows x stride float array on `device` (stride * 4 bytes each, the pattern of
+ * the local join's and the insert search's row reads), each writing one float.
+ * *ms receives the kernel's device time. No counterpart in the reference. */
+int coresg_hip_probe_gather(int device, size_t rows, uint32_t stride, size_t gathers,
+                            double* ms);
+
 /* ---- 3. serving session -------------------------------------------------- */
 
 /* One device-resident CORE-SG served by continuous batching. The reference

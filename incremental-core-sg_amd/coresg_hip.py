@@ -139,6 +139,8 @@ SIGNATURES = [
     ("coresg_hip_sampled_recall", C.c_int, [_vp, C.c_size_t, C.c_uint64,
                                             C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("coresg_hip_graph_set_plain", C.c_int, [_vp, C.c_int]),
+    ("coresg_hip_probe_gather", C.c_int, [C.c_int, C.c_size_t, C.c_uint32, C.c_size_t,
+                                          C.POINTER(C.c_double)]),
     ("coresg_hip_graph_derive_hubness", C.c_int, [_vp, C.POINTER(BuildParams)]),
     ("coresg_hip_autotune_sweep", C.c_int, [_vp, _vp, C.c_uint32, C.POINTER(TuneResult)]),
     ("coresg_hip_tune_table", C.c_size_t, [C.c_char_p, C.c_size_t]),
@@ -490,6 +492,13 @@ def mrd_block(pts_padded: np.ndarray, dim: int, core: np.ndarray, r0: int = 0,
     check(lib().coresg_hip_mrd_block(device, ptr(pts), m, dim, stride, metric, ptr(core), r0, r1,
                                      ptr(out)))
     return out.reshape(r1 - r0, m)
+
+
+def probe_gather(rows: int, stride: int, gathers: int, device: int = 0) -> float:
+    """Device ms of `gathers` random whole-row reads of a rows x stride array."""
+    ms = C.c_double()
+    check(lib().coresg_hip_probe_gather(device, rows, stride, gathers, C.byref(ms)))
+    return ms.value
 
 
 def tune_table() -> str:
