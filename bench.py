@@ -255,12 +255,18 @@ def exact_mst_run(chip, datagen, coresg_io, cpu_n=5000):
     return out
 
 
+MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense FP16 / BF16 matrix-core peak
+
+
 def mixed_precision_run(chip, datagen, n=2_000_000, d=128, k=32, mpts=(4, 16, 32)):
     """The mixed-precision distance path: a 64-component anisotropic mixture
     (eigenvalues log-uniform in [0.01, 4]), built with the exact FP32 join and
-    with the BF16 Gram filter (matrix cores, every retained edge recomputed in
-    exact FP32); the BF16 build's fast distances are audited against FP64 on
-    every visited pair, and mpts = 4, 16, 32 are extracted from each graph."""
+    with the FP16 and BF16 Gram filters (gfx950 16x16x32 matrix cores, every
+    pair the bound cannot rule out computed in exact FP32); per filter, a
+    profiled build gives the matrix-core work (16 x 16 tiles x 32-coordinate
+    steps, 2 * 16 * 16 * 32 FLOP each) over the join's device time; the BF16
+    build's fast distances are audited against FP64 on every visited pair,
+    and mpts = 4, 16, 32 are extracted from each graph."""
     import struct
     f32 = lambda b: struct.unpack("f", struct.pack("I", b & 0xFFFFFFFF))[0]
     t0 = time.perf_counter()
@@ -271,7 +277,7 @@ def mixed_precision_run(chip, datagen, n=2_000_000, d=128, k=32, mpts=(4, 16, 32
     del x
     out = dict(workload=f"anisotropic-mixture-{n // 1_000_000}m-d{d}", n=n, d=d, k_max=k,
                data_gen_s=gen_s)
-    for name, gram in (("exact_fp32", 1), ("bf16_gram_filter", 3)):
+    for name, gram in (("exact_fp32", 1), ("fp16_gram_filter", 2), ("bf16_gram_filter", 3)):
         p = chip.build_params(sample=16, iters=10, gram=gram)
         g.build(p)  # warm
         t0 = time.perf_counter()
@@ -285,8 +291,25 @@ def mixed_precision_run(chip, datagen, n=2_000_000, d=128, k=32, mpts=(4, 16, 32
         out[name] = dict(build_ms=build_s * 1e3, pts_per_s=n / build_s, sweep_ms=sweep_s * 1e3,
                          recall_sampled=rec, rounds=st["rounds"],
                          mst_weights={int(m): r["weight"] for m, r in zip(mpts, res)})
+        # the join's device time and work in a profiled build
+        g.reset_timers()
+        g.set_profiling(True)
+        g.build(p)
+        g.set_profiling(False)
+        jms, jn = g.kernel_time("nnd_join")
+        out[name]["join_ms_per_launch"] = jms
+        if gram != 1:
+            tiles, ksteps = g.kernel_work("nnd_mfma")[:2]
+            pairs, exact = g.kernel_work("nnd_gram")[:2]
+            flops = tiles * ksteps * 2 * 16 * 16 * 32 / max(1, jn)
+            tflops = flops / (jms * 1e-3) / 1e12 if jms else None
+            out[name]["mfma"] = dict(
+                tiles_per_launch=tiles / max(1, jn), k_steps=ksteps, flops_per_launch=flops,
+                tflops=tflops, peak_tflops=MFMA_PEAK_TFLOPS,
+                frac=(tflops / MFMA_PEAK_TFLOPS) if tflops else None,
+                pairs_per_launch=pairs / max(1, jn), exact_fraction=exact / max(1, pairs))
         print(f"[bench] mixed precision {name}: build {build_s * 1e3:.1f} ms, "
-              f"recall {rec:.4f}", file=sys.stderr, flush=True)
+              f"join {jms:.2f} ms/launch, recall {rec:.4f}", file=sys.stderr, flush=True)
     # audit of the BF16 fast distances (FP64 per visited pair; slow, untimed)
     g.reset_timers()
     g.set_profiling(True)

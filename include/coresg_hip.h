@@ -148,11 +148,14 @@ typedef struct coresg_hip_build_params {
     /* local join: per-node proposal buffer capacity per round (16..512;
      * 0 = library default, 512) */
     uint32_t join_capacity;
-    /* local-join distances: 0 = library choice, 1 = exact FP32 for every
-     * pair, 2 = FP16 Gram on the matrix cores as a filter, 3 = BF16 Gram as a
-     * filter. The filters use a rigorous error bound to skip only pairs that
-     * cannot pass either threshold test and compute every other pair in exact
-     * FP32, so all modes propose the same candidates. */
+    /* local-join distances: 0 = library choice (exact for rows of up to 64
+     * floats, the FP16 filter for wider rows), 1 = exact FP32 for every
+     * pair, 2 = FP16 Gram on the matrix cores (gfx950 16x16x32) as a filter,
+     * 3 = BF16 Gram as a filter. The filters use a rigorous error bound to
+     * skip only pairs that cannot pass either threshold test and compute every
+     * other pair in exact FP32, so all modes propose the same candidates.
+     * Strides the filter join is not built for (other than 8, 16, 24, 32, 48,
+     * 64, 96, 128) take the exact join. */
     uint32_t gram;
     /* bridge set (MissingEdgeMode, coresg.hpp:54-58; coresg.cpp:143-169):
      * 0 = AntiHub (default), 1 = ExactMst (the exact MST at k_max, stored
@@ -332,6 +335,12 @@ int coresg_hip_tune_table_load(const char* text);
  * *ms receives the kernel's device time. No counterpart in the reference. */
 int coresg_hip_probe_gather(int device, size_t rows, uint32_t stride, size_t gathers,
                             double* ms);
+/* The lane layout of the matrix-core instruction the filter join uses
+ * (gfx950 v_mfma_f32_16x16x32_{f16,bf16}): c (16 x 16) = a b^T for a, b of
+ * 16 x 32 floats (row-major, rounded to BF16 when bf16 != 0, else FP16),
+ * computed by one wavefront with the join's operand and result layout. No
+ * counterpart in the reference. */
+int coresg_hip_probe_mfma(int device, int bf16, const float* a, const float* b, float* c);
 
 /* ---- 3. serving session -------------------------------------------------- */
 

@@ -141,6 +141,7 @@ SIGNATURES = [
     ("coresg_hip_graph_set_plain", C.c_int, [_vp, C.c_int]),
     ("coresg_hip_probe_gather", C.c_int, [C.c_int, C.c_size_t, C.c_uint32, C.c_size_t,
                                           C.POINTER(C.c_double)]),
+    ("coresg_hip_probe_mfma", C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp]),
     ("coresg_hip_graph_derive_hubness", C.c_int, [_vp, C.POINTER(BuildParams)]),
     ("coresg_hip_autotune_sweep", C.c_int, [_vp, _vp, C.c_uint32, C.POINTER(TuneResult)]),
     ("coresg_hip_tune_table", C.c_size_t, [C.c_char_p, C.c_size_t]),
@@ -499,6 +500,18 @@ def probe_gather(rows: int, stride: int, gathers: int, device: int = 0) -> float
     ms = C.c_double()
     check(lib().coresg_hip_probe_gather(device, rows, stride, gathers, C.byref(ms)))
     return ms.value
+
+
+def probe_mfma(a: np.ndarray, b: np.ndarray, bf16: bool, device: int = 0) -> np.ndarray:
+    """a @ b.T for 16 x 32 operands on one wavefront of the filter join's
+    matrix-core instruction and lane layout."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    if a.shape != (16, 32) or b.shape != (16, 32):
+        raise ValueError("probe_mfma: operands are 16 x 32")
+    c = np.empty((16, 16), np.float32)
+    check(lib().coresg_hip_probe_mfma(device, int(bool(bf16)), ptr(a), ptr(b), ptr(c)))
+    return c
 
 
 def tune_table() -> str:

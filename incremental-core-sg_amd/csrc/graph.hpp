@@ -204,7 +204,7 @@ struct EventPair {
 struct RoundReadback {
     unsigned long long upd;
     unsigned long long tally[3];       // join work (profiling)
-    unsigned long long gram_tally[2];  // Gram-filter work (profiling)
+    unsigned long long gram_tally[3];  // Gram-filter work: pairs, exact, tiles (profiling)
     unsigned int gram_audit[2];        // Gram-filter audit maxima (profiling)
 };
 

@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "build.hpp"
+#include "mfma.hpp"
 #include "scan.hpp"
 
 namespace coresg_hip {
@@ -55,6 +56,8 @@ constexpr std::uint32_t kRevMaxBuckets = 8192;
 constexpr unsigned kRevBlocks = 2048;
 constexpr unsigned kRevThreads = 1024;
 constexpr std::uint32_t kJoinRegStride = 64;  // widest row a join thread keeps in registers
+constexpr std::uint32_t kMfmaThreads = 256;   // matrix-core filter join: threads per node
+constexpr std::uint32_t kMfmaList = 1536;     // pairs listed for exact distances per batch
 
 // ---------------------------------------------------------------------------
 // init_random
@@ -535,6 +538,7 @@ __device__ __forceinline__ float l2sqr_reg(const float (&a)[STRIDE],
 
 // Stage the node's members (ids, thresholds, rows). Returns false (and touches
 // nothing) for a node without new members.
+template <std::uint32_t NT = kJoinThreads>
 __device__ __forceinline__ bool join_stage(const JoinArgs& a, std::uint32_t v, std::uint32_t S,
                                            std::uint32_t stride, std::uint32_t* s_id,
                                            float* s_thr, float* s_rows, std::uint32_t* na_out,
@@ -545,13 +549,13 @@ __device__ __forceinline__ bool join_stage(const JoinArgs& a, std::uint32_t v, s
     *na_out = na;
     *ns_out = ns;
     if (na == 0) return false;  // block-uniform
-    for (std::uint32_t i = tid; i < ns; i += kJoinThreads) {
+    for (std::uint32_t i = tid; i < ns; i += NT) {
         s_id[i] = a.ids[std::size_t(v) * S + i];
         s_thr[i] = a.thr[std::size_t(v) * S + i];
     }
     __syncthreads();
     const std::uint32_t q4 = stride / 4;
-    for (std::uint32_t e = tid; e < ns * q4; e += kJoinThreads) {
+    for (std::uint32_t e = tid; e < ns * q4; e += NT) {
         const std::uint32_t i = e / q4, f = e - i * q4;
         reinterpret_cast<float4*>(s_rows)[e] =
             reinterpret_cast<const float4*>(a.pts + std::size_t(s_id[i]) * stride)[f];
@@ -817,70 +821,6 @@ __global__ void __launch_bounds__(kJoinThreads) join_reg_kernel(JoinArgs a) {
 // exactly those of the exact join: the matrix cores only decide which exact
 // distances are needed.
 
-typedef _Float16 gram_f16x4 __attribute__((ext_vector_type(4)));
-typedef short gram_i16x4 __attribute__((ext_vector_type(4)));
-typedef float gram_f32x4 __attribute__((ext_vector_type(4)));
-
-// fp32 -> bf16 bits, round to nearest even (finite inputs).
-__device__ __forceinline__ std::uint32_t bf16_bits(float x) {
-    std::uint32_t u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return u >> 16;
-}
-
-template <bool BF16>
-__device__ __forceinline__ float gram_round(float x) {
-    if constexpr (BF16) return __uint_as_float(bf16_bits(x) << 16);
-    else return static_cast<float>(static_cast<_Float16>(x));
-}
-
-template <bool BF16>
-struct GramConst {
-    // unit roundoff of the format plus the centring's fp32 rounding
-    static constexpr float rho = BF16 ? (0x1p-8f + 0x1p-23f) : (0x1p-11f + 0x1p-23f);
-    // per-coordinate absolute floor (a flushed subnormal)
-    static constexpr float absc = BF16 ? 0x1p-126f : 0x1p-14f;
-};
-
-template <bool BF16>
-__device__ __forceinline__ gram_f32x4 gram_mfma(const float (&x)[4], const float (&y)[4],
-                                                gram_f32x4 acc) {
-    if constexpr (BF16) {
-        gram_i16x4 av, bv;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            av[q] = static_cast<short>(bf16_bits(x[q]));
-            bv[q] = static_cast<short>(bf16_bits(y[q]));
-        }
-        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
-    } else {
-        gram_f16x4 av, bv;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            av[q] = static_cast<_Float16>(x[q]);
-            bv[q] = static_cast<_Float16>(y[q]);
-        }
-        return __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv, acc, 0, 0, 0);
-    }
-}
-
-// Four centred coordinates k..k+3 of LDS row r (zeros past the stride or for
-// rows beyond `rows`).
-__device__ __forceinline__ void centred4(const float* s_rows, std::uint32_t stride,
-                                         std::uint32_t r, std::uint32_t rows, std::uint32_t k,
-                                         float (&out)[4]) {
-    if (r < rows && k < stride) {
-        const float4 x = *reinterpret_cast<const float4*>(s_rows + std::size_t(r) * stride + k);
-        const float4 c = *reinterpret_cast<const float4*>(s_rows + k);
-        out[0] = x.x - c.x;
-        out[1] = x.y - c.y;
-        out[2] = x.z - c.z;
-        out[3] = x.w - c.w;
-    } else {
-        out[0] = out[1] = out[2] = out[3] = 0.0f;
-    }
-}
-
 __device__ __forceinline__ unsigned int wave_max_u32(unsigned int v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -890,123 +830,258 @@ __device__ __forceinline__ unsigned int wave_max_u32(unsigned int v) {
     return v;
 }
 
-// LDS: ids[S], thr[S], rows[S*stride], D[A][S+1], N[S] (squared norms of the
-// rounded centred rows), nb[S] (outward bound on both norms of a row), cnt[S],
-// base[S].
-template <std::uint32_t STRIDE, bool BF16>
-__global__ void __launch_bounds__(kJoinThreads) join_gram_kernel(JoinArgs a) {
+// Eight coordinates k..k+7 of LDS row r centred on row 0 (zeros past the
+// stride or for rows at or beyond `rows`).
+template <std::uint32_t STRIDE>
+__device__ __forceinline__ void centred8(const float* s_rows, std::uint32_t r, std::uint32_t rows,
+                                         std::uint32_t k, float (&out)[8]) {
+    if (r < rows && k < STRIDE) {
+        const float4* x = reinterpret_cast<const float4*>(s_rows + __umul24(r, STRIDE) + k);
+        const float4* c = reinterpret_cast<const float4*>(s_rows + k);
+        const float4 x0 = x[0], x1 = x[1], c0 = c[0], c1 = c[1];
+        out[0] = x0.x - c0.x;
+        out[1] = x0.y - c0.y;
+        out[2] = x0.z - c0.z;
+        out[3] = x0.w - c0.w;
+        out[4] = x1.x - c1.x;
+        out[5] = x1.y - c1.y;
+        out[6] = x1.z - c1.z;
+        out[7] = x1.w - c1.w;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) out[q] = 0.0f;
+    }
+}
+
+// Matrix-core filter join. One 256-thread group (four wavefronts) per node.
+//
+// 1. Staging as in the exact joins; per member, the squared norm N of its
+//    centred row rounded to FP16 / BF16, an outward bound nb on the norms of
+//    the centred row and of its rounding, and st = sqrt(thr / (1 - eta))
+//    rounded up (the distance a pair must exceed to fail the member's
+//    try_pair test, whatever the exact sum's rounding).
+// 2. Gram tiles of 16 new rows x 16 members, one 16x16x32 MFMA per 32
+//    coordinates, D~ = N_i + N_j - 2 G. With S the distance of the rounded
+//    rows and T the true one, |S - T| <= delta = rho (nb_i + nb_j) + 2 abs
+//    and |D~ - S^2| <= eps = cG (nb_i + nb_j)^2, so D~ - eps >= (delta +
+//    max(st_i, st_j))^2 proves that the exact distance fails both tests.
+//    Every other live pair (the exact joins' visited pairs: new x all, each
+//    new x new pair once) is appended to an LDS list by ballot prefix.
+// 3. The listed pairs get their exact distance, densely over the threads
+//    (the reference's summation order), and their proposals are counted.
+// 4. One reservation per receiving member, then every listed pair's keys are
+//    written from the stored distance: nothing is computed twice.
+// The proposals are exactly the exact joins'. Tiles are taken in batches
+// whose pair count fits the list (one batch for all but the largest nodes).
+// LDS: ids[S], thr[S], cnt[S], base[S], (N, nb, st, -)[S], rows[S*STRIDE],
+// list idx u16[L], list distance f32[L], list length.
+// Nodes per CU the filter join is compiled for: its LDS per node at sample
+// 16 allows about this many; the register budget follows from it.
+constexpr unsigned mfma_join_blocks(std::uint32_t stride) {
+    return stride <= 24 ? 6u : stride <= 48 ? 4u : stride <= 96 ? 3u : 2u;
+}
+
+// AUDIT: the FP64 check of every live pair (a.audit set) is compiled in.
+template <std::uint32_t STRIDE, bool BF16, bool AUDIT>
+__global__ void __launch_bounds__(kMfmaThreads, mfma_join_blocks(STRIDE))
+    join_mfma_kernel(JoinArgs a) {
+    static_assert(STRIDE % 8 == 0, "rows padded to 8 floats");
+    constexpr std::uint32_t KP = (STRIDE + 31u) & ~31u;
     extern __shared__ float lds[];
     if (*a.halt) return;
-    const std::uint32_t S = 6u * a.smp, A = 2u * a.smp, P = S + 1;
-    const std::uint32_t stride = STRIDE ? STRIDE : a.stride;
-    const std::uint32_t KP = (stride + 15u) & ~15u;
+    const std::uint32_t S = 6u * a.smp;
     std::uint32_t* s_id = reinterpret_cast<std::uint32_t*>(lds);
     float* s_thr = lds + S;
-    float* s_rows = lds + 2 * S;
-    float* s_D = s_rows + std::size_t(S) * stride;
-    float* s_N = s_D + std::size_t(A) * P;
-    float* s_nb = s_N + S;
-    std::uint32_t* s_cnt = reinterpret_cast<std::uint32_t*>(s_nb + S);
+    std::uint32_t* s_cnt = reinterpret_cast<std::uint32_t*>(lds + 2 * S);
     std::uint32_t* s_base = s_cnt + S;
+    float4* s_m4 = reinterpret_cast<float4*>(lds + 4 * S);  // S multiple of 2: 16-B aligned
+    float* s_rows = lds + 8 * S;
+    float* s_ld = s_rows + std::size_t(S) * STRIDE;
+    std::uint16_t* s_li = reinterpret_cast<std::uint16_t*>(s_ld + kMfmaList);
+    std::uint32_t* s_len = reinterpret_cast<std::uint32_t*>(s_li + kMfmaList);
     const std::uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-    const float rho = GramConst<BF16>::rho;
+    const float rho = GramConst<BF16>::rho * (1.0f + 0x1p-16f);
+    const float abs2 = 2.0f * GramConst<BF16>::absc * sqrtf(float(KP)) * 1.01f * (1.0f + 0x1p-16f);
     const float abs_norm = GramConst<BF16>::absc * sqrtf(float(KP)) * 1.01f;
-    const float cG = 2.0f * float(KP + 8) * 0x1p-24f;
-    const float eta = float(stride + 24) * 0x1p-24f + 0x1p-18f;
+    const float cG = 2.0f * float(KP + 8) * 0x1p-24f * (1.0f + 0x1p-16f);
+    const float eta = float(STRIDE + 24) * 0x1p-24f + 0x1p-18f;
     const float nscale = 1.0f + float(KP + 4) * 0x1p-24f;
-    unsigned long long n_pairs = 0, n_exact = 0;
+    const float sscale = (1.0f + 0x1p-20f) / (1.0f - eta);
+    unsigned long long n_pairs = 0, n_exact = 0, n_tiles = 0;
     unsigned int worst_rel = 0, worst_ratio = 0;
     for (std::uint32_t v = blockIdx.x; v < a.n; v += gridDim.x) {
         std::uint32_t na, ns;
-        for (std::uint32_t x = tid; x < S; x += kJoinThreads) s_cnt[x] = 0;
-        if (!join_stage(a, v, S, stride, s_id, s_thr, s_rows, &na, &ns)) continue;
-        // 1. squared norms of the rounded centred rows and their outward bound
-        for (std::uint32_t j = tid; j < ns; j += kJoinThreads) {
+        if (!join_stage<kMfmaThreads>(a, v, S, STRIDE, s_id, s_thr, s_rows, &na, &ns)) continue;
+        // 1. member constants
+        for (std::uint32_t j = tid; j < ns; j += kMfmaThreads) {
             float N = 0.0f;
-            for (std::uint32_t k = 0; k < stride; k += 4) {
-                float c4[4];
-                centred4(s_rows, stride, j, ns, k, c4);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const float h = gram_round<BF16>(c4[q]);
+            for (std::uint32_t k = 0; k < STRIDE; k += 8) {
+                float c8[8];
+                centred8<STRIDE>(s_rows, j, ns, k, c8);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const float h = gram_round<BF16>(c8[q]);
                     N = __builtin_fmaf(h, h, N);
                 }
             }
-            s_N[j] = N;
-            s_nb[j] = (sqrtf(N) * nscale + abs_norm) * (1.0f + 2.0f * rho);
+            const float nb = (sqrtf(N) * nscale + abs_norm) * (1.0f + 2.0f * GramConst<BF16>::rho);
+            const float st = sqrtf(s_thr[j] * sscale) * (1.0f + 0x1p-20f);
+            s_m4[j] = make_float4(N, nb, st, 0.0f);
         }
-        __syncthreads();
-        // 2. Gram tiles of 16 new rows x 16 members; wavefront w takes tiles
-        // w, w+2, ... Lane (li, lk) supplies row/column li, coordinates
-        // lk..lk+3 of each 16-wide step and receives G[4*(lane/16)+q][li].
-        const std::uint32_t rt = (na + 15u) / 16u, ct = (ns + 15u) / 16u;
-        const std::uint32_t li = lane & 15u, lk = 4u * (lane >> 4);
-        for (std::uint32_t t = w; t < rt * ct; t += 2) {
-            const std::uint32_t r0 = 16u * (t / ct), c0 = 16u * (t % ct);
-            gram_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            for (std::uint32_t k0 = 0; k0 < KP; k0 += 16) {
-                float x[4], y[4];
-                centred4(s_rows, stride, r0 + li, na, k0 + lk, x);
-                centred4(s_rows, stride, c0 + li, ns, k0 + lk, y);
-                acc = gram_mfma<BF16>(x, y, acc);
-            }
-            const std::uint32_t j = c0 + li;
+        const std::uint32_t rt = (na + 15u) / 16u, ct = (ns + 15u) / 16u, T = rt * ct;
+        // pairs a tile can hold: none when every member column is new and at
+        // or below every row (new x new pairs are taken with i < j)
+        // t / ct by a rounded-up 16-bit reciprocal: exact for t < 2^16 / ct^2
+        // (T <= 4 * 12 tiles at sample 32)
+        const std::uint32_t ct_inv = (65536u + ct - 1u) / ct;
+        auto tile_row = [&](std::uint32_t t) -> std::uint32_t { return (t * ct_inv) >> 16; };
+        auto tile_cap = [&](std::uint32_t t) -> std::uint32_t {
+            const std::uint32_t tr = tile_row(t);
+            const std::uint32_t r0 = 16u * tr, c0 = 16u * (t - tr * ct);
+            if (c0 + 16u <= na && c0 + 15u <= r0) return 0u;
+            const std::uint32_t rr = na - r0 < 16u ? na - r0 : 16u;
+            const std::uint32_t cc = ns - c0 < 16u ? ns - c0 : 16u;
+            return rr * cc;
+        };
+        for (std::uint32_t t0 = 0; t0 < T;) {
+            std::uint32_t t1 = t0, fill = 0;
+            while (t1 < T && fill + tile_cap(t1) <= kMfmaList) fill += tile_cap(t1++);
+            for (std::uint32_t x = tid; x < ns; x += kMfmaThreads) s_cnt[x] = 0;
+            if (tid == 0) *s_len = 0;
+            __syncthreads();
+            // 2. filter: wavefront w takes tiles t0 + w, + 4, ...
+            const std::uint32_t li = lane & 15u, kq = lane >> 4;
+            for (std::uint32_t t = t0 + w; t < t1; t += kMfmaThreads / 64u) {
+                if (tile_cap(t) == 0) continue;  // wave-uniform
+                const std::uint32_t tr = tile_row(t);
+                const std::uint32_t r0 = 16u * tr, c0 = 16u * (t - tr * ct);
+                if (a.tally && lane == 0) ++n_tiles;
+                gram_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const std::uint32_t i = r0 + lk + static_cast<std::uint32_t>(q);
-                if (i < na && j < ns) s_D[i * P + j] = (s_N[i] + s_N[j]) - 2.0f * acc[q];
-            }
-        }
-        __syncthreads();
-        // 3. pair pass: +inf where the bound rules out both tests, else the
-        // exact distance
-        PairWalk w(tid, kJoinThreads, ns, na);
-        for (std::uint32_t p = tid; p < na * ns; p += kJoinThreads, w.next()) {
-            if (!w.live()) continue;
-            const std::uint32_t i = w.i, j = w.j;
-            const float dt = s_D[__umul24(i, P) + j];
-            const float nbs = s_nb[i] + s_nb[j];
-            float tlo = 0.0f;
-            const float s2 = dt - cG * nbs * nbs;
-            if (s2 > 0.0f) {
-                const float slo = sqrtf(s2) * (1.0f - 0x1p-20f);
-                const float delta = (rho * nbs + 2.0f * abs_norm) * (1.0f + 0x1p-16f) + slo * 0x1p-21f;
-                tlo = slo - delta;
-            }
-            bool reject = false;
-            if (tlo > 0.0f) {
-                const float rlo = tlo * tlo * (1.0f - eta);
-                reject = rlo >= s_thr[i] && rlo >= s_thr[j];
-            }
-            const float* ri = s_rows + __umul24(i, stride);
-            const float* rj = s_rows + __umul24(j, stride);
-            const float d = reject ? __builtin_inff() : l2sqr_exact(ri, rj, stride);
-            s_D[__umul24(i, P) + j] = d;
-            if (d < s_thr[i]) atomicAdd(&s_cnt[i], 1u);
-            if (d < s_thr[j]) atomicAdd(&s_cnt[j], 1u);
-            if (a.tally) {
-                ++n_pairs;
-                n_exact += reject ? 0u : 1u;
-            }
-            if (a.audit) {
-                double t2 = 0.0;
-                for (std::uint32_t k = 0; k < stride; ++k) {
-                    const double e = double(ri[k]) - double(rj[k]);
-                    t2 += e * e;
+                for (std::uint32_t k0 = 0; k0 < KP; k0 += 32) {
+                    float x[8], y[8];
+                    centred8<STRIDE>(s_rows, r0 + li, na, k0 + 8u * kq, x);
+                    centred8<STRIDE>(s_rows, c0 + li, ns, k0 + 8u * kq, y);
+                    acc = gram_mfma32<BF16>(x, y, acc);
                 }
-                const double T = sqrt(t2);
-                if (T > 0.0) {
-                    const double fast = sqrt(dt > 0.0f ? double(dt) : 0.0);
-                    const float rel = static_cast<float>(fabs(fast - T) / T);
-                    const float ratio = static_cast<float>(double(tlo > 0.0f ? tlo : 0.0f) / T);
-                    worst_rel = max(worst_rel, __float_as_uint(rel));
-                    worst_ratio = max(worst_ratio, __float_as_uint(ratio));
+                const std::uint32_t j = c0 + li;
+                const float4 mj = s_m4[j < ns ? j : 0];
+                bool keep[4];
+                std::uint32_t total = 0;
+                std::uint64_t bal[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const std::uint32_t i = r0 + 4u * kq + static_cast<std::uint32_t>(q);
+                    const bool live = i < na && j < ns && !(j < na && j <= i);
+                    bool kp = false;
+                    if (live) {
+                        const float4 mi = s_m4[i];
+                        const float dt = (mi.x + mj.x) - 2.0f * acc[q];
+                        const float nbs = mi.y + mj.y;
+                        const float qd = __builtin_fmaf(rho, nbs, abs2) + fmaxf(mi.z, mj.z);
+                        const float lhs = dt - cG * nbs * nbs;
+                        const bool reject = lhs >= qd * qd * (1.0f + 0x1p-18f);
+                        kp = !reject;
+                        if (a.tally) {
+                            ++n_pairs;
+                            n_exact += kp ? 1u : 0u;
+                        }
+                        if (AUDIT && a.audit) {
+                            const float* ri = s_rows + __umul24(i, STRIDE);
+                            const float* rj = s_rows + __umul24(j, STRIDE);
+                            double t2 = 0.0;
+                            for (std::uint32_t k = 0; k < STRIDE; ++k) {
+                                const double e = double(ri[k]) - double(rj[k]);
+                                t2 += e * e;
+                            }
+                            const double Tt = sqrt(t2);
+                            if (Tt > 0.0) {
+                                const double fast = sqrt(dt > 0.0f ? double(dt) : 0.0);
+                                const double s2 = double(dt) - double(cG) * nbs * nbs;
+                                const double delta = double(rho) * nbs + abs2;
+                                const double tlo = (s2 > 0.0 ? sqrt(s2) : 0.0) - delta;
+                                const float rel = static_cast<float>(fabs(fast - Tt) / Tt);
+                                const float ratio =
+                                    static_cast<float>((tlo > 0.0 ? tlo : 0.0) / Tt);
+                                worst_rel = max(worst_rel, __float_as_uint(rel));
+                                worst_ratio = max(worst_ratio, __float_as_uint(ratio));
+                            }
+                        }
+                    }
+                    keep[q] = kp;
+                    bal[q] = __ballot(kp);
+                    total += static_cast<std::uint32_t>(__popcll(bal[q]));
+                }
+                std::uint32_t base = 0;
+                if (lane == 0 && total) base = atomicAdd(s_len, total);
+                base = __shfl(base, 0, 64);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (keep[q]) {
+                        const std::uint32_t pos =
+                            base + static_cast<std::uint32_t>(__popcll(bal[q] & ((1ull << lane) - 1ull)));
+                        const std::uint32_t i = r0 + 4u * kq + static_cast<std::uint32_t>(q);
+                        s_li[pos] = static_cast<std::uint16_t>(i | (j << 6));
+                    }
+                    base += static_cast<std::uint32_t>(__popcll(bal[q]));
                 }
             }
+            __syncthreads();
+            // 3. exact distances of the listed pairs, proposals counted
+            const std::uint32_t L = *s_len;
+            for (std::uint32_t e = tid; e < L; e += kMfmaThreads) {
+                const std::uint32_t ij = s_li[e], i = ij & 63u, j = ij >> 6;
+                float d;
+                if constexpr (STRIDE <= kJoinRegStride) {
+                    float rj[STRIDE];
+                    const float4* src =
+                        reinterpret_cast<const float4*>(s_rows + __umul24(j, STRIDE));
+#pragma unroll
+                    for (std::uint32_t q = 0; q < STRIDE / 4; ++q) {
+                        const float4 f = src[q];
+                        rj[4 * q] = f.x;
+                        rj[4 * q + 1] = f.y;
+                        rj[4 * q + 2] = f.z;
+                        rj[4 * q + 3] = f.w;
+                    }
+                    d = l2sqr_reg<STRIDE>(rj, s_rows + __umul24(i, STRIDE));
+                } else {
+                    d = l2sqr_exact(s_rows + __umul24(i, STRIDE), s_rows + __umul24(j, STRIDE),
+                                    STRIDE);
+                }
+                s_ld[e] = d;
+                if (d < s_thr[i]) atomicAdd(&s_cnt[i], 1u);
+                if (d < s_thr[j]) atomicAdd(&s_cnt[j], 1u);
+            }
+            __syncthreads();
+            // 4. one reservation per receiving member, then the keys
+            for (std::uint32_t x = tid; x < ns; x += kMfmaThreads) {
+                const std::uint32_t c = s_cnt[x];
+                s_base[x] = c ? atomicAdd(&a.fill[s_id[x]], c) : a.cap;
+                s_cnt[x] = 0;
+            }
+            __syncthreads();
+            for (std::uint32_t e = tid; e < L; e += kMfmaThreads) {
+                const float d = s_ld[e];
+                const std::uint32_t ij = s_li[e], i = ij & 63u, j = ij >> 6;
+                const bool to_i = d < s_thr[i] && s_base[i] < a.cap;
+                const bool to_j = d < s_thr[j] && s_base[j] < a.cap;
+                if (!(to_i || to_j)) continue;
+                const unsigned long long hb =
+                    static_cast<unsigned long long>(fbits(metric_finalize(d, a.cosine))) << 32;
+                if (to_i) {
+                    const std::uint32_t pos = s_base[i] + atomicAdd(&s_cnt[i], 1u);
+                    if (pos < a.cap) a.cand[std::size_t(s_id[i]) * a.cap + pos] = hb | s_id[j];
+                }
+                if (to_j) {
+                    const std::uint32_t pos = s_base[j] + atomicAdd(&s_cnt[j], 1u);
+                    if (pos < a.cap) a.cand[std::size_t(s_id[j]) * a.cap + pos] = hb | s_id[i];
+                }
+            }
+            __syncthreads();
+            t0 = t1;
         }
-        __syncthreads();
-        join_emit(a, s_id, s_thr, s_D, P, s_cnt, s_base, na, ns);
-        __syncthreads();
     }
     if (a.tally) {
         n_pairs = wave_sum(n_pairs);
@@ -1014,6 +1089,7 @@ __global__ void __launch_bounds__(kJoinThreads) join_gram_kernel(JoinArgs a) {
         if (lane == 0) {
             atomicAdd(&a.tally[0], n_pairs);
             atomicAdd(&a.tally[1], n_exact);
+            atomicAdd(&a.tally[2], n_tiles);
         }
     }
     if (a.audit) {
@@ -1504,31 +1580,34 @@ JoinFn pick_join_reg(std::uint32_t stride) {
     }
 }
 
-template <bool BF16>
-JoinFn pick_join_gram_t(std::uint32_t stride) {
+template <bool BF16, bool AUDIT>
+JoinFn pick_join_mfma_t(std::uint32_t stride) {
     switch (stride) {
-        case 8: return join_gram_kernel<8, BF16>;
-        case 16: return join_gram_kernel<16, BF16>;
-        case 24: return join_gram_kernel<24, BF16>;
-        case 32: return join_gram_kernel<32, BF16>;
-        case 48: return join_gram_kernel<48, BF16>;
-        case 64: return join_gram_kernel<64, BF16>;
-        case 128: return join_gram_kernel<128, BF16>;
-        default: return join_gram_kernel<0, BF16>;
+        case 8: return join_mfma_kernel<8, BF16, AUDIT>;
+        case 16: return join_mfma_kernel<16, BF16, AUDIT>;
+        case 24: return join_mfma_kernel<24, BF16, AUDIT>;
+        case 32: return join_mfma_kernel<32, BF16, AUDIT>;
+        case 48: return join_mfma_kernel<48, BF16, AUDIT>;
+        case 64: return join_mfma_kernel<64, BF16, AUDIT>;
+        case 96: return join_mfma_kernel<96, BF16, AUDIT>;
+        case 128: return join_mfma_kernel<128, BF16, AUDIT>;
+        default: return nullptr;
     }
 }
 
 // Local-join distance modes (coresg_hip_build_params::gram).
 enum : std::uint32_t { kGramAuto = 0, kGramExact = 1, kGramF16 = 2, kGramBF16 = 3 };
 
-// The library's choice is the exact join: in NN-Descent a third (d = 24) to
-// almost half (d = 128) of the visited pairs pass a threshold test and need
-// their exact distance anyway, and the Gram phase, the norms and the bound
-// cost more than the exact distances they save (MI355X, n = 1M, d = 24:
-// 51 ms exact vs 78 ms FP16 filter per build; n = 200k, d = 128: 54 vs 68 ms).
+// The library's choice: the exact member-stationary join for rows of up to
+// 64 floats, the FP16 matrix-core filter join for wider rows. In NN-Descent a
+// third (d = 24) to almost half (d = 128) of the visited pairs pass a
+// threshold test and need their exact distance anyway; at d = 24 an exact
+// distance costs about as much as the filter's per-pair bound and list work
+// over 16 x 16 tiles (MI355X, n = 1M: 2.37 ms exact vs 6.99 ms FP16 filter per
+// join launch), at d = 128 it costs six times more (n = 2M: 64.5 ms exact vs
+// 45.8 ms FP16 and 47.8 ms BF16 filter).
 std::uint32_t resolve_gram(std::uint32_t mode, std::uint32_t stride) {
-    (void)stride;
-    if (mode == kGramAuto) return kGramExact;
+    if (mode == kGramAuto) return stride >= 96 ? kGramF16 : kGramExact;
     if (mode > kGramBF16) throw std::runtime_error("nndescent: gram must be 0..3");
     return mode;
 }
@@ -1593,19 +1672,28 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     b.gthr.reserve(std::size_t(n) * S);
     b.gcnt.reserve(n);
     gram = resolve_gram(gram, stride);
+    // the matrix-core filter join is built for the common strides; any
+    // other stride takes the exact join (which proposes the same pairs)
+    const bool audit = gp && gp->profiling && gp->audit;
+    const JoinFn mfma_fn =
+        gram == kGramF16    ? (audit ? pick_join_mfma_t<false, true>(stride)
+                                     : pick_join_mfma_t<false, false>(stride))
+        : gram == kGramBF16 ? (audit ? pick_join_mfma_t<true, true>(stride)
+                                     : pick_join_mfma_t<true, false>(stride))
+                            : nullptr;
+    if (!mfma_fn) gram = kGramExact;
     const bool join_reg = gram == kGramExact && S <= kJoinThreads && pick_join_reg(stride);
-    const JoinFn join_fn = join_reg ? pick_join_reg(stride)
-                           : gram == kGramExact ? pick_join(stride)
-                           : gram == kGramF16 ? pick_join_gram_t<false>(stride)
-                                              : pick_join_gram_t<true>(stride);
+    const JoinFn join_fn = mfma_fn    ? mfma_fn
+                           : join_reg ? pick_join_reg(stride)
+                                      : pick_join(stride);
     const std::size_t lds =
-        join_reg    ? (4 * std::size_t(S) + std::size_t(S) * stride) * sizeof(float)
-        : gram == kGramExact
-            ? (4 * std::size_t(S) + std::size_t(S) * stride + std::size_t(A) * S) * sizeof(float)
-            : (2 * std::size_t(S) + std::size_t(S) * stride + std::size_t(A) * (S + 1) +
-               4 * std::size_t(S)) *
-                  sizeof(float);
-    const unsigned join_threads = kJoinThreads;
+        mfma_fn ? (8 * std::size_t(S) + std::size_t(S) * stride + kMfmaList) * sizeof(float) +
+                      kMfmaList * sizeof(std::uint16_t) + sizeof(std::uint32_t)
+        : join_reg
+            ? (4 * std::size_t(S) + std::size_t(S) * stride) * sizeof(float)
+            : (4 * std::size_t(S) + std::size_t(S) * stride + std::size_t(A) * S) * sizeof(float);
+    if (mfma_fn && S > 1024u) throw std::runtime_error("nndescent: sample too large for the filter join");
+    const unsigned join_threads = mfma_fn ? kMfmaThreads : kJoinThreads;
     const unsigned join_grid = static_cast<unsigned>(std::min<std::size_t>(n, 65536));
     if (lds > 150 * 1024) throw std::runtime_error("nndescent: candidate tile exceeds LDS");
     if (lds > 64 * 1024)
@@ -1627,7 +1715,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     const bool tallies = gp && gp->profiling;
     const bool gram_audit = tallies && gram != kGramExact;
     if (gram_audit) {
-        b.gram_tally.reserve(2);
+        b.gram_tally.reserve(3);
         b.gram_audit.reserve(2);
         ja.tally = b.gram_tally.p;
         ja.audit = gp->audit ? b.gram_audit.p : nullptr;
@@ -1640,6 +1728,9 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         if (tallies) gp->add_work("nnd_join", {rb.tally[0], rb.tally[1], rb.tally[2], n});
         if (gram_audit) {
             gp->add_work("nnd_gram", {rb.gram_tally[0], rb.gram_tally[1], 0, 0});
+            // matrix-core work: 16 x 16 tiles, 32-coordinate steps per tile
+            gp->add_work("nnd_mfma", {rb.gram_tally[2]});
+            gp->max_work("nnd_mfma", 1, (stride + 31u) / 32u);
             gp->max_work("nnd_gram", 2, rb.gram_audit[0]);
             gp->max_work("nnd_gram", 3, rb.gram_audit[1]);
         }
@@ -1701,7 +1792,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
             CSG_LAUNCH_CHECK();
         }
         if (gram_audit) {
-            fill_u64(reinterpret_cast<std::uint64_t*>(b.gram_tally.p), 2, 0ull, s);
+            fill_u64(reinterpret_cast<std::uint64_t*>(b.gram_tally.p), 3, 0ull, s);
             fill_u32(b.gram_audit.p, 2, 0u, s);
         }
         {
@@ -1731,7 +1822,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         if (tallies)
             CSG_CHECK(hipMemcpyAsync(rb->tally, b.tally.p, 24, hipMemcpyDeviceToHost, s));
         if (gram_audit) {
-            CSG_CHECK(hipMemcpyAsync(rb->gram_tally, b.gram_tally.p, 16, hipMemcpyDeviceToHost, s));
+            CSG_CHECK(hipMemcpyAsync(rb->gram_tally, b.gram_tally.p, 24, hipMemcpyDeviceToHost, s));
             CSG_CHECK(hipMemcpyAsync(rb->gram_audit, b.gram_audit.p, 8, hipMemcpyDeviceToHost, s));
         }
         CSG_CHECK(hipEventRecord(b.round_done.e[round & 1u], s));
@@ -2663,8 +2754,8 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
         }
         unsigned long long* tally = nullptr;
         if (gram_scan && g.profiling) {
-            b.gram_tally.reserve(2);
-            fill_u64(reinterpret_cast<std::uint64_t*>(b.gram_tally.p), 2, 0ull, s);
+            b.gram_tally.reserve(3);
+            fill_u64(reinterpret_cast<std::uint64_t*>(b.gram_tally.p), 3, 0ull, s);
             tally = b.gram_tally.p;
         }
         InsertArgs ia{g.pts.p, n0, n1, k, g.ids.p, g.dist.p, g.flags.p, g.count.p,

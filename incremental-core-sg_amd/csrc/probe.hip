@@ -7,6 +7,7 @@
 #include "../../include/coresg_hip.h"
 #include "graph.hpp"
 #include "handle.hpp"
+#include "mfma.hpp"
 #include "scan.hpp"
 
 namespace coresg_hip {
@@ -29,6 +30,23 @@ __global__ void __launch_bounds__(256)
         }
         out[t] = s;
     }
+}
+
+// One wavefront: C = A B^T through gram_mfma32 (A, B: 16 x 32, row-major),
+// with the lane layout the filter join assumes.
+template <bool BF16>
+__global__ void __launch_bounds__(64)
+    mfma_probe_kernel(const float* __restrict__ a, const float* __restrict__ b,
+                      float* __restrict__ c) {
+    const unsigned lane = threadIdx.x, li = lane & 15u, kq = lane >> 4;
+    float x[8], y[8];
+    for (int q = 0; q < 8; ++q) {
+        x[q] = a[li * 32 + 8 * kq + q];
+        y[q] = b[li * 32 + 8 * kq + q];
+    }
+    gram_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    acc = gram_mfma32<BF16>(x, y, acc);
+    for (int q = 0; q < 4; ++q) c[(4 * kq + q) * 16 + li] = acc[q];
 }
 
 }  // namespace
@@ -62,6 +80,28 @@ int coresg_hip_probe_gather(int device, size_t rows, uint32_t stride, size_t gat
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (ms) *ms = t;
+    });
+}
+
+int coresg_hip_probe_mfma(int device, int bf16, const float* a, const float* b, float* c) {
+    return coresg_hip::guarded([&] {
+        using namespace coresg_hip;
+        if (!a || !b || !c) throw std::runtime_error("probe_mfma: null operand");
+        CSG_CHECK(hipSetDevice(device));
+        OwnedStream os;
+        DBuf<float> da, db, dc;
+        da.reserve(16 * 32);
+        db.reserve(16 * 32);
+        dc.reserve(16 * 16);
+        CSG_CHECK(hipMemcpyAsync(da.p, a, 16 * 32 * sizeof(float), hipMemcpyHostToDevice, os.s));
+        CSG_CHECK(hipMemcpyAsync(db.p, b, 16 * 32 * sizeof(float), hipMemcpyHostToDevice, os.s));
+        if (bf16)
+            mfma_probe_kernel<true><<<1, 64, 0, os.s>>>(da.p, db.p, dc.p);
+        else
+            mfma_probe_kernel<false><<<1, 64, 0, os.s>>>(da.p, db.p, dc.p);
+        CSG_LAUNCH_CHECK();
+        CSG_CHECK(hipMemcpyAsync(c, dc.p, 16 * 16 * sizeof(float), hipMemcpyDeviceToHost, os.s));
+        CSG_CHECK(hipStreamSynchronize(os.s));
     });
 }
 

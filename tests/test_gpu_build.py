@@ -186,6 +186,20 @@ def test_nndescent_recall_at_least_reference(name):
     assert recall_gate(runs, ref, x.shape[0], k), (ours, ref)
 
 
+@pytest.mark.parametrize("bf16", [False, True])
+def test_mfma_16x16x32_lane_layout(bf16):
+    """The filter join's matrix-core instruction (gfx950 16x16x32 FP16/BF16)
+    with its operand and result lane layout: small integers are exact in both
+    formats and in the FP32 sums, so C must equal A B^T exactly; a transposed
+    or shifted layout would put other entries in place."""
+    print(f"[gpu] mfma layout bf16={bf16}", flush=True)
+    rng = np.random.default_rng(11)
+    a = rng.integers(-4, 5, size=(16, 32)).astype(np.float32)
+    b = rng.integers(-4, 5, size=(16, 32)).astype(np.float32)
+    c = chip.probe_mfma(a, b, bf16)
+    assert np.array_equal(c, (a.astype(np.float64) @ b.T.astype(np.float64)).astype(np.float32))
+
+
 def _gram_builds(x, k, gram, runs=7):
     """Builds with the given local-join mode; returns the exact recalls, the
     'nnd_gram' work tallies of the last build and checks stored distances."""
