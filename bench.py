@@ -61,13 +61,27 @@ def percentile(xs, p):
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the same
-    counters on a copy kernel of known size), or None."""
+    counters on a copy kernel of known size; and scaled by a random-row
+    gather probe of known bytes), or (None, None) when there is none or when
+    the kernel's source has changed since the counters were taken."""
+    import hashlib
     path = os.path.join(REPO, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get(kernel, {}).get("bytes_per_launch")
+            e = json.load(f).get(kernel)
     except (OSError, ValueError):
-        return None
+        return None, None
+    if not e:
+        return None, None
+    src = e.get("source")
+    if src:
+        try:
+            with open(os.path.join(REPO, src), "rb") as f:
+                if hashlib.sha256(f.read()).hexdigest() != e.get("source_sha256"):
+                    return None, None  # stale: the counters describe older code
+        except OSError:
+            return None, None
+    return e.get("bytes_per_launch"), e.get("bytes_per_launch_gather_scaled")
 
 
 def cpu_baseline(x, k, extract_mpts, threads, recall_queries):
@@ -539,9 +553,11 @@ def main():
             algo = (entries * (12 + 4 * slots) + verts * (8 + 13 * slots)) / launches
             extra = dict(adjacency_entries=entries / launches, slots=slots)
         achieved = algo / avg_s / 1e9
+        traffic, traffic_gather = measured_traffic(name)
         rooflines[name] = dict(kernel=name, bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS,
                                unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                               traffic=measured_traffic(name), algo_bytes_per_launch=algo,
+                               traffic=traffic, traffic_gather_scaled=traffic_gather,
+                               algo_bytes_per_launch=algo,
                                avg_launch_ms=kernels[name]["avg_ms"], **extra)
     roofline = rooflines.get(dominant) or (
         dict(kernel=dominant, bound="hbm", achieved=None, peak=HBM_PEAK_GBS, unit="GB/s",
