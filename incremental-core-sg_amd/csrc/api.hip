@@ -26,9 +26,10 @@ Graph::~Graph() {
         (void)hipSetDevice(device);
         (void)hipStreamSynchronize(stream);
     }
-    if (sweep_exec) {
+    if (sweep_exec || descent_exec) {
         std::lock_guard<std::mutex> lk(device_api_mutex());
-        (void)hipGraphExecDestroy(sweep_exec);
+        if (sweep_exec) (void)hipGraphExecDestroy(sweep_exec);
+        if (descent_exec) (void)hipGraphExecDestroy(descent_exec);
     }
     destroy_stream(stream);
 }

@@ -298,6 +298,9 @@ struct Graph {
     // launch parameters it was captured with
     hipGraphExec_t sweep_exec = nullptr;
     std::vector<unsigned char> sweep_key;
+    // the descent's rounds as one captured graph, and what it was made for
+    hipGraphExec_t descent_exec = nullptr;
+    std::vector<std::uint64_t> descent_key;
     DBuf<float> out_w;
     // autotuned sweep launch shapes of this graph, by slots (power of two)
     std::map<std::uint32_t, SweepShape> sweep_tune;

@@ -30,6 +30,7 @@
 // The early stop is the reference's: updates <= delta * k * n.
 #include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <memory>
 #include <vector>
 
@@ -1622,7 +1623,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
                           std::uint32_t* count, std::uint32_t smp, std::uint32_t iters,
                           double delta, std::uint64_t seed, bool random_fill,
                           std::uint32_t cap, std::uint32_t gram, hipStream_t s,
-                          bool cosine = false) {
+                          bool cosine = false, bool graph_rounds = false) {
     DescentResult res;
     if (n == 0) return res;
     if (k > kMaxK) throw std::runtime_error("nndescent: k above 64 is not supported on device");
@@ -1722,8 +1723,8 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     }
     const double stop = delta * double(k) * double(n);
     // Reads round r's counts once its copies have landed; true: it converged.
-    auto consume = [&](std::uint32_t r) -> bool {
-        CSG_CHECK(hipEventSynchronize(b.round_done.e[r & 1u]));
+    auto consume = [&](std::uint32_t r, bool wait) -> bool {
+        if (wait) CSG_CHECK(hipEventSynchronize(b.round_done.e[r & 1u]));
         const RoundReadback& rb = b.readback.p[r];
         if (tallies) gp->add_work("nnd_join", {rb.tally[0], rb.tally[1], rb.tally[2], n});
         if (gram_audit) {
@@ -1738,11 +1739,8 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         res.rounds = r + 1;
         return double(rb.upd) <= stop;
     };
-    std::uint32_t last = 0;
-    bool converged = false;
-    const bool plain = gp && gp->plain;
-    for (std::uint32_t round = 0; round < iters; ++round) {
-        last = round;
+    // One round's launches and the copies of its counts into `readback`.
+    auto enqueue_round = [&](std::uint32_t round) {
         {
             std::unique_ptr<ScopedKernelTimer> t;
             if (gp) t.reset(new ScopedKernelTimer(*gp, "nnd_sample"));
@@ -1825,22 +1823,85 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
             CSG_CHECK(hipMemcpyAsync(rb->gram_tally, b.gram_tally.p, 24, hipMemcpyDeviceToHost, s));
             CSG_CHECK(hipMemcpyAsync(rb->gram_audit, b.gram_audit.p, 8, hipMemcpyDeviceToHost, s));
         }
+    };
+    const bool plain = gp && gp->plain;
+    // Every round of the descent as one captured hipGraph, replayed per build
+    // (graph_rounds: a build, whose sizes repeat; inserts change n each time):
+    // the rounds after convergence return at once (the device's halt flag),
+    // and the counts are read after the replay. Profiled builds keep the
+    // eager launches (their per-kernel events), the plain form its host
+    // synchronisation after every launch.
+    if (graph_rounds && gp && !plain && !tallies) {
+        std::vector<std::uint64_t> key = {
+            reinterpret_cast<std::uint64_t>(pts), n, stride, k,
+            reinterpret_cast<std::uint64_t>(ids), reinterpret_cast<std::uint64_t>(dist),
+            reinterpret_cast<std::uint64_t>(flags), reinterpret_cast<std::uint64_t>(count), smp,
+            iters, cap, gram, cosine ? 1u : 0u, reinterpret_cast<std::uint64_t>(join_fn),
+            lds, join_threads, join_grid, merge_blocks, rev_buckets ? nb : 0u, wave_blocks};
+        std::uint64_t stop_bits;
+        std::memcpy(&stop_bits, &stop, 8);
+        key.push_back(stop_bits);
+        for (const void* q : {static_cast<const void*>(b.new_list.p), static_cast<const void*>(b.new_cnt.p),
+                              static_cast<const void*>(b.old_list.p), static_cast<const void*>(b.old_cnt.p),
+                              static_cast<const void*>(b.rev_new.p), static_cast<const void*>(b.rev_new_cnt.p),
+                              static_cast<const void*>(b.rev_old.p), static_cast<const void*>(b.rev_old_cnt.p),
+                              static_cast<const void*>(b.cand.p), static_cast<const void*>(b.updates.p),
+                              static_cast<const void*>(b.partial.p), static_cast<const void*>(b.tally.p),
+                              static_cast<const void*>(b.fill.p), static_cast<const void*>(b.gids.p),
+                              static_cast<const void*>(b.gcnt.p), static_cast<const void*>(b.thr_node.p),
+                              static_cast<const void*>(b.rev_pairs.p), static_cast<const void*>(b.rev_hist.p),
+                              static_cast<const void*>(b.rev_tmp.p), static_cast<const void*>(b.rev_total.p),
+                              static_cast<const void*>(b.gthr.p), static_cast<const void*>(b.halt.p),
+                              static_cast<const void*>(b.readback.p)})
+            key.push_back(reinterpret_cast<std::uint64_t>(q));
+        if (!gp->descent_exec || gp->descent_key != key) {
+            if (gp->descent_exec) {
+                std::lock_guard<std::mutex> lk(device_api_mutex());
+                CSG_CHECK(hipGraphExecDestroy(gp->descent_exec));
+            }
+            gp->descent_exec = nullptr;
+            hipGraph_t graph = nullptr;
+            {
+                CaptureGuard guard(s);
+                for (std::uint32_t round = 0; round < iters; ++round) enqueue_round(round);
+                graph = guard.end();
+            }
+            hipError_t e;
+            {
+                std::lock_guard<std::mutex> lk(device_api_mutex());
+                e = hipGraphInstantiate(&gp->descent_exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+            }
+            CSG_CHECK(e);
+            gp->descent_key = key;
+        }
+        CSG_CHECK(hipGraphLaunch(gp->descent_exec, s));
+        CSG_CHECK(hipStreamSynchronize(s));
+        for (std::uint32_t round = 0; round < iters; ++round)
+            if (consume(round, false)) break;
+        return res;
+    }
+    std::uint32_t last = 0;
+    bool converged = false;
+    for (std::uint32_t round = 0; round < iters; ++round) {
+        last = round;
+        enqueue_round(round);
         CSG_CHECK(hipEventRecord(b.round_done.e[round & 1u], s));
         if (plain) {
             // the plain form: the host reads the round's count before it
             // enqueues the next round
-            if (consume(round)) {
+            if (consume(round, true)) {
                 converged = true;
                 break;
             }
             continue;
         }
-        if (round > 0 && consume(round - 1)) {
+        if (round > 0 && consume(round - 1, true)) {
             converged = true;  // this round's kernels returned at once
             break;
         }
     }
-    if (!converged && !plain) consume(last);
+    if (!converged && !plain) consume(last, true);
     // the skipped round's copies land in `readback` too: wait for them
     CSG_CHECK(hipEventSynchronize(b.round_done.e[last & 1u]));
     return res;
@@ -2606,7 +2667,7 @@ void build_knn(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_stat
         const std::uint32_t cap = p.join_capacity ? p.join_capacity : kDefaultCap;
         check_join_capacity(cap);
         r = run_descent(&g, g.pts.p, n, g.stride, k, g.ids.p, g.dist.p, g.flags.p, g.count.p,
-                        p.sample, p.iters, p.delta, p.seed, true, cap, p.gram, s);
+                        p.sample, p.iters, p.delta, p.seed, true, cap, p.gram, s, false, true);
     }
     CSG_CHECK(hipStreamSynchronize(s));
     g.knn_set = true;

@@ -7,6 +7,8 @@ from __future__ import annotations
 
 import threading
 
+import time
+
 import numpy as np
 import pytest
 
@@ -132,15 +134,30 @@ def test_session_extract_overlaps_insert():
 
 
 def test_session_fuses_waiting_extracts():
-    """Extracts submitted while a sweep runs join the next sweep together."""
+    """Extracts submitted while a sweep runs join the next sweep together.
+    Which requests were waiting is read from the clock rather than assumed:
+    those submitted before the first request's sweep had ended (its submit
+    time plus its latency, less a margin) were queued behind it, and two or
+    more of them must have been served by one sweep."""
+    print("[gpu] session: waiting extracts fused", flush=True)
     x = datagen.gaussian_mixture(150_000, 16, 12, 0.3, 2.0, 0.05, seed=10)
     k = 16
     s = chip.Session(x, k)
+    t_first = time.perf_counter()
     first = s.submit_extract(list(range(1, k + 1)))  # keeps the worker busy
-    ids = [s.submit_extract([m]) for m in range(1, k + 1)]
+    ids, t_sub = [], []
+    for m in range(1, k + 1):
+        t_sub.append(time.perf_counter())
+        ids.append(s.submit_extract([m]))
     sts = [s.wait(i) for i in [first] + ids]
     assert all(st["state"] == chip.REQ_DONE for st in sts)
-    assert max(st["batch_requests"] for st in sts[1:]) > 1
+    first_done = t_first + sts[0]["latency_ms"] * 1e-3
+    waited = [st for st, t in zip(sts[1:], t_sub) if t < first_done - 2e-4]
+    print(f"  {len(waited)} of {k} extracts waited for the first sweep "
+          f"({sts[0]['latency_ms']:.2f} ms); batch sizes "
+          f"{sorted(set(st['batch_requests'] for st in sts[1:]))}", flush=True)
+    assert len(waited) >= 2, "the first sweep ended before two more extracts were submitted"
+    assert max(st["batch_requests"] for st in waited) > 1
     s.close()
 
 
