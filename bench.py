@@ -58,29 +58,69 @@ def percentile(xs, p):
     return xs[i]
 
 
+# The source functions each counter entry describes: their text is
+# fingerprinted, so a change to one of them marks the entry stale.
+TRAFFIC_SOURCES = {
+    "nnd_join": ("incremental-core-sg_amd/csrc/nndescent.hip",
+                 ("l2sqr_reg", "join_stage", "join_partner", "join_reg_kernel")),
+}
+
+
+def source_fingerprint(path, names):
+    """sha256 over the definitions (signature and body) of the named
+    functions in a source file, each found as `name(` followed by a balanced
+    parameter list and a braced body."""
+    import hashlib
+    with open(path) as f:
+        text = f.read()
+    h = hashlib.sha256()
+    for name in names:
+        for m in re.finditer(r"\b" + re.escape(name) + r"\s*\(", text):
+            i, depth = m.end() - 1, 0
+            while i < len(text):  # the parameter list
+                depth += {"(": 1, ")": -1}.get(text[i], 0)
+                i += 1
+                if depth == 0:
+                    break
+            j = i
+            while j < len(text) and text[j] not in "{;":
+                j += 1
+            if j >= len(text) or text[j] == ";" or text[i:j].strip() not in ("", "const"):
+                continue  # a call or a declaration
+            depth, k = 0, j
+            while k < len(text):  # the body
+                depth += {"{": 1, "}": -1}.get(text[k], 0)
+                k += 1
+                if depth == 0:
+                    break
+            h.update(text[m.start():k].encode())
+            break
+        else:
+            h.update(("missing:" + name).encode())
+    return h.hexdigest()
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the same
     counters on a copy kernel of known size; and scaled by a random-row
     gather probe of known bytes), or (None, None) when there is none or when
-    the kernel's source has changed since the counters were taken."""
-    import hashlib
+    the kernel's source functions have changed since the counters were
+    taken."""
     path = os.path.join(REPO, "profiles", "traffic.json")
     try:
         with open(path) as f:
             e = json.load(f).get(kernel)
     except (OSError, ValueError):
         return None, None
-    if not e:
+    if not e or kernel not in TRAFFIC_SOURCES:
         return None, None
-    src = e.get("source")
-    if src:
-        try:
-            with open(os.path.join(REPO, src), "rb") as f:
-                if hashlib.sha256(f.read()).hexdigest() != e.get("source_sha256"):
-                    return None, None  # stale: the counters describe older code
-        except OSError:
-            return None, None
+    src, names = TRAFFIC_SOURCES[kernel]
+    try:
+        if source_fingerprint(os.path.join(REPO, src), names) != e.get("source_fingerprint"):
+            return None, None  # stale: the counters describe other code
+    except OSError:
+        return None, None
     return e.get("bytes_per_launch"), e.get("bytes_per_launch_gather_scaled")
 
 
