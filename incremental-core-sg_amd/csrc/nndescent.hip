@@ -57,7 +57,9 @@ constexpr std::uint32_t kRevMaxBuckets = 8192;
 constexpr unsigned kRevBlocks = 2048;
 constexpr unsigned kRevThreads = 1024;
 constexpr std::uint32_t kJoinRegStride = 64;  // widest row a join thread keeps in registers
-constexpr std::uint32_t kMfmaThreads = 256;   // matrix-core filter join: threads per node
+// matrix-core filter join: threads per node (eight wavefronts for rows wide
+// enough that only two or three nodes fit a CU's LDS)
+constexpr std::uint32_t mfma_join_threads(std::uint32_t stride) { return stride >= 96 ? 512u : 256u; }
 constexpr std::uint32_t kMfmaList = 1536;     // pairs listed for exact distances per batch
 
 // ---------------------------------------------------------------------------
@@ -884,10 +886,11 @@ constexpr unsigned mfma_join_blocks(std::uint32_t stride) {
 
 // AUDIT: the FP64 check of every live pair (a.audit set) is compiled in.
 template <std::uint32_t STRIDE, bool BF16, bool AUDIT>
-__global__ void __launch_bounds__(kMfmaThreads, mfma_join_blocks(STRIDE))
+__global__ void __launch_bounds__(mfma_join_threads(STRIDE), mfma_join_blocks(STRIDE))
     join_mfma_kernel(JoinArgs a) {
     static_assert(STRIDE % 8 == 0, "rows padded to 8 floats");
     constexpr std::uint32_t KP = (STRIDE + 31u) & ~31u;
+    constexpr std::uint32_t kMfmaThreads = mfma_join_threads(STRIDE);
     extern __shared__ float lds[];
     if (*a.halt) return;
     const std::uint32_t S = 6u * a.smp;
@@ -1694,7 +1697,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
             ? (4 * std::size_t(S) + std::size_t(S) * stride) * sizeof(float)
             : (4 * std::size_t(S) + std::size_t(S) * stride + std::size_t(A) * S) * sizeof(float);
     if (mfma_fn && S > 1024u) throw std::runtime_error("nndescent: sample too large for the filter join");
-    const unsigned join_threads = mfma_fn ? kMfmaThreads : kJoinThreads;
+    const unsigned join_threads = mfma_fn ? mfma_join_threads(stride) : kJoinThreads;
     const unsigned join_grid = static_cast<unsigned>(std::min<std::size_t>(n, 65536));
     if (lds > 150 * 1024) throw std::runtime_error("nndescent: candidate tile exceeds LDS");
     if (lds > 64 * 1024)
