@@ -464,7 +464,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16,
                     help="host threads of the reference CPU baseline (the box's CPU share)")
-    ap.add_argument("--recall-queries", type=int, default=300)
+    ap.add_argument("--recall-queries", type=int, default=1000)
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched uneven instances (256 mixtures, d in {8,16,32,64})")
     ap.add_argument("--no-exact", action="store_true",
