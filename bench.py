@@ -252,7 +252,8 @@ def serving_run(chip, datagen, k, seed_n, batches, insert_rate, extract_rate, bu
 
 def exact_mst_run(chip, datagen, coresg_io, cpu_n=5000):
     """CoreSG::exact_mst (Prim over the complete MRD graph, one cooperative
-    launch) at mpts = k_max on cfg1's shape (20k x 16) and on the first 100k
+    launch, and beside it the one-launch-per-step form) at mpts = k_max on
+    cfg1's shape (20k x 16) and on the first 100k
     points of the headline generator (d=24), plus a bounded CPU comparison: the
     oracle's Prim (single thread) and the device on the first `cpu_n` points."""
     out = {}
@@ -269,11 +270,19 @@ def exact_mst_run(chip, datagen, coresg_io, cpu_n=5000):
             r = g.exact_mst(k)
             ts.append(time.perf_counter() - t0)
         ms = 1e3 * min(ts)
+        # the one-launch-per-step form (kernel boundaries as the step barrier)
+        g.set_exact_launch(1)
+        t0 = time.perf_counter()
+        rs = g.exact_mst(k)
+        stepped_ms = 1e3 * (time.perf_counter() - t0)
+        g.set_exact_launch(0)
         out[name] = dict(n=int(x.shape[0]), d=int(x.shape[1]), mpts=k, ms=ms,
                          us_per_step=1e3 * ms / (x.shape[0] - 1), num_edges=r["num_edges"],
-                         weight=r["weight"])
+                         weight=r["weight"], stepped_launch_ms=stepped_ms,
+                         stepped_same_weight=rs["weight"] == r["weight"])
         g.close()
-        print(f"[bench] exact_mst {name}: {ms:.1f} ms", file=sys.stderr, flush=True)
+        print(f"[bench] exact_mst {name}: {ms:.1f} ms (one launch per step: {stepped_ms:.1f} ms)",
+              file=sys.stderr, flush=True)
     # Bounded CPU comparison on the first cpu_n points of the 100k case.
     x = cases[1][1][:cpu_n]
     k = cases[1][2]

@@ -284,6 +284,13 @@ int coresg_hip_extract(coresg_hip_graph* g, const uint32_t* mpts, uint32_t count
  * be NULL. info->rounds = Prim steps. */
 int coresg_hip_exact_mst(coresg_hip_graph* g, uint32_t mpts, coresg_hip_mst_info* info,
                          uint32_t* out_u, uint32_t* out_v, float* out_w, uint32_t* out_labels);
+/* How coresg_hip_exact_mst launches its Prim steps: 0 (default) one
+ * cooperative launch that runs every step with its blocks exchanging keys on
+ * the device; 1 one launch per step (the kernel boundary as the step's
+ * barrier, batches of launches replayed from a captured hipGraph), for callers
+ * that avoid cooperative launches. Both give the same edges. No counterpart
+ * in the reference. */
+int coresg_hip_graph_set_exact_launch(coresg_hip_graph* g, int mode);
 
 /* CoreSG::save (io.cpp:232-260): the resident graph in the reference's
  * .coresg format, byte for byte (metric L2, the bridges' missing-edge mode). */

@@ -139,6 +139,7 @@ SIGNATURES = [
     ("coresg_hip_sampled_recall", C.c_int, [_vp, C.c_size_t, C.c_uint64,
                                             C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("coresg_hip_graph_set_plain", C.c_int, [_vp, C.c_int]),
+    ("coresg_hip_graph_set_exact_launch", C.c_int, [_vp, C.c_int]),
     ("coresg_hip_probe_gather", C.c_int, [C.c_int, C.c_size_t, C.c_uint32, C.c_size_t,
                                           C.POINTER(C.c_double)]),
     ("coresg_hip_probe_mfma", C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp]),
@@ -436,6 +437,11 @@ class Graph:
         r = TuneResult()
         check(lib().coresg_hip_autotune_sweep(self.h, ptr(mp), mp.size, C.byref(r)))
         return r.asdict()
+
+    def set_exact_launch(self, mode: int):
+        """exact_mst's launches: 0 one cooperative launch for every Prim step,
+        1 one launch per step (captured batches); same edges."""
+        check(lib().coresg_hip_graph_set_exact_launch(self.h, int(mode)))
 
     def set_plain(self, on: bool):
         """The plain one-launch-per-step form (host sync after every launch,

@@ -396,6 +396,13 @@ int coresg_hip_graph_set_plain(coresg_hip_graph* h, int on) {
     return guarded([&] { h->g.plain = on != 0; });
 }
 
+int coresg_hip_graph_set_exact_launch(coresg_hip_graph* h, int mode) {
+    return guarded_host([&] {
+        if (mode != 0 && mode != 1) throw std::runtime_error("exact launch mode must be 0 or 1");
+        h->g.exact_stepped = mode == 1;
+    });
+}
+
 int coresg_hip_graph_set_profiling(coresg_hip_graph* h, int on) {
     return guarded([&] {
         h->g.resolve_timers();

@@ -328,6 +328,117 @@ __global__ void __launch_bounds__(kThreads) exact_prim_lds_kernel(Prim p) {
     }
 }
 
+// The same Prim as exact_prim_kernel with one launch per step instead of one
+// cooperative launch for all (coresg_hip_graph_set_exact_launch(g, 1)): the
+// kernel boundary is the step's barrier, so nothing waits on another block.
+// Each block relaxes its owned vertices against the current tree vertex and
+// publishes its minimum key; the block that finishes last (a device-scope
+// counter) reduces the keys to the pick, records the step's edge, marks the
+// pick in the tree and makes it the next step's current vertex. ctl[0] = the
+// current vertex, ctl[1] = the finished-block counter, ctl[2] = the step;
+// launches past the last step return at once, so a captured batch of
+// launches can be replayed over the whole run.
+__global__ void __launch_bounds__(kThreads) exact_prim_step_kernel(Prim p, std::uint32_t* ctl) {
+    __shared__ __align__(16) float crow[kMaxStride];
+    __shared__ std::uint64_t wmin[kThreads / kWave];
+    __shared__ bool s_last;
+    const std::uint32_t step = ctl[2];
+    if (step >= p.n) return;  // block-uniform
+    const std::uint32_t tid = threadIdx.x, G = gridDim.x;
+    const std::uint32_t j0 = blockIdx.x * p.chunk;
+    const std::uint32_t j1 = j0 + p.chunk < p.n ? j0 + p.chunk : p.n;
+    const std::uint32_t current = ctl[0];
+    std::uint64_t key_min = kNone64;
+    if (current != kNone) {
+        const float* row = p.pts + std::size_t(current) * p.stride;
+        for (std::uint32_t t = tid; t < p.stride; t += kThreads) crow[t] = row[t];
+        const float cc = p.core[current];
+        __syncthreads();
+        for (std::uint32_t j = j0 + tid; j < j1; j += kThreads) {
+            if (p.in_tree[j]) continue;
+            const float d =
+                sqrt_exact(l2sqr_exact(crow, p.pts + std::size_t(j) * p.stride, p.stride));
+            const float w = fmax_ref(d, fmax_ref(cc, p.core[j]));
+            float b = p.best[j];
+            if (w < b) {
+                b = w;
+                p.best[j] = w;
+                __hip_atomic_store(&p.from[j], current, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            const std::uint64_t key = (std::uint64_t(fbits(b)) << 32) | j;
+            key_min = key < key_min ? key : key_min;
+        }
+    }
+    key_min = wave_min_u64(key_min);
+    if ((tid & (kWave - 1)) == 0) wmin[tid / kWave] = key_min;
+    __syncthreads();
+    if (tid == 0) {
+        std::uint64_t m = wmin[0];
+#pragma unroll
+        for (unsigned w = 1; w < kThreads / kWave; ++w) m = wmin[w] < m ? wmin[w] : m;
+        __hip_atomic_store(&p.cand[blockIdx.x], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // release: this block's key and from[] stores before the count
+        const std::uint32_t done =
+            __hip_atomic_fetch_add(&ctl[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = done == G - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;  // block-uniform
+    std::uint64_t k = kNone64;
+    for (std::uint32_t i = tid; i < G; i += kThreads) {
+        const std::uint64_t c =
+            __hip_atomic_load(&p.cand[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        k = c < k ? c : k;
+    }
+    k = wave_min_u64(k);
+    __syncthreads();
+    if ((tid & (kWave - 1)) == 0) wmin[tid / kWave] = k;
+    __syncthreads();
+    if (tid == 0) {
+        k = wmin[0];
+#pragma unroll
+        for (unsigned w = 1; w < kThreads / kWave; ++w) k = wmin[w] < k ? wmin[w] : k;
+        std::uint32_t next = kNone;
+        if (k != kNone64 && step >= 1) {
+            const std::uint32_t pick = static_cast<std::uint32_t>(k);
+            const float pw = __uint_as_float(static_cast<std::uint32_t>(k >> 32));
+            const std::uint32_t f =
+                __hip_atomic_load(&p.from[pick], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            p.in_tree[pick] = 1;
+            if (f != kNone && pw < INFINITY) {
+                p.out_a[step - 1] = f < pick ? f : pick;
+                p.out_b[step - 1] = f < pick ? pick : f;
+                p.out_w[step - 1] = pw;
+            } else {
+                p.out_a[step - 1] = kNone;
+            }
+            next = pick;
+        } else if (step >= 1) {
+            p.out_a[step - 1] = kNone;
+        }
+        ctl[0] = next;
+        ctl[1] = 0;
+        ctl[2] = step + 1;
+    }
+}
+
+// Initial state of the stepped Prim: vertex 0 in the tree and current.
+__global__ void __launch_bounds__(kThreads) exact_step_init_kernel(Prim p, std::uint32_t* ctl) {
+    for (std::uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < p.n;
+         j += gridDim.x * blockDim.x) {
+        p.best[j] = j == 0 ? 0.0f : INFINITY;
+        p.from[j] = kNone;
+        p.in_tree[j] = j == 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl[0] = 0;  // current
+        ctl[1] = 0;  // finished blocks
+        ctl[2] = 1;  // step
+    }
+}
+
+constexpr std::uint32_t kStepsPerGraph = 256;  // launches per captured replay
+
 using PrimKernel = void (*)(Prim);
 
 // Base distance of each (u, v) pair, metric_distance (distance.hpp:196-201).
@@ -427,7 +538,54 @@ void exact_mst(Graph& g, std::uint32_t mpts, void* info_out, std::uint32_t* out_
         Prim p{g.pts.p, core.p, best.p, from.p, in_tree.p, cand.p, oa.p, ob.p, ow.p,
                ctl.p, ctl.p + 1, n, g.stride, chunk};
         void* args[] = {&p};
-        {
+        if (g.exact_stepped) {
+            // one launch per Prim step, kStepsPerGraph of them captured once
+            // and replayed until every step has run
+            DBuf<std::uint32_t> sctl;
+            sctl.reserve(3);
+            exact_step_init_kernel<<<blocks_for(n, kThreads), kThreads, 0, g.stream>>>(p, sctl.p);
+            CSG_LAUNCH_CHECK();
+            if (g.plain) {  // the plain form: every launch on its own, synchronised
+                ScopedKernelTimer t(g, "exact_prim");
+                for (std::uint32_t step = 1; step < n; ++step) {
+                    exact_prim_step_kernel<<<blocks, kThreads, 0, g.stream>>>(p, sctl.p);
+                    CSG_LAUNCH_CHECK();
+                }
+            }
+            hipGraph_t graph = nullptr;
+            if (!g.plain) {
+                CaptureGuard cap(g.stream);
+                for (std::uint32_t i = 0; i < kStepsPerGraph; ++i) {
+                    exact_prim_step_kernel<<<blocks, kThreads, 0, g.stream>>>(p, sctl.p);
+                    CSG_CHECK(hipGetLastError());
+                }
+                graph = cap.end();
+            }
+            if (graph) {
+                hipGraphExec_t exec = nullptr;
+                hipError_t e;
+                {
+                    std::lock_guard<std::mutex> lk(device_api_mutex());
+                    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(graph);
+                }
+                CSG_CHECK(e);
+                hipError_t le = hipSuccess;
+                {
+                    ScopedKernelTimer t(g, "exact_prim");
+                    for (std::uint32_t done = 0; done + 1 < n && le == hipSuccess;
+                         done += kStepsPerGraph)
+                        le = hipGraphLaunch(exec, g.stream);
+                }
+                const hipError_t se = hipStreamSynchronize(g.stream);
+                {
+                    std::lock_guard<std::mutex> lk(device_api_mutex());
+                    (void)hipGraphExecDestroy(exec);
+                }
+                CSG_CHECK(le);
+                CSG_CHECK(se);
+            }
+        } else {
             // A cooperative launch cannot be captured; say so instead of
             // launching into a stream that a failed capture left open.
             hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;

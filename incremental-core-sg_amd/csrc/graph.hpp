@@ -331,6 +331,8 @@ struct Graph {
     // sweep one mpts value per pass with every Boruvka round launched and
     // checked from the host, the descent's convergence read after each round.
     bool plain = false;
+    // exact_mst: one launch per Prim step instead of one cooperative launch
+    bool exact_stepped = false;
     // Profiling (audit: FP64 check of the Gram-filter join's distances).
     bool profiling = false;
     bool audit = false;

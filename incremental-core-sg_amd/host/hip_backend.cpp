@@ -208,17 +208,36 @@ public:
         return out;
     }
 
-    // Switched off: in a process driving the reference's CoreSG through this
-    // adapter, coresg_hip_exact_mst's cooperative launch once returned
-    // hipErrorIllegalState after an extract_msts call, with no cause found in
-    // this code. Until one is shown, exact_mst stays on the host path (the
-    // reference's Prim over mrd_block rows, which this backend serves).
-    // coresg_hip_exact_mst itself is unaffected and tested on its own.
+    // CoreSG::exact_mst (coresg.cpp:419-430) on the resident device copy,
+    // with one launch per Prim step: in a process driving the reference's
+    // CoreSG through this adapter, the one-launch cooperative form once
+    // returned hipErrorIllegalState after an extract_msts call, with no cause
+    // shown, so the adapter does not use cooperative launches.
     MstResult exact_mst(const CoreSG& sg, std::uint32_t mpts) override {
-        // A fresh start for the resident MRD operands: each call hands
-        // mrd_block a new core-distance vector.
-        coresg_hip_release_thread_cache();
-        return sg.exact_mst(mpts);
+        int code = 0;
+        const std::size_t n = sg.knn().size();
+        if (!metric_to_hip(sg.data().metric(), &code) || code != CORESG_HIP_METRIC_L2 || n < 2) {
+            // A fresh start for the resident MRD operands: each call hands
+            // mrd_block a new core-distance vector.
+            coresg_hip_release_thread_cache();
+            return sg.exact_mst(mpts);
+        }
+        std::lock_guard<std::mutex> lk(resident_mu_);
+        DeviceGraph& dg = resident(sg);
+        hip_check(coresg_hip_graph_set_exact_launch(dg.g, 1), "set_exact_launch");
+        coresg_hip_mst_info info{};
+        std::vector<std::uint32_t> u(n - 1), v(n - 1), lab(n);
+        std::vector<float> w(n - 1);
+        hip_check(coresg_hip_exact_mst(dg.g, mpts, &info, u.data(), v.data(), w.data(),
+                                       lab.data()),
+                  "coresg_hip_exact_mst");
+        MstResult r;
+        r.edges.reserve(info.num_edges);
+        for (std::size_t e = 0; e < info.num_edges; ++e) r.edges.emplace_back(u[e], v[e], w[e]);
+        r.component = std::move(lab);
+        r.num_components = info.num_components;
+        r.weight = info.weight;
+        return r;
     }
 
     Hubness hubness(const KnnGraph& knn, std::uint32_t budget_dim,

@@ -56,6 +56,48 @@ def test_exact_mst_matches_reference(name, mpts):
     g.close()
 
 
+@pytest.mark.parametrize("name,mpts", RUNS)
+def test_exact_mst_stepped_launch_matches_reference(name, mpts):
+    """The one-launch-per-step form (coresg_hip_graph_set_exact_launch(g, 1):
+    the kernel boundary as the step barrier, batches of launches replayed from
+    a captured graph) gives the reference's Prim output bit for bit too."""
+    print(f"[gpu] exact_mst stepped vs reference: {name} mpts={mpts}", flush=True)
+    f = coresg_io.loads(golden(name)["coresg_bytes"].tobytes())
+    g = graph_from_file(f)
+    g.set_exact_launch(1)
+    r = g.exact_mst(mpts, labels=True)
+    p = f"{name}_exact{mpts}_"
+    assert_same(r, dict(u=EXACT[p + "mst_u"], v=EXACT[p + "mst_v"], w=EXACT[p + "mst_w"],
+                        labels=EXACT[p + "component"],
+                        ncomp=int(EXACT[p + "num_components"][0]),
+                        weight=EXACT[p + "weight"][0]))
+    g.close()
+
+
+def test_exact_mst_stepped_equals_cooperative_at_5k():
+    """Both launch forms on a fresh 5 000-point build (more steps than one
+    captured batch of launches, 20 vertices per block), back to back with a
+    sweep in between: the same edges, weights and labels."""
+    print("[gpu] exact_mst stepped vs cooperative: n=5000", flush=True)
+    x = datagen.gaussian_mixture(5000, 24, 12, 0.3, 2.0, 0.05, 17)
+    g = chip.Graph(x.shape[0], x.shape[1], 16)
+    g.set_points(x)
+    g.build(chip.build_params())
+    a = g.exact_mst(8, labels=True)
+    g.extract([4, 8])
+    g.set_exact_launch(1)
+    b = g.exact_mst(8, labels=True)
+    c = g.exact_mst(3, labels=True)
+    g.set_exact_launch(0)
+    d = g.exact_mst(3, labels=True)
+    g.close()
+    for p, q in ((a, b), (c, d)):
+        assert np.array_equal(p["u"], q["u"]) and np.array_equal(p["v"], q["v"])
+        assert np.array_equal(p["w"].view(np.uint32), q["w"].view(np.uint32))
+        assert np.array_equal(p["labels"], q["labels"])
+        assert np.float64(p["weight"]).view(np.uint64) == np.float64(q["weight"]).view(np.uint64)
+
+
 def test_exact_mst_matches_oracle_and_clamps_mpts(tmp_path):
     """A fresh device build of 3000 points (12 blocks of 250 vertices): device
     Prim == oracle Prim on the same lists, edge for edge; mpts 0

@@ -73,14 +73,13 @@ def test_reference_coresg_routes_extraction_to_device(name, mpts, tmp_path):
 @pytest.mark.skipif(not os.path.exists(EXTRACT_CHECK), reason="integration build not present")
 @pytest.mark.parametrize("name,mpts", [("mix_d16", "1 4 16 2 8"), ("ties", "1 3 8 2"),
                                        ("islands_repair", "1 3 5 2")])
-def test_reference_coresg_exact_mst_over_device_rows_back_to_back(name, mpts, tmp_path):
-    """The reference's CoreSG::exact_mst with HipBackend (Prim on the host
-    over mrd_block rows served by the device) for several mpts in a row, after
-    an all_msts call in the same process: every result equals the host
-    backend's bit for bit. CoreSG::exact_mst hands a fresh core-distance vector
-    to each call, usually at the address of the previous one; the device's
-    resident operands must follow the contents, not the address."""
-    print(f"[gpu] exact_mst over device mrd rows: {name} mpts {mpts}", flush=True)
+def test_reference_coresg_exact_mst_routed_back_to_back(name, mpts, tmp_path):
+    """The reference's CoreSG::exact_mst routed through HipBackend's
+    DeviceExtract capability (the device Prim in its one-launch-per-step form
+    over the resident copy, no cooperative launch) for several mpts in a row,
+    after two all_msts calls in the same process: every result equals the host
+    backend's bit for bit."""
+    print(f"[gpu] exact_mst routed to the device: {name} mpts {mpts}", flush=True)
     path = tmp_path / "g.coresg"
     golden(name)["coresg_bytes"].tofile(path)
     r = subprocess.run([EXTRACT_CHECK, "--exact", str(path)] + mpts.split(),
@@ -89,15 +88,7 @@ def test_reference_coresg_exact_mst_over_device_rows_back_to_back(name, mpts, tm
     assert "exact msts equal (backend hip)" in r.stdout
 
 
-@pytest.mark.skip(reason="HipBackend::exact_mst routing is switched off: in the extract_check "
-                         "process its cooperative launch once returned hipErrorIllegalState "
-                         "after an extract_msts call, with no cause found yet")
-def test_reference_coresg_routes_exact_mst_to_device(tmp_path):
-    path = tmp_path / "g.coresg"
-    golden("mix_d16")["coresg_bytes"].tofile(path)
-    r = subprocess.run([EXTRACT_CHECK, "--exact", str(path), "1", "4", "16"],
-                       capture_output=True, text=True, timeout=120)
-    assert r.returncode == 0, r.stdout + r.stderr
+
 
 
 BACKEND_CHECK = os.path.join(REPO, "oracle", "_ref", "backend_check")
