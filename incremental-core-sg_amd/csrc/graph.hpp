@@ -225,6 +225,8 @@ struct DescentBuffers {
     DBuf<unsigned int> gram_audit;
     // device-side convergence flag and the per-round counts read back
     DBuf<std::uint32_t> halt;
+    // filter join: nodes set aside for its second pass, and their count
+    DBuf<std::uint32_t> defer, defer_count;
     HostBuf<RoundReadback> readback;
     EventPair round_done;
 };

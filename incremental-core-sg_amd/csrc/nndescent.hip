@@ -57,10 +57,14 @@ constexpr std::uint32_t kRevMaxBuckets = 8192;
 constexpr unsigned kRevBlocks = 2048;
 constexpr unsigned kRevThreads = 1024;
 constexpr std::uint32_t kJoinRegStride = 64;  // widest row a join thread keeps in registers
-// matrix-core filter join: threads per node (eight wavefronts for rows wide
-// enough that only two or three nodes fit a CU's LDS)
-constexpr std::uint32_t mfma_join_threads(std::uint32_t stride) { return stride >= 96 ? 512u : 256u; }
+// matrix-core filter join: threads per node, and the members whose rows a
+// first pass stages per node for rows of at least 64 floats (nodes with more
+// members -- a few percent -- go to a second pass that stages all S): the
+// smaller LDS footprint lets four nodes share a CU at d = 128 instead of two
+constexpr std::uint32_t mfma_join_threads(std::uint32_t) { return 256u; }
+constexpr std::uint32_t mfma_row_cap(std::uint32_t stride) { return stride >= 64 ? 48u : 0u; }
 constexpr std::uint32_t kMfmaList = 1536;     // pairs listed for exact distances per batch
+constexpr unsigned kDeferBlocks = 2048;        // grid of the filter join's second pass
 
 // ---------------------------------------------------------------------------
 // init_random
@@ -489,6 +493,13 @@ struct JoinArgs {
     unsigned int* audit;
     const std::uint32_t* halt;  // nonzero: the descent has converged, skip
     bool cosine;                // stored values are r / 2 instead of sqrt(r)
+    // filter join only: nodes of more than row_cap members are set aside in
+    // `defer` (count in *defer_count) for a second pass that stages up to S
+    // rows per node (deferred_pass); 0 = every node in one pass
+    std::uint32_t row_cap = 0;
+    std::uint32_t* defer = nullptr;
+    std::uint32_t* defer_count = nullptr;
+    bool deferred_pass = false;
 };
 
 // Exact squared distance with `a` held in registers (compile-time stride),
@@ -881,7 +892,7 @@ __device__ __forceinline__ void centred8(const float* s_rows, std::uint32_t r, s
 // Nodes per CU the filter join is compiled for: its LDS per node at sample
 // 16 allows about this many; the register budget follows from it.
 constexpr unsigned mfma_join_blocks(std::uint32_t stride) {
-    return stride <= 24 ? 6u : stride <= 48 ? 4u : stride <= 96 ? 3u : 2u;
+    return stride <= 24 ? 6u : 4u;
 }
 
 // AUDIT: the FP64 check of every live pair (a.audit set) is compiled in.
@@ -900,7 +911,10 @@ __global__ void __launch_bounds__(mfma_join_threads(STRIDE), mfma_join_blocks(ST
     std::uint32_t* s_base = s_cnt + S;
     float4* s_m4 = reinterpret_cast<float4*>(lds + 4 * S);  // S multiple of 2: 16-B aligned
     float* s_rows = lds + 8 * S;
-    float* s_ld = s_rows + std::size_t(S) * STRIDE;
+    // rows staged per node: row_cap in a first pass (nodes with more members
+    // are set aside), S in the pass over the set-aside nodes
+    const std::uint32_t rows_alloc = a.row_cap && !a.deferred_pass ? a.row_cap : S;
+    float* s_ld = s_rows + std::size_t(rows_alloc) * STRIDE;
     std::uint16_t* s_li = reinterpret_cast<std::uint16_t*>(s_ld + kMfmaList);
     std::uint32_t* s_len = reinterpret_cast<std::uint32_t*>(s_li + kMfmaList);
     const std::uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
@@ -913,7 +927,16 @@ __global__ void __launch_bounds__(mfma_join_threads(STRIDE), mfma_join_blocks(ST
     const float sscale = (1.0f + 0x1p-20f) / (1.0f - eta);
     unsigned long long n_pairs = 0, n_exact = 0, n_tiles = 0;
     unsigned int worst_rel = 0, worst_ratio = 0;
-    for (std::uint32_t v = blockIdx.x; v < a.n; v += gridDim.x) {
+    const std::uint32_t nodes = a.deferred_pass ? *a.defer_count : a.n;
+    for (std::uint32_t it = blockIdx.x; it < nodes; it += gridDim.x) {
+        const std::uint32_t v = a.deferred_pass ? a.defer[it] : it;
+        if (a.row_cap && !a.deferred_pass) {
+            const std::uint32_t c = a.cnt[v];
+            if ((c & 0xFFFFu) != 0 && (c >> 16) > a.row_cap) {  // block-uniform
+                if (tid == 0) a.defer[atomicAdd(a.defer_count, 1u)] = v;
+                continue;
+            }
+        }
         std::uint32_t na, ns;
         if (!join_stage<kMfmaThreads>(a, v, S, STRIDE, s_id, s_thr, s_rows, &na, &ns)) continue;
         // 1. member constants
@@ -1690,20 +1713,32 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     const JoinFn join_fn = mfma_fn    ? mfma_fn
                            : join_reg ? pick_join_reg(stride)
                                       : pick_join(stride);
+    // the filter join's first pass stages at most row_cap rows per node (0:
+    // all S); its second pass, over the nodes set aside, stages S
+    const std::uint32_t row_cap = mfma_fn && mfma_row_cap(stride) < S ? mfma_row_cap(stride) : 0u;
+    auto mfma_lds = [&](std::uint32_t rows) {
+        return (8 * std::size_t(S) + std::size_t(rows) * stride + kMfmaList) * sizeof(float) +
+               kMfmaList * sizeof(std::uint16_t) + sizeof(std::uint32_t);
+    };
+    const std::size_t lds_full = mfma_fn ? mfma_lds(S) : 0;
     const std::size_t lds =
-        mfma_fn ? (8 * std::size_t(S) + std::size_t(S) * stride + kMfmaList) * sizeof(float) +
-                      kMfmaList * sizeof(std::uint16_t) + sizeof(std::uint32_t)
+        mfma_fn ? mfma_lds(row_cap ? row_cap : S)
         : join_reg
             ? (4 * std::size_t(S) + std::size_t(S) * stride) * sizeof(float)
             : (4 * std::size_t(S) + std::size_t(S) * stride + std::size_t(A) * S) * sizeof(float);
     if (mfma_fn && S > 1024u) throw std::runtime_error("nndescent: sample too large for the filter join");
     const unsigned join_threads = mfma_fn ? mfma_join_threads(stride) : kJoinThreads;
     const unsigned join_grid = static_cast<unsigned>(std::min<std::size_t>(n, 65536));
-    if (lds > 150 * 1024) throw std::runtime_error("nndescent: candidate tile exceeds LDS");
-    if (lds > 64 * 1024)
+    if (std::max(lds, lds_full) > 150 * 1024)
+        throw std::runtime_error("nndescent: candidate tile exceeds LDS");
+    if (std::max(lds, lds_full) > 64 * 1024)
         CSG_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(join_fn),
                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      static_cast<int>(lds)));
+                                      static_cast<int>(std::max(lds, lds_full))));
+    if (row_cap) {
+        b.defer.reserve(n);
+        b.defer_count.reserve(1);
+    }
     // Convergence (nndescent.cpp:172, 204) is decided on the device: the round
     // whose update count is at or below `stop` sets `halt`, and every kernel of
     // the rounds after it returns at once. The host enqueues round r + 1
@@ -1715,6 +1750,9 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     b.round_done.create();
     JoinArgs ja{pts, n, stride, smp, b.gids.p, b.gthr.p, b.gcnt.p, b.cand.p, b.fill.p, cap,
                 nullptr, nullptr, b.halt.p, cosine};
+    ja.row_cap = row_cap;
+    ja.defer = row_cap ? b.defer.p : nullptr;
+    ja.defer_count = row_cap ? b.defer_count.p : nullptr;
     const unsigned merge_blocks = static_cast<unsigned>(std::min<std::size_t>(n, kMergeBlocks));
     const bool tallies = gp && gp->profiling;
     const bool gram_audit = tallies && gram != kGramExact;
@@ -1799,8 +1837,16 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         {
             std::unique_ptr<ScopedKernelTimer> t;
             if (gp) t.reset(new ScopedKernelTimer(*gp, "nnd_join"));
+            if (row_cap) fill_u32(b.defer_count.p, 1, 0u, s);
             hipLaunchKernelGGL(join_fn, dim3(join_grid), dim3(join_threads), lds, s, ja);
             CSG_LAUNCH_CHECK();
+            if (row_cap) {  // the nodes set aside, all S rows staged
+                JoinArgs jd = ja;
+                jd.deferred_pass = true;
+                hipLaunchKernelGGL(join_fn, dim3(kDeferBlocks), dim3(join_threads), lds_full, s,
+                                   jd);
+                CSG_LAUNCH_CHECK();
+            }
         }
         if (tallies) {
             fill_u64(reinterpret_cast<std::uint64_t*>(b.tally.p), 3, 0ull, s);
@@ -1840,7 +1886,8 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
             reinterpret_cast<std::uint64_t>(ids), reinterpret_cast<std::uint64_t>(dist),
             reinterpret_cast<std::uint64_t>(flags), reinterpret_cast<std::uint64_t>(count), smp,
             iters, cap, gram, cosine ? 1u : 0u, reinterpret_cast<std::uint64_t>(join_fn),
-            lds, join_threads, join_grid, merge_blocks, rev_buckets ? nb : 0u, wave_blocks};
+            lds, lds_full, row_cap, join_threads, join_grid, merge_blocks, rev_buckets ? nb : 0u,
+            wave_blocks};
         std::uint64_t stop_bits;
         std::memcpy(&stop_bits, &stop, 8);
         key.push_back(stop_bits);
@@ -1855,7 +1902,9 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
                               static_cast<const void*>(b.rev_pairs.p), static_cast<const void*>(b.rev_hist.p),
                               static_cast<const void*>(b.rev_tmp.p), static_cast<const void*>(b.rev_total.p),
                               static_cast<const void*>(b.gthr.p), static_cast<const void*>(b.halt.p),
-                              static_cast<const void*>(b.readback.p)})
+                              static_cast<const void*>(b.readback.p),
+                              static_cast<const void*>(ja.defer),
+                              static_cast<const void*>(ja.defer_count)})
             key.push_back(reinterpret_cast<std::uint64_t>(q));
         if (!gp->descent_exec || gp->descent_key != key) {
             if (gp->descent_exec) {
