@@ -302,7 +302,7 @@ struct Graph {
     std::vector<unsigned char> sweep_key;
     // the descent's rounds as one captured graph, and what it was made for
     hipGraphExec_t descent_exec = nullptr;
-    std::vector<std::uint64_t> descent_key;
+    std::vector<std::uint64_t> descent_key, descent_key_seen;
     DBuf<float> out_w;
     // autotuned sweep launch shapes of this graph, by slots (power of two)
     std::map<std::uint32_t, SweepShape> sweep_tune;

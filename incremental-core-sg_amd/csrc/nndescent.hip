@@ -1874,12 +1874,14 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
         }
     };
     const bool plain = gp && gp->plain;
-    // Every round of the descent as one captured hipGraph, replayed per build
-    // (graph_rounds: a build, whose sizes repeat; inserts change n each time):
+    // Every round of the descent as one captured hipGraph, replayed by the
+    // builds of a graph after its first (graph_rounds: a build, whose sizes
+    // repeat; inserts change n each time):
     // the rounds after convergence return at once (the device's halt flag),
     // and the counts are read after the replay. Profiled builds keep the
     // eager launches (their per-kernel events), the plain form its host
     // synchronisation after every launch.
+    bool replay = false;
     if (graph_rounds && gp && !plain && !tallies) {
         std::vector<std::uint64_t> key = {
             reinterpret_cast<std::uint64_t>(pts), n, stride, k,
@@ -1906,12 +1908,18 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
                               static_cast<const void*>(ja.defer),
                               static_cast<const void*>(ja.defer_count)})
             key.push_back(reinterpret_cast<std::uint64_t>(q));
+        // A graph pays for its capture only when it is replayed: the first
+        // build of a key runs eagerly and remembers the key, the second
+        // captures, every later one replays.
         if (!gp->descent_exec || gp->descent_key != key) {
             if (gp->descent_exec) {
                 std::lock_guard<std::mutex> lk(device_api_mutex());
                 CSG_CHECK(hipGraphExecDestroy(gp->descent_exec));
             }
             gp->descent_exec = nullptr;
+            gp->descent_key.clear();
+        }
+        if (!gp->descent_exec && gp->descent_key_seen == key) {
             hipGraph_t graph = nullptr;
             {
                 CaptureGuard guard(s);
@@ -1927,6 +1935,10 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
             CSG_CHECK(e);
             gp->descent_key = key;
         }
+        replay = gp->descent_exec != nullptr;
+        gp->descent_key_seen = std::move(key);
+    }
+    if (replay) {
         CSG_CHECK(hipGraphLaunch(gp->descent_exec, s));
         CSG_CHECK(hipStreamSynchronize(s));
         for (std::uint32_t round = 0; round < iters; ++round)

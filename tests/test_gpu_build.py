@@ -210,8 +210,10 @@ def test_mfma_16x16x32_lane_layout(bf16):
 
 
 def test_captured_descent_replays_equal_eager():
-    """A build replays its descent rounds as one captured hipGraph; a profiled
-    build launches the same rounds eagerly. The descent's proposal order
+    """The builds of a graph after its first replay their descent rounds from
+    one captured hipGraph (the first unprofiled build runs eagerly and records
+    what the rounds depend on, the second captures); a profiled build launches
+    the same rounds eagerly. The descent's proposal order
     follows its atomics, so two eager builds agree bit for bit only when no
     reverse list or proposal buffer overflows. When they agree, every replay
     (three, with eager builds in between) must equal them bit for bit, rounds
@@ -238,7 +240,7 @@ def test_captured_descent_replays_equal_eager():
     (e1, s1), (e2, s2) = run(True), run(True)
     deterministic = same(e1, e2) and s1["rounds"] == s2["rounds"] and s1["updates"] == s2["updates"]
     ref = [exact_recall(e1[0], e1[3], xp, k), exact_recall(e2[0], e2[3], xp, k)]
-    for rep in range(3):
+    for rep in range(4):  # an eager build, a capture, two replays
         c, sc = run(False)
         assert oracle.lib().orc_validate(c[0], c[1], c[3], x.shape[0], k) == 1
         if deterministic:
