@@ -532,14 +532,17 @@ def main():
         step(None)
         print(f"[bench] warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
     # the sweep's launch shapes, autotuned on this graph for the batched sweep
-    # (32 slots), the 16-value sweep and single-mpts queries (an untimed build
-    # first when there was no warmup step)
+    # (32 slots), the 16-value sweep, single-mpts queries and 2, 4 and 8
+    # slots (the sizes of the serving scenario's extracts, whose graphs take
+    # the shape of the nearest tuned size bucket); an untimed build first when
+    # there was no warmup step
     if args.warmup == 0:
         g.build(params)
     g.prepare_edges()
     tuned = {}
     for name, mp in (("sweep32", mpts), ("sweep16", list(range(2, k + 1, 2))[:16]),
-                     ("single", [mpts[len(mpts) // 2]])):
+                     ("single", [mpts[len(mpts) // 2]]), ("sweep2", [8, 24]),
+                     ("sweep4", [4, 12, 20, 28]), ("sweep8", list(range(4, k + 1, 4))[:8])):
         tuned[name] = g.autotune_sweep(mp)
         print(f"[bench] autotune {name}: {tuned[name]['lanes']} lanes, <= "
               f"{tuned[name]['max_blocks']} blocks, {tuned[name]['best_ms']:.2f} ms "

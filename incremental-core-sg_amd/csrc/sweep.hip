@@ -928,13 +928,22 @@ TuneKey tune_key(const Graph& g, std::uint32_t Bp) {
 
 }  // namespace
 
+// The graph's own tuned shape for Bp slots, else the table's for its bucket,
+// else the table's for the nearest bucket with the same (k, Bp) at most two
+// size doublings away (the smaller one on a tie), else the default.
 SweepShape sweep_shape(const Graph& g, std::uint32_t Bp) {
     auto it = g.sweep_tune.find(Bp);
     if (it != g.sweep_tune.end()) return it->second;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        auto jt = g_tune.find(tune_key(g, Bp));
-        if (jt != g_tune.end()) return jt->second;
+        const TuneKey key = tune_key(g, Bp);
+        for (std::uint32_t dist = 0; dist <= 2; ++dist) {
+            for (const std::uint32_t l : {key.log2n - dist, key.log2n + dist}) {
+                if (l > 63) continue;  // below bucket 0
+                auto jt = g_tune.find(TuneKey{l, key.k, key.Bp});
+                if (jt != g_tune.end()) return jt->second;
+            }
+        }
     }
     return SweepShape{Bp > 16 ? 32u : 16u, 16384u};
 }
