@@ -63,6 +63,9 @@ def percentile(xs, p):
 TRAFFIC_SOURCES = {
     "nnd_join": ("incremental-core-sg_amd/csrc/nndescent.hip",
                  ("l2sqr_reg", "join_stage", "join_partner", "join_reg_kernel")),
+    "nnd_merge": ("incremental-core-sg_amd/csrc/nndescent.hip",
+                  ("merge_kernel", "small_unique_top", "sort_unique_top", "select_top_large",
+                   "wave_sort_regs", "count_before", "list_slot")),
 }
 
 
@@ -579,10 +582,12 @@ def main():
     ne, ke = g.prepare_edges()
     stride = (d + 7) // 8 * 8
     rooflines = {}
-    for name in ("nnd_join", "boruvka_round1"):
+    for name in ("nnd_join", "nnd_merge", "boruvka_round1"):
         if name not in kernels:
             continue
-        w = g.kernel_work(name)
+        if name == "nnd_merge" and "nnd_join" not in kernels:
+            continue
+        w = g.kernel_work("nnd_join" if name == "nnd_merge" else name)
         launches = kernels[name]["launches"]
         avg_s = kernels[name]["avg_ms"] * 1e-3
         if name == "nnd_join":
@@ -595,6 +600,13 @@ def main():
             extra = dict(flops_per_launch=flops, fp32_tflops=flops / avg_s / 1e12,
                          fp32_frac=flops / avg_s / 1e12 / FP32_PEAK_TFLOPS,
                          members_per_launch=rows / launches, proposals_per_launch=props / launches)
+        elif name == "nnd_merge":
+            # per launch (one per join launch): every node's list (ids, distances,
+            # flags: 9k B), its count and proposal-fill words (8 B) and every key
+            # the join offered it (8 B); the lists it rewrites are left out
+            rows, dists, props, nodes = w
+            algo = (nodes * (9 * k + 8) + props * 8) / launches
+            extra = dict(proposals_per_launch=props / launches)
         else:
             # per launch (one launch serves B mpts slots): every adjacency
             # entry (neighbour, rank, base weight: 12 B) plus the neighbour's

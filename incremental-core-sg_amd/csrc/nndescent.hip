@@ -1341,6 +1341,9 @@ __device__ __forceinline__ std::uint32_t count_before(const float* a, std::uint3
     return pos;
 }
 
+// Rows of 64 proposal keys the merge loads before it tests them.
+constexpr std::uint32_t kMergeBatch = 4;
+
 // Set of a list's ids for the merge's listed test (at most kMaxK of kListSet
 // slots used).
 constexpr std::uint32_t kListSet = 2 * kMaxK;
@@ -1395,27 +1398,39 @@ __global__ void __launch_bounds__(64)
                 while (atomicCAS(&s_list[h], kNone, my_id) != kNone) h = (h + 1) & (kListSet - 1);
             }
             __syncthreads();
-            // 1. per-candidate tests, survivors compacted into s_key
+            // 1. per-candidate tests, survivors compacted into s_key; the
+            // keys are loaded kMergeBatch rows of 64 at a time, so a batch
+            // waits for memory once
             std::uint32_t nv = 0;
-            for (std::uint32_t i0 = 0; i0 < m; i0 += 64) {
-                const std::uint32_t i = i0 + lane;
-                const unsigned long long key = i < m ? cand[std::size_t(v) * cap + i] : ~0ull;
-                const std::uint32_t cid = static_cast<std::uint32_t>(key);
-                const float cd = __uint_as_float(static_cast<std::uint32_t>(key >> 32));
-                bool ok = key != ~0ull && cid != v && cid < id_limit && cd < last;
-                if (ok) {
-                    for (std::uint32_t h = list_slot(cid);; h = (h + 1) & (kListSet - 1)) {
-                        const std::uint32_t x = s_list[h];
-                        if (x == cid) {
-                            ok = false;
-                            break;
-                        }
-                        if (x == kNone) break;
-                    }
+            const unsigned long long* ck = cand + std::size_t(v) * cap;
+            for (std::uint32_t b0 = 0; b0 < m; b0 += 64 * kMergeBatch) {
+                unsigned long long kb[kMergeBatch];
+#pragma unroll
+                for (std::uint32_t r = 0; r < kMergeBatch; ++r) {
+                    const std::uint32_t i = b0 + r * 64 + lane;
+                    kb[r] = i < m ? ck[i] : ~0ull;
                 }
-                const std::uint64_t vm = __ballot(ok);
-                if (ok) s_key[nv + __popcll(vm & lt)] = key;
-                nv += __popcll(vm);
+#pragma unroll
+                for (std::uint32_t r = 0; r < kMergeBatch; ++r) {
+                    if (b0 + r * 64 >= m) break;  // wavefront-uniform
+                    const unsigned long long key = kb[r];
+                    const std::uint32_t cid = static_cast<std::uint32_t>(key);
+                    const float cd = __uint_as_float(static_cast<std::uint32_t>(key >> 32));
+                    bool ok = key != ~0ull && cid != v && cid < id_limit && cd < last;
+                    if (ok) {
+                        for (std::uint32_t h = list_slot(cid);; h = (h + 1) & (kListSet - 1)) {
+                            const std::uint32_t x = s_list[h];
+                            if (x == cid) {
+                                ok = false;
+                                break;
+                            }
+                            if (x == kNone) break;
+                        }
+                    }
+                    const std::uint64_t vm = __ballot(ok);
+                    if (ok) s_key[nv + __popcll(vm & lt)] = key;
+                    nv += __popcll(vm);
+                }
             }
             __syncthreads();
             // 2. sort, keep first copies, rank

@@ -44,3 +44,11 @@ def test_committed_join_traffic_matches_its_source():
     assert traffic is not None and gathered is not None, \
         "profiles/traffic.json no longer describes the join's source: profile it again"
     assert traffic > gathered > 0
+
+
+def test_committed_merge_traffic_matches_its_source():
+    b = _bench()
+    traffic, gathered = b.measured_traffic("nnd_merge")
+    assert traffic is not None and gathered is not None, \
+        "profiles/traffic.json no longer describes the merge's source: profile it again"
+    assert traffic > gathered > 0
