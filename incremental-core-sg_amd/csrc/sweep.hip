@@ -52,6 +52,10 @@ constexpr unsigned kReduceBlocks = 1024;  // blocks of the per-slot reducing ker
 constexpr std::uint32_t kErrEdgeIndex = 1u, kErrHookCycle = 2u, kErrHookCount = 4u,
                         kErrTooManyEdges = 8u;
 constexpr std::uint32_t kGraphRounds = 4;  // Boruvka rounds per replay of the captured graph
+// adjacency entries a round-kernel lane has in flight (each with the
+// neighbour's core distance and component behind it); 6 and 8 measured no
+// faster at n = 1M (fewer waves per SIMD)
+constexpr int kSweepInFlight = 4;
 constexpr std::uint64_t kOver = 1ull << 63;  // saturated checked sum
 constexpr std::uint64_t kExpInf = 1ull << 32;
 
@@ -214,21 +218,22 @@ __global__ void __launch_bounds__(kThreads) sweep_round_kernel(Sweep s) {
             const std::uint32_t e1 = s.off[v + 1];
             std::uint32_t j = s.off[v] + ph;
             if (act) {
-                // four entries in flight per lane
-                for (; j + 3 * P < e1; j += 4 * P) {
-                    AdjEntry a[4];
+                // kSweepInFlight entries in flight per lane
+                constexpr int F = kSweepInFlight;
+                for (; j + (F - 1) * P < e1; j += F * P) {
+                    AdjEntry a[F];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) a[u] = s.adj[j + u * P];
-                    std::uint32_t cq_comp[4];
-                    float cq[4];
+                    for (int u = 0; u < F; ++u) a[u] = s.adj[j + u * P];
+                    std::uint32_t cq_comp[F];
+                    float cq[F];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
+                    for (int u = 0; u < F; ++u) {
                         const std::size_t iq = std::size_t(a[u].nbr & ~kSecondEnd) * s.B + b;
                         cq[u] = s.core[iq];
                         cq_comp[u] = FIRST ? ~0u : s.comp[iq];
                     }
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
+                    for (int u = 0; u < F; ++u) {
                         if (!FIRST && cq_comp[u] == cv) continue;
                         const std::uint64_t key = entry_key(a[u].nbr, a[u].w, cx, cq[u], a[u].rank);
                         key_min = key < key_min ? key : key_min;
