@@ -12,6 +12,9 @@
 #include <cstring>
 #include <vector>
 
+#include <sys/syscall.h>
+#include <unistd.h>
+
 #include "build.hpp"
 #include "hubness.hpp"
 #include "scan.hpp"
@@ -370,7 +373,21 @@ struct MrdCache {
         stream = nullptr;
         device = -1;
     }
-    ~MrdCache() { release(); }
+    // The main thread's thread_local objects are destroyed only when the
+    // process exits, when the HIP runtime (and any tool attached to it, such
+    // as a profiler, which failed on this call) may already be torn down:
+    // there the device memory is left to the process's end. Other threads
+    // release theirs when they exit.
+    ~MrdCache() {
+        if (static_cast<pid_t>(syscall(SYS_gettid)) == getpid()) {
+            pts.forget();
+            core.forget();
+            out.forget();
+            stream = nullptr;
+            return;
+        }
+        release();
+    }
 };
 
 thread_local MrdCache t_cache;

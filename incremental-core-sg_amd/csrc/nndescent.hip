@@ -500,6 +500,9 @@ struct JoinArgs {
     std::uint32_t* defer = nullptr;
     std::uint32_t* defer_count = nullptr;
     bool deferred_pass = false;
+    // member-stationary join only: entries per wavefront of the LDS list of
+    // passing pairs (the rest are recomputed in the emit pass)
+    std::uint32_t list_cap = 0;
 };
 
 // Exact squared distance with `a` held in registers (compile-time stride),
@@ -543,6 +546,65 @@ __device__ __forceinline__ float l2sqr_reg(const float (&a)[STRIDE],
     for (int j = 0; j < 8; ++j) v[j] = acc0[j] + acc1[j];
     const float l0 = v[0] + v[4], l1 = v[1] + v[5], l2 = v[2] + v[6], l3 = v[3] + v[7];
     return (l0 + l1) + (l2 + l3);
+}
+
+// l2sqr_reg's value from the same operations on each accumulator in the same
+// order, computed four accumulator lanes at a time: accumulators j = 4q .. 4q+3
+// take coordinates 16c + j and 16c + 8 + j of every 16-wide chunk c, then
+// the 8-wide tail, for q = 0 and then q = 1. An empty register fence after
+// the first half keeps its sums computed, and the second half's loads
+// issued, on either side of it, so only half of the other row is held in
+// registers at once.
+template <std::uint32_t STRIDE>
+__device__ __forceinline__ float l2sqr_reg_halves(const float (&a)[STRIDE],
+                                                  const float* __restrict__ b) {
+    constexpr std::uint32_t M = STRIDE / 16;
+    float v[8];
+#pragma unroll
+    for (std::uint32_t q = 0; q < 2; ++q) {
+        float acc0[4], acc1[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            acc0[u] = 0.0f;
+            acc1[u] = 0.0f;
+        }
+#pragma unroll
+        for (std::uint32_t c = 0; c < M; ++c) {
+            const float4 b0 = *reinterpret_cast<const float4*>(b + 16 * c + 4 * q);
+            const float4 b1 = *reinterpret_cast<const float4*>(b + 16 * c + 8 + 4 * q);
+            const float x0[4] = {b0.x, b0.y, b0.z, b0.w};
+            const float x1[4] = {b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float d0 = a[16 * c + 4 * q + u] - x0[u];
+                const float d1 = a[16 * c + 8 + 4 * q + u] - x1[u];
+                acc0[u] = __builtin_fmaf(d0, d0, acc0[u]);
+                acc1[u] = __builtin_fmaf(d1, d1, acc1[u]);
+            }
+        }
+        if constexpr (STRIDE % 16 == 8) {
+            const float4 b0 = *reinterpret_cast<const float4*>(b + 16 * M + 4 * q);
+            const float x0[4] = {b0.x, b0.y, b0.z, b0.w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float d0 = a[16 * M + 4 * q + u] - x0[u];
+                acc0[u] = __builtin_fmaf(d0, d0, acc0[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[4 * q + u] = acc0[u] + acc1[u];
+        if (q == 0) asm volatile("" : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3])::"memory");
+    }
+    const float l0 = v[0] + v[4], l1 = v[1] + v[5], l2 = v[2] + v[6], l3 = v[3] + v[7];
+    return (l0 + l1) + (l2 + l3);
+}
+
+// The member-stationary join's exact distance: l2sqr_reg_halves where its
+// lower register count pays, l2sqr_reg at stride 48 (measured faster there).
+template <std::uint32_t STRIDE>
+__device__ __forceinline__ float l2sqr_join(const float (&a)[STRIDE], const float* __restrict__ b) {
+    if constexpr (STRIDE == 48) return l2sqr_reg<STRIDE>(a, b);
+    else return l2sqr_reg_halves<STRIDE>(a, b);
 }
 
 // Shared phases of the two local-join kernels. One 128-thread group (two
@@ -693,18 +755,42 @@ __device__ __forceinline__ std::uint32_t join_partner(bool circ, std::uint32_t j
     return j >= o ? j - o : j + na - o;
 }
 
+// Largest value over the wavefront's lanes.
+__device__ __forceinline__ unsigned int wave_max_u32(unsigned int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned int o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// Number of set bits of a wavefront mask below the calling lane.
+__device__ __forceinline__ std::uint32_t lanes_below(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<std::uint32_t>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<std::uint32_t>(m), 0u));
+}
+
 // Exact local join, member-stationary, for rows of at most kJoinRegStride
 // floats and at most kJoinThreads members: thread t keeps member j = t % ns
 // (its row, threshold and proposal count) in registers and visits the new
 // rows i = t / ns, + G, ... (G = kJoinThreads / ns; the threads left over
-// give the last members one more phase), so a pair reads one row from LDS. The distance pass keeps, per thread, one bit per visited pair
-// for each proposal direction instead of the distance; after the per-member
-// reservation the emit pass recomputes the distances of the pairs with a set
-// bit (the same function on the same rows: the same bits). With no distance
-// tile a node needs LDS for ids[S], thr[S], cnt[S], base[S] and rows[S*STRIDE]
-// only, which doubles the blocks a CU holds.
-template <std::uint32_t STRIDE>
-__global__ void __launch_bounds__(kJoinThreads) join_reg_kernel(JoinArgs a) {
+// give the last members one more phase), so a pair reads one row from LDS.
+// A visited pair that passes a threshold test is appended, with its exact
+// distance, to its wavefront's LDS list (list_cap entries) at the ballot's
+// prefix count over a running length that is the same in every lane still
+// visiting (no atomics); after the per-member reservation each wavefront
+// writes its list's keys one entry per lane. Passing pairs beyond the list's
+// capacity keep one bit per visited pair and direction (Mask: 32 bits when a
+// lane visits at most 2 smp <= 32 pairs), and the emit pass recomputes their
+// distances (the same function on the same rows: the same bits). With no
+// distance tile a node needs LDS for ids[S], thr[S], cnt[S], base[S],
+// rows[S*STRIDE] and the two lists only.
+// (At stride 64 the register budget of four waves per SIMD is enforced: it
+// spills two loop-invariant registers, and three waves would cost more.)
+template <std::uint32_t STRIDE, typename Mask>
+__global__ void __launch_bounds__(kJoinThreads)
+    __attribute__((amdgpu_waves_per_eu(STRIDE == 64 ? 4 : 1))) join_reg_kernel(JoinArgs a) {
     static_assert(STRIDE % 8 == 0 && STRIDE <= kJoinRegStride, "row kept in registers");
     extern __shared__ float lds[];
     if (*a.halt) return;
@@ -714,7 +800,11 @@ __global__ void __launch_bounds__(kJoinThreads) join_reg_kernel(JoinArgs a) {
     std::uint32_t* s_cnt = reinterpret_cast<std::uint32_t*>(lds + 2 * S);
     std::uint32_t* s_base = s_cnt + S;
     float* s_rows = lds + 4 * S;
-    const std::uint32_t tid = threadIdx.x;
+    const std::uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const std::uint32_t L = a.list_cap;
+    // entry: exact distance bits, i | j << 8 | to_i << 16 | to_j << 17
+    uint2* s_list = reinterpret_cast<uint2*>(s_rows + S * STRIDE) +
+                    __builtin_amdgcn_readfirstlane(tid >> 6) * L;
     for (std::uint32_t v = blockIdx.x; v < a.n; v += gridDim.x) {
         std::uint32_t na, ns;
         for (std::uint32_t x = tid; x < S; x += kJoinThreads) s_cnt[x] = 0;
@@ -744,7 +834,11 @@ __global__ void __launch_bounds__(kJoinThreads) join_reg_kernel(JoinArgs a) {
         const std::uint32_t iend =
             circ ? (na - 1) / 2 + ((na % 2 == 0 && j >= na / 2) ? 1u : 0u) : (j < na ? j : na);
         float rj[STRIDE];
-        std::uint64_t to_i = 0, to_j = 0;  // bit r: pair (partner i0 + r G, j) proposes
+        // bit r: pair (partner i0 + r G, j) proposes and did not fit the list
+        Mask to_i = 0, to_j = 0;
+        // the list's running length: equal in the lanes still visiting,
+        // never above it in the lanes that left
+        std::uint32_t run = 0, cj = 0;
         {
             const float4* src = reinterpret_cast<const float4*>(s_rows + j * STRIDE);
 #pragma unroll
@@ -757,23 +851,35 @@ __global__ void __launch_bounds__(kJoinThreads) join_reg_kernel(JoinArgs a) {
             }
             const float thr_j = s_thr[j];
             // the partner order is block-uniform (na == ns): one loop each
-            auto visit = [&](std::uint32_t i, std::uint64_t bit) {
-                const float d = l2sqr_reg<STRIDE>(rj, s_rows + i * STRIDE);
-                if (d < s_thr[i]) {
-                    to_i |= bit;
-                    atomicAdd(&s_cnt[i], 1u);
+            auto visit = [&](std::uint32_t i, Mask bit) {
+                const float d = l2sqr_join<STRIDE>(rj, s_rows + i * STRIDE);
+                const bool pi = d < s_thr[i], pj = d < thr_j;
+                if (pi) atomicAdd(&s_cnt[i], 1u);
+                cj += pj ? 1u : 0u;
+                const bool any = pi || pj;
+                const unsigned long long bal = __ballot(any);
+                const std::uint32_t pos = run + lanes_below(bal);
+                run += static_cast<std::uint32_t>(__popcll(bal));
+                if (any) {
+                    if (pos < L) {
+                        s_list[pos] = make_uint2(fbits(d), i | (j << 8) | (pi ? 0x10000u : 0u) |
+                                                               (pj ? 0x20000u : 0u));
+                    } else {
+                        if (pi) to_i |= bit;
+                        if (pj) to_j |= bit;
+                    }
                 }
-                if (d < thr_j) to_j |= bit;
             };
-            std::uint64_t bit = 1;
+            Mask bit = 1;
             if (na == ns) {
                 for (std::uint32_t t = i0; t < iend; t += G, bit <<= 1)
                     visit(join_partner(true, j, na, t), bit);
             } else {
                 for (std::uint32_t t = i0; t < iend; t += G, bit <<= 1) visit(t, bit);
             }
-            if (to_j) atomicAdd(&s_cnt[j], static_cast<std::uint32_t>(__popcll(to_j)));
+            if (cj) atomicAdd(&s_cnt[j], cj);
         }
+        run = wave_max_u32(run);
         __syncthreads();
         // one reservation per receiving member, then write at LDS cursors
         for (std::uint32_t x = tid; x < ns; x += kJoinThreads) {
@@ -782,23 +888,43 @@ __global__ void __launch_bounds__(kJoinThreads) join_reg_kernel(JoinArgs a) {
             s_cnt[x] = 0;
         }
         __syncthreads();
+        // the wavefront's listed pairs, one per lane
+        const std::uint32_t len = run < L ? run : L;
+        for (std::uint32_t e = lane; e < len; e += 64) {
+            const uint2 en = s_list[e];
+            const std::uint32_t ii = en.y & 0xFFu, jj = (en.y >> 8) & 0xFFu;
+            const unsigned long long hb =
+                static_cast<unsigned long long>(
+                    fbits(metric_finalize(__uint_as_float(en.x), a.cosine)))
+                << 32;
+            if (en.y & 0x10000u) {
+                const std::uint32_t pos = s_base[ii] + atomicAdd(&s_cnt[ii], 1u);
+                if (pos < a.cap) a.cand[std::size_t(s_id[ii]) * a.cap + pos] = hb | s_id[jj];
+            }
+            if (en.y & 0x20000u) {
+                const std::uint32_t pos = s_base[jj] + atomicAdd(&s_cnt[jj], 1u);
+                if (pos < a.cap) a.cand[std::size_t(s_id[jj]) * a.cap + pos] = hb | s_id[ii];
+            }
+        }
+        // the passing pairs the list did not hold: recomputed
         if (to_i | to_j) {
             const std::uint32_t id_j = s_id[j];
             std::uint32_t pj = to_j ? s_base[j] + atomicAdd(&s_cnt[j],
                                                             static_cast<std::uint32_t>(__popcll(to_j)))
                                     : 0u;
-            std::uint64_t m = to_i | to_j;
+            Mask m = to_i | to_j;
             while (m) {
-                const std::uint32_t r = static_cast<std::uint32_t>(__ffsll(static_cast<unsigned long long>(m)) - 1);
+                const std::uint32_t r =
+                    static_cast<std::uint32_t>(__ffsll(static_cast<unsigned long long>(m)) - 1);
                 m &= m - 1;
                 const std::uint32_t i = join_partner(circ, j, na, i0 + r * G);
                 // a direction whose range starts at the capacity writes
                 // nothing: skip it, and the distance when both do
-                const bool wi = ((to_i >> r) & 1ull) && s_base[i] < a.cap;
-                const bool dj = (to_j >> r) & 1ull;
+                const bool wi = ((to_i >> r) & 1u) && s_base[i] < a.cap;
+                const bool dj = (to_j >> r) & 1u;
                 const bool wj = dj && pj < a.cap;
                 if (wi || wj) {
-                    const float d = l2sqr_reg<STRIDE>(rj, s_rows + i * STRIDE);
+                    const float d = l2sqr_join<STRIDE>(rj, s_rows + i * STRIDE);
                     const unsigned long long hb =
                         static_cast<unsigned long long>(fbits(metric_finalize(d, a.cosine))) << 32;
                     if (wi) {
@@ -834,15 +960,6 @@ __global__ void __launch_bounds__(kJoinThreads) join_reg_kernel(JoinArgs a) {
 // +inf; every other pair gets its exact distance. The proposals are therefore
 // exactly those of the exact join: the matrix cores only decide which exact
 // distances are needed.
-
-__device__ __forceinline__ unsigned int wave_max_u32(unsigned int v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned int o = __shfl_xor(v, off, 64);
-        v = o > v ? o : v;
-    }
-    return v;
-}
 
 // Eight coordinates k..k+7 of LDS row r centred on row 0 (zeros past the
 // stride or for rows at or beyond `rows`).
@@ -1610,16 +1727,32 @@ JoinFn pick_join(std::uint32_t stride) {
 }
 
 // Member-stationary exact join for the strides it is built for, else nullptr.
-JoinFn pick_join_reg(std::uint32_t stride) {
+template <typename Mask>
+JoinFn pick_join_reg_t(std::uint32_t stride) {
     switch (stride) {
-        case 8: return join_reg_kernel<8>;
-        case 16: return join_reg_kernel<16>;
-        case 24: return join_reg_kernel<24>;
-        case 32: return join_reg_kernel<32>;
-        case 48: return join_reg_kernel<48>;
-        case 64: return join_reg_kernel<64>;
+        case 8: return join_reg_kernel<8, Mask>;
+        case 16: return join_reg_kernel<16, Mask>;
+        case 24: return join_reg_kernel<24, Mask>;
+        case 32: return join_reg_kernel<32, Mask>;
+        case 48: return join_reg_kernel<48, Mask>;
+        case 64: return join_reg_kernel<64, Mask>;
         default: return nullptr;
     }
+}
+
+// Entries per wavefront of the member-stationary join's list of passing
+// pairs. MI355X, 1M points, ms per join launch without a list / with 48
+// entries: d = 8 1.31 / 1.29, 16 1.61 / 1.58, 24 2.34 / 2.26, 32 3.44 /
+// 3.34, 64 9.08 / 8.78; 64 and 96 entries cost the stride-24 join a
+// block per CU for nothing (2.33), 160 two (2.38). At stride 48 a list of 32
+// keeps its eight blocks per CU (4.25 / 4.14).
+std::uint32_t join_list_cap(std::uint32_t stride) { return stride == 48 ? 32u : 48u; }
+
+// A lane of the member-stationary join visits at most A = 2 smp pairs: a
+// 32-bit overflow mask serves samples of up to 16.
+JoinFn pick_join_reg(std::uint32_t stride, std::uint32_t smp) {
+    return 2u * smp <= 32u ? pick_join_reg_t<std::uint32_t>(stride)
+                           : pick_join_reg_t<std::uint64_t>(stride);
 }
 
 template <bool BF16, bool AUDIT>
@@ -1724,9 +1857,9 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
                                      : pick_join_mfma_t<true, false>(stride))
                             : nullptr;
     if (!mfma_fn) gram = kGramExact;
-    const bool join_reg = gram == kGramExact && S <= kJoinThreads && pick_join_reg(stride);
+    const bool join_reg = gram == kGramExact && S <= kJoinThreads && pick_join_reg(stride, smp);
     const JoinFn join_fn = mfma_fn    ? mfma_fn
-                           : join_reg ? pick_join_reg(stride)
+                           : join_reg ? pick_join_reg(stride, smp)
                                       : pick_join(stride);
     // the filter join's first pass stages at most row_cap rows per node (0:
     // all S); its second pass, over the nodes set aside, stages S
@@ -1736,10 +1869,12 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
                kMfmaList * sizeof(std::uint16_t) + sizeof(std::uint32_t);
     };
     const std::size_t lds_full = mfma_fn ? mfma_lds(S) : 0;
+    const std::uint32_t list_cap = join_reg ? join_list_cap(stride) : 0u;
     const std::size_t lds =
         mfma_fn ? mfma_lds(row_cap ? row_cap : S)
         : join_reg
-            ? (4 * std::size_t(S) + std::size_t(S) * stride) * sizeof(float)
+            ? (4 * std::size_t(S) + std::size_t(S) * stride) * sizeof(float) +
+                  2 * std::size_t(list_cap) * sizeof(uint2)
             : (4 * std::size_t(S) + std::size_t(S) * stride + std::size_t(A) * S) * sizeof(float);
     if (mfma_fn && S > 1024u) throw std::runtime_error("nndescent: sample too large for the filter join");
     const unsigned join_threads = mfma_fn ? mfma_join_threads(stride) : kJoinThreads;
@@ -1766,6 +1901,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     JoinArgs ja{pts, n, stride, smp, b.gids.p, b.gthr.p, b.gcnt.p, b.cand.p, b.fill.p, cap,
                 nullptr, nullptr, b.halt.p, cosine};
     ja.row_cap = row_cap;
+    ja.list_cap = list_cap;
     ja.defer = row_cap ? b.defer.p : nullptr;
     ja.defer_count = row_cap ? b.defer_count.p : nullptr;
     const unsigned merge_blocks = static_cast<unsigned>(std::min<std::size_t>(n, kMergeBlocks));
@@ -1904,7 +2040,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
             reinterpret_cast<std::uint64_t>(flags), reinterpret_cast<std::uint64_t>(count), smp,
             iters, cap, gram, cosine ? 1u : 0u, reinterpret_cast<std::uint64_t>(join_fn),
             lds, lds_full, row_cap, join_threads, join_grid, merge_blocks, rev_buckets ? nb : 0u,
-            wave_blocks};
+            wave_blocks, list_cap};
         std::uint64_t stop_bits;
         std::memcpy(&stop_bits, &stop, 8);
         key.push_back(stop_bits);
