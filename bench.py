@@ -62,7 +62,7 @@ def percentile(xs, p):
 # fingerprinted, so a change to one of them marks the entry stale.
 TRAFFIC_SOURCES = {
     "nnd_join": ("incremental-core-sg_amd/csrc/nndescent.hip",
-                 ("l2sqr_reg", "l2sqr_reg_halves", "l2sqr_join", "join_stage", "join_partner",
+                 ("l2sqr_reg", "l2sqr_reg_halves", "l2sqr_join", "join_stage_reg", "join_partner",
                   "lanes_below", "wave_max_u32", "join_reg_kernel", "join_list_cap")),
     "nnd_merge": ("incremental-core-sg_amd/csrc/nndescent.hip",
                   ("merge_kernel", "small_unique_top", "sort_unique_top", "select_top_large",

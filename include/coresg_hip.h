@@ -103,7 +103,8 @@ int coresg_hip_enable_peer_access(const int* devices, size_t count);
 /* cuda_api.hpp:84 (device_memory). */
 int coresg_hip_device_memory(int device, size_t* free_bytes, size_t* total_bytes);
 /* cuda_api.hpp:86 (release_thread_cache): drop this thread's cached MRD
- * operands. */
+ * operands, and those of threads that have exited since (an exiting thread
+ * makes no HIP call; its cache is freed here or by the next MRD call). */
 void coresg_hip_release_thread_cache(void);
 
 /* No counterpart in the reference. For callers that cannot allocate device

@@ -78,12 +78,6 @@ struct DBuf {
         CSG_CHECK(hipMalloc(reinterpret_cast<void**>(&p), bytes));
         n = count;
     }
-    // Drop the allocation without freeing it (the process is ending and the
-    // runtime may be gone; the driver reclaims device memory with it).
-    void forget() {
-        p = nullptr;
-        n = 0;
-    }
     // Take over another buffer's allocation (this one's is freed).
     void adopt(DBuf& o) {
         release();

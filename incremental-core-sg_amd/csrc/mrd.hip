@@ -10,10 +10,8 @@
 //     distance in global ids.
 //   * mrd_row_min (cuda_api.hpp:74-78): per-row minimum across components.
 #include <cstring>
+#include <mutex>
 #include <vector>
-
-#include <sys/syscall.h>
-#include <unistd.h>
 
 #include "build.hpp"
 #include "hubness.hpp"
@@ -373,27 +371,53 @@ struct MrdCache {
         stream = nullptr;
         device = -1;
     }
-    // The main thread's thread_local objects are destroyed only when the
-    // process exits, when the HIP runtime (and any tool attached to it, such
-    // as a profiler, which failed on this call) may already be torn down:
-    // there the device memory is left to the process's end. Other threads
-    // release theirs when they exit.
-    ~MrdCache() {
-        if (static_cast<pid_t>(syscall(SYS_gettid)) == getpid()) {
-            pts.forget();
-            core.forget();
-            out.forget();
-            stream = nullptr;
-            return;
-        }
-        release();
-    }
 };
 
-thread_local MrdCache t_cache;
+// Caches of threads that have exited, freed by the next call on a live thread
+// (a leaked, never destroyed list: threads can exit during static destruction).
+struct RetiredCaches {
+    std::mutex mu;
+    std::vector<MrdCache*> list;
+};
+RetiredCaches& retired_caches() {
+    static RetiredCaches* r = new RetiredCaches();
+    return *r;
+}
+
+void free_retired_caches() {
+    std::vector<MrdCache*> r;
+    {
+        std::lock_guard<std::mutex> lk(retired_caches().mu);
+        r.swap(retired_caches().list);
+    }
+    for (MrdCache* c : r) {
+        c->release();
+        delete c;
+    }
+}
+
+// The calling thread's cache. An exiting thread makes no HIP call: by then the
+// runtime, or a tool attached to it, may have torn down its state for the
+// thread (a profiler failed on exactly that call), so the cache is handed to
+// the retired list; at process exit the driver reclaims what is left.
+struct MrdCacheHolder {
+    MrdCache* c = nullptr;
+    MrdCache& get() {
+        if (!c) c = new MrdCache();
+        return *c;
+    }
+    ~MrdCacheHolder() {
+        if (!c) return;
+        std::lock_guard<std::mutex> lk(retired_caches().mu);
+        retired_caches().list.push_back(c);
+    }
+};
+thread_local MrdCacheHolder t_holder;
 
 void ensure_resident(int device, const float* host_pts, std::size_t m, std::uint32_t stride,
                      const float* host_core, bool check_points) {
+    free_retired_caches();
+    MrdCache& t_cache = t_holder.get();
     const std::size_t np = m * stride;
     if (t_cache.device == device && t_cache.pts_key == host_pts &&
         t_cache.core_key == host_core && t_cache.m == m && t_cache.stride == stride) {
@@ -575,6 +599,7 @@ void mrd_block_host(int device, const float* pts, std::size_t m, std::uint32_t d
     if (stride % 8 != 0) throw std::runtime_error("mrd_block: stride must be a multiple of 8");
     if (r1 > m) throw std::runtime_error("mrd_block: row range exceeds m");
     ensure_resident(device, pts, m, stride, core, r0 == 0);
+    MrdCache& t_cache = t_holder.get();
     const std::size_t rows = r1 - r0;
     t_cache.out.reserve(rows * m);
     launch_mrd(t_cache.pts.p, static_cast<std::uint32_t>(m), stride, t_cache.core.p,
@@ -620,6 +645,9 @@ void mrd_row_min_host(int device, const float* pts, std::size_t m, std::uint32_t
     CSG_CHECK(hipStreamSynchronize(s));
 }
 
-void release_thread_cache() { t_cache.release(); }
+void release_thread_cache() {
+    free_retired_caches();
+    if (t_holder.c) t_holder.c->release();
+}
 
 }  // namespace coresg_hip

@@ -640,6 +640,52 @@ __device__ __forceinline__ bool join_stage(const JoinArgs& a, std::uint32_t v, s
     return true;
 }
 
+// Per member count ns <= kJoinThreads: ceil(65536 / ns), and kJoinThreads / ns,
+// read from constant memory (ns is the same in the whole block, so these are
+// scalar loads instead of a division on the vector unit).
+struct JoinDivTable {
+    std::uint32_t recip16[kJoinThreads + 1];
+    std::uint32_t per_member[kJoinThreads + 1];
+    constexpr JoinDivTable() : recip16(), per_member() {
+        for (std::uint32_t ns = 1; ns <= kJoinThreads; ++ns) {
+            recip16[ns] = (65536u + ns - 1) / ns;
+            per_member[ns] = kJoinThreads / ns;
+        }
+    }
+};
+__constant__ JoinDivTable c_join_div = JoinDivTable();
+
+// join_stage for the member-stationary join: thread t stages float4 t % Q of
+// members t / Q, + kJoinThreads / Q, ... (Q = STRIDE / 4 float4 per row; the
+// kJoinThreads % Q threads left over idle), so the index arithmetic is fixed
+// per thread instead of a division per float4.
+template <std::uint32_t STRIDE>
+__device__ __forceinline__ bool join_stage_reg(const JoinArgs& a, std::uint32_t v,
+                                               std::uint32_t S, std::uint32_t* s_id,
+                                               float* s_thr, float* s_rows,
+                                               std::uint32_t* na_out, std::uint32_t* ns_out) {
+    constexpr std::uint32_t Q = STRIDE / 4, MSTEP = kJoinThreads / Q;
+    const std::uint32_t tid = threadIdx.x;
+    const std::uint32_t cnt = a.cnt[v];
+    const std::uint32_t na = cnt & 0xFFFFu, ns = cnt >> 16;
+    *na_out = na;
+    *ns_out = ns;
+    if (na == 0) return false;  // block-uniform
+    for (std::uint32_t i = tid; i < ns; i += kJoinThreads) {
+        s_id[i] = a.ids[std::size_t(v) * S + i];
+        s_thr[i] = a.thr[std::size_t(v) * S + i];
+    }
+    __syncthreads();
+    const std::uint32_t qi = tid / Q, qf = tid - qi * Q;
+    const float4* src = reinterpret_cast<const float4*>(a.pts);
+    float4* dst = reinterpret_cast<float4*>(s_rows);
+    if (qi < MSTEP)
+        for (std::uint32_t m = qi; m < ns; m += MSTEP)
+            dst[m * Q + qf] = src[std::size_t(s_id[m]) * Q + qf];
+    __syncthreads();
+    return true;
+}
+
 // A thread's walk over the (new i, member j) pairs p = i * ns + j, p = p0,
 // p0 + step, ...: (i, j) starts at (p0 / ns, p0 % ns) and advances by
 // (step / ns, step % ns) with one carry, so no pair costs a division. The
@@ -808,14 +854,14 @@ __global__ void __launch_bounds__(kJoinThreads)
     for (std::uint32_t v = blockIdx.x; v < a.n; v += gridDim.x) {
         std::uint32_t na, ns;
         for (std::uint32_t x = tid; x < S; x += kJoinThreads) s_cnt[x] = 0;
-        if (!join_stage(a, v, S, STRIDE, s_id, s_thr, s_rows, &na, &ns)) continue;
+        if (!join_stage_reg<STRIDE>(a, v, S, s_id, s_thr, s_rows, &na, &ns)) continue;
         // G0 = 128 / ns threads per member (i0 = 0..G0-1, step G0); the
         // E = 128 - G0 ns threads left over become an extra phase (i0 = G0)
         // of the last E members, which then step by G0 + 1: every thread has
         // a member
-        const std::uint32_t G0 = kJoinThreads / ns, E = kJoinThreads - G0 * ns;
+        const std::uint32_t G0 = c_join_div.per_member[ns], E = kJoinThreads - G0 * ns;
         // t / ns by a rounded-up 16-bit reciprocal: exact for t < 128, ns < 500
-        std::uint32_t i0 = (tid * ((65536u + ns - 1) / ns)) >> 16;
+        std::uint32_t i0 = (tid * c_join_div.recip16[ns]) >> 16;
         std::uint32_t j = tid - i0 * ns;
         if (i0 >= G0) {  // an extra thread
             j = ns - E + (tid - G0 * ns);
