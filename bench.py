@@ -218,6 +218,7 @@ def serving_run(chip, datagen, k, seed_n, batches, insert_rate, extract_rate, bu
     run_ms = {"insert": [], "extract": []}
     phases = {"knn_seconds": [], "hubness_seconds": [], "bridge_seconds": [], "total_seconds": []}
     batch_sizes, failed, msts = [], 0, 0
+    by_size = {}  # extract latency by the number of mpts values asked for
     for kind, rid in ids:
         st = sess.wait(rid)
         if st["state"] != chip.REQ_DONE:
@@ -231,6 +232,7 @@ def serving_run(chip, datagen, k, seed_n, batches, insert_rate, extract_rate, bu
         if kind == "extract":
             batch_sizes.append(st["batch_requests"])
             msts += st["num_msts"]
+            by_size.setdefault(st["num_msts"], []).append(st["latency_ms"])
         sess.release(rid)
     wall = time.perf_counter() - start
     info = sess.graph_info()
@@ -246,6 +248,9 @@ def serving_run(chip, datagen, k, seed_n, batches, insert_rate, extract_rate, bu
         insert_p50_ms=percentile(lat["insert"], 50), insert_p99_ms=percentile(lat["insert"], 99),
         extract_p50_ms=percentile(lat["extract"], 50),
         extract_p99_ms=percentile(lat["extract"], 99),
+        extract_by_mpts_count={str(c): dict(requests=len(v), p50_ms=percentile(v, 50),
+                                            p99_ms=percentile(v, 99))
+                               for c, v in sorted(by_size.items())},
         extract_msts=msts, extracts_per_sweep_mean=float(np.mean(batch_sizes)) if batch_sizes else None,
         insert_run_ms_mean=float(np.mean(run_ms["insert"])) if run_ms["insert"] else None,
         extract_run_ms_mean=float(np.mean(run_ms["extract"])) if run_ms["extract"] else None,

@@ -217,6 +217,7 @@ void build_bridges(Graph& g, const coresg_hip_build_params& p, coresg_hip_build_
         knn_edges = count_knn_edges(g);
         const std::size_t budget = antihub_budget(p, g.dim, knn_edges, g.n);
         g.selected = select_antihubs(g, budget, p.min_per_component);
+        if (g.yield_hook) g.yield_hook();
         const auto t1 = Clock::now();
         // coresg.cpp:148-166: the anti-hub selection is made in every mode
         // (the hubness index is refreshed regardless); the bridges follow the

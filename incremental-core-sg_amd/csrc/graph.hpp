@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <initializer_list>
 #include <map>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -249,6 +250,10 @@ struct Graph {
     int device = 0;
     hipStream_t stream = nullptr;
     std::uint32_t n = 0, dim = 0, stride = 0, k = 0;
+    // Called by an insert between its phases (search, scan, merge, each
+    // refinement round, hubness, bridges); a serving session sets it to let
+    // an extract sweep in flight run first. Empty: no pauses.
+    std::function<void()> yield_hook;
 
     // Points and k-NN lists (reference SoA layout, metric units).
     DBuf<float> pts;

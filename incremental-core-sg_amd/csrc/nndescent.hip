@@ -2146,6 +2146,7 @@ DescentResult run_descent(Graph* gp, const float* pts, std::uint32_t n, std::uin
     bool converged = false;
     for (std::uint32_t round = 0; round < iters; ++round) {
         last = round;
+        if (gp && gp->yield_hook) gp->yield_hook();
         enqueue_round(round);
         CSG_CHECK(hipEventRecord(b.round_done.e[round & 1u], s));
         if (plain) {
@@ -3102,6 +3103,7 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
                           kp, p.seed ^ (std::uint64_t(n0) << 20), evals};
             const unsigned sblocks = static_cast<unsigned>(
                 std::min<std::size_t>((count + kSearchWaves - 1) / kSearchWaves, 16384));
+            if (g.yield_hook) g.yield_hook();
             ScopedKernelTimer t(g, "insert_search");
             insert_search_kernel<<<sblocks, 64 * kSearchWaves, 0, s>>>(sa);
             CSG_LAUNCH_CHECK();
@@ -3125,6 +3127,7 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
                                           static_cast<int>(lds)));
         const unsigned blocks = static_cast<unsigned>(
             std::min<std::size_t>(qblocks, 65535 / chunks) * chunks);
+        if (g.yield_hook) g.yield_hook();
         {
             ScopedKernelTimer t(g, "insert_scan");
             hipLaunchKernelGGL(fn, dim3(blocks), dim3(kInsThreads), lds, s, ia);
@@ -3148,6 +3151,7 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
             }
         }
         if (n0) {
+            if (g.yield_hook) g.yield_hook();
             const unsigned merge_blocks = static_cast<unsigned>(std::min<std::size_t>(n0, kMergeBlocks));
             pick_merge(k)<<<merge_blocks, 64, 0, s>>>(n0, n1, k, g.ids.p, g.dist.p, g.flags.p,
                                                      g.count.p, b.cand.p, b.fill.p, cap,
@@ -3183,6 +3187,7 @@ void insert_points(Graph& g, const float* rows, std::uint32_t count, int where, 
     }
     CSG_CHECK(hipStreamSynchronize(s));
     const double knn_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (g.yield_hook) g.yield_hook();
     build_bridges(g, p, st);
     if (st) {
         st->knn_seconds = knn_s;

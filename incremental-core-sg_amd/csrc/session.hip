@@ -47,6 +47,7 @@ using namespace coresg_hip;
 namespace {
 
 constexpr std::uint32_t kBatchSlots = 32;  // mpts values per fused sweep
+constexpr double kYieldMs = 15.0;           // longest pause of an insert for extracts
 
 using Clock = std::chrono::steady_clock;
 
@@ -128,6 +129,10 @@ struct coresg_hip_session {
     std::uint64_t next_id = 1, version = 0;
     std::size_t snapshots_alive = 0, snapshot_bytes_alive = 0;
     bool stop = false, busy_insert = false, busy_extract = false;
+    // extract sweeps on the device now: the insert worker pauses between its
+    // phases while one runs (at most kYieldMs per pause), so an extract
+    // shares the device only with the insert work already queued
+    std::atomic<int> extracts_running{0};
     std::thread insert_worker, extract_worker;
 
     void run_inserts();
@@ -153,6 +158,7 @@ Snapshot::~Snapshot() {
 // new extraction-only graph, with copies of the k-NN distances and counts.
 std::shared_ptr<Snapshot> coresg_hip_session::make_snapshot(std::uint64_t ver) {
     Graph& w = graph->g;
+    if (w.yield_hook) w.yield_hook();
     prepare_edges(w);
     std::unique_ptr<coresg_hip_graph> h(new coresg_hip_graph());
     Graph& g = h->g;
@@ -330,11 +336,13 @@ void coresg_hip_session::run_extracts() {
             }
         }
         std::string err;
+        extracts_running.fetch_add(1);
         try {
             extract_batch(batch, *snap);
         } catch (const std::exception& e) {
             err = e.what();
         }
+        extracts_running.fetch_sub(1);
         {
             std::lock_guard<std::mutex> lk(mu);
             const auto t = Clock::now();
@@ -361,6 +369,12 @@ coresg_hip_session* start_session(coresg_hip_graph* g, const coresg_hip_build_pa
     s->params = *p;
     s->hbm_budget = budget;
     s->published = s->make_snapshot(0);
+    coresg_hip_session* self = s.get();
+    g->g.yield_hook = [self] {
+        const auto t0 = Clock::now();
+        while (self->extracts_running.load() > 0 && ms_between(t0, Clock::now()) < kYieldMs)
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
+    };
     coresg_hip_session* raw = s.release();
     raw->insert_worker = std::thread([raw] { raw->run_inserts(); });
     raw->extract_worker = std::thread([raw] { raw->run_extracts(); });
@@ -555,6 +569,7 @@ void coresg_hip_session_destroy(coresg_hip_session* s) {
     if (s->insert_worker.joinable()) s->insert_worker.join();
     if (s->extract_worker.joinable()) s->extract_worker.join();
     s->published.reset();
+    s->graph->g.yield_hook = nullptr;
     coresg_hip_graph_destroy(s->graph);
     delete s;
 }
