@@ -69,15 +69,19 @@ struct DBuf {
         p = nullptr;
         n = 0;
     }
-    // Ensure capacity for `count` elements (contents are not preserved).
+    // Ensure capacity for `count` elements (contents are not preserved). A
+    // buffer that has to grow takes a quarter more than asked, so buffers
+    // sized by a growing point count (a stream of inserts) are reallocated
+    // once per quarter of growth instead of on every insert.
     void reserve(std::size_t count) {
         if (count <= n && p) return;
+        const std::size_t want = p ? std::max(count, n + n / 4) : count;
         release();
-        const std::size_t bytes = (count ? count : 1) * sizeof(T);
+        const std::size_t bytes = (want ? want : 1) * sizeof(T);
         budget_charge(bytes);
         std::lock_guard<std::mutex> lk(device_api_mutex());
         CSG_CHECK(hipMalloc(reinterpret_cast<void**>(&p), bytes));
-        n = count;
+        n = want;
     }
     // Take over another buffer's allocation (this one's is freed).
     void adopt(DBuf& o) {
