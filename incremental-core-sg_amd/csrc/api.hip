@@ -321,7 +321,8 @@ size_t coresg_hip_graph_device_bytes(const coresg_hip_graph* h) {
     return g.pts.bytes() + g.ids.bytes() + g.dist.bytes() + g.flags.bytes() + g.count.bytes() +
            g.eu.bytes() + g.ev.bytes() + g.ew.bytes() + g.eoff.bytes() + g.core.bytes() +
            g.comp.bytes() + g.parent.bytes() + g.newroot.bytes() + g.minv.bytes() +
-           g.labels_tmp.bytes() + g.best.bytes() + g.mst_keys.bytes() + g.active.bytes() + g.adj_off.bytes() + g.adj.bytes() +
+           g.labels_tmp.bytes() + g.best.bytes() + g.mst_keys.bytes() + g.active.bytes() +
+           g.sweep_vkey.bytes() + g.sweep_vfar.bytes() + g.adj_off.bytes() + g.adj.bytes() +
            g.mst_flag.bytes() + g.scan_tmp.bytes() + g.out_pos.bytes() + g.slot_part.bytes() + g.slot_out.bytes() +
            g.slot_mpts.bytes() + g.label_buf.bytes() + g.out_u.bytes() + g.out_v.bytes() +
            g.out_w.bytes() +

@@ -295,6 +295,10 @@ struct Graph {
     DBuf<std::uint32_t> comp, parent, newroot, minv, labels_tmp;
     DBuf<std::uint64_t> best, mst_keys;
     DBuf<std::uint8_t> active;  // Boruvka: vertex may still have an outgoing edge
+    // Boruvka, sweeps of few slots: each vertex's best outgoing edge at its
+    // last scan (key, far endpoint)
+    DBuf<std::uint64_t> sweep_vkey;
+    DBuf<std::uint32_t> sweep_vfar;
     DBuf<std::uint8_t> mst_flag;
     DBuf<std::uint32_t> counters;     // small device counters
     DBuf<std::uint64_t> counters64;

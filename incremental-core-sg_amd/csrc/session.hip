@@ -94,6 +94,7 @@ std::size_t version_bytes(std::size_t n, std::size_t E, std::uint32_t stride, st
         + (4 * k + 4)                 // snapshot: distances, counts
         + 2 * 4                       // snapshot: offsets
         + 32 * (4 + 4 + 4 + 4 + 8 + 8 + 1)  // sweep: core, comp, parent, newroot, best, key, active
+        + (8 + 4)                     // one-slot sweeps: each vertex's best edge (key, far end)
         + 4 * 4;                      // labels and scan scratch
     const std::size_t per_edge = 12    // (u, v, w) of the sorted list
                                  + 16  // scattered records while laying it out

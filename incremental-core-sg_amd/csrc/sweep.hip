@@ -56,6 +56,12 @@ constexpr std::uint32_t kGraphRounds = 4;  // Boruvka rounds per replay of the c
 // neighbour's core distance and component behind it); 6 and 8 measured no
 // faster at n = 1M (fewer waves per SIMD)
 constexpr int kSweepInFlight = 4;
+// Largest slot count whose sweeps keep each vertex's best outgoing edge. At
+// n = 1M it pays for one slot over many rounds (mpts 1: 5.87 -> 4.29 ms; mpts
+// 16, four rounds, unchanged); with 2 to 32 slots a vertex skips its scan
+// only when every slot's edge still leaves, and the checks cost more than
+// they save (1-6 % slower).
+constexpr std::uint32_t kCacheSlots = 1;
 constexpr std::uint64_t kOver = 1ull << 63;  // saturated checked sum
 constexpr std::uint64_t kExpInf = 1ull << 32;
 
@@ -76,6 +82,10 @@ struct Sweep {
     std::uint8_t* active;
     std::uint64_t* part;  // kReduceBlocks*B per-block partials
     std::uint32_t* err;
+    // per (vertex, slot): the key of the vertex's best outgoing edge at its
+    // last scan and that edge's far endpoint (nullptr: no cache)
+    std::uint64_t* vkey;
+    std::uint32_t* vfar;
 };
 
 // Elements i = v*B + b are visited with a grid stride that is a multiple of B,
@@ -183,6 +193,14 @@ __device__ __forceinline__ std::uint64_t entry_key(std::uint32_t raw, float w, f
     return (raw & kSecondEnd) ? mrd_key(w, cq, cx, rank) : mrd_key(w, cx, cq, rank);
 }
 
+// The far endpoint, seen from v, of the edge a key names (its rank).
+__device__ __forceinline__ std::uint32_t key_far(const Sweep& s, std::uint64_t key,
+                                                 std::uint32_t v) {
+    const std::uint32_t e = static_cast<std::uint32_t>(key);
+    const std::uint32_t a = s.eu[e];
+    return a == v ? s.ev[e] : a;
+}
+
 // One Boruvka round. A group of W lanes serves one vertex: lane sub handles
 // slot sub % Bp and every (W/Bp)-th adjacency entry from sub / Bp, so each
 // entry's neighbour, rank and weight are read once for all slots. The first
@@ -191,6 +209,11 @@ __device__ __forceinline__ std::uint64_t entry_key(std::uint32_t raw, float w, f
 // component's best with a read-filtered 64-bit atomic-min. A vertex with no
 // edge leaving its component stays inside for good (components only merge)
 // and is deactivated.
+// With the cache (vkey/vfar, few slots), a vertex whose best outgoing edge at
+// its last scan still leaves its component skips the scan: its outgoing
+// edges are a subset of those it had (components only merge), so that edge
+// is still its minimum. Only when it has turned internal is the vertex's
+// adjacency read again.
 template <std::uint32_t W, bool FIRST>
 __global__ void __launch_bounds__(kThreads) sweep_round_kernel(Sweep s) {
     const std::uint32_t lane = __lane_id();
@@ -209,15 +232,22 @@ __global__ void __launch_bounds__(kThreads) sweep_round_kernel(Sweep s) {
         const bool act = slot && (FIRST || s.active[i] != 0);
         std::uint64_t key_min = kNone64;
         std::uint32_t cv = v;
-        if (__ballot(act) & gmask) {
-            float cx = 0.0f;
-            if (act) {
-                cx = s.core[i];
-                if (!FIRST) cv = s.comp[i];
+        bool need = act;  // the vertex's adjacency is read for this slot
+        std::uint64_t cached = kNone64;
+        if (!FIRST && act) {
+            cv = s.comp[i];
+            if (s.vkey) {
+                cached = s.vkey[i];
+                if (cached != kNone64 && s.comp[std::size_t(s.vfar[i]) * s.B + b] != cv)
+                    need = false;
             }
+        }
+        if (__ballot(need) & gmask) {
+            float cx = 0.0f;
+            if (need) cx = s.core[i];
             const std::uint32_t e1 = s.off[v + 1];
             std::uint32_t j = s.off[v] + ph;
-            if (act) {
+            if (need) {
                 // kSweepInFlight entries in flight per lane
                 constexpr int F = kSweepInFlight;
                 for (; j + (F - 1) * P < e1; j += F * P) {
@@ -253,6 +283,11 @@ __global__ void __launch_bounds__(kThreads) sweep_round_kernel(Sweep s) {
             key_min = x < key_min ? x : key_min;
         }
         if (ph == 0 && act) {
+            if (s.vkey && need) {
+                s.vkey[i] = key_min;
+                if (key_min != kNone64) s.vfar[i] = key_far(s, key_min, v);
+            }
+            if (!need) key_min = cached;
             if (FIRST) {
                 s.best[i] = key_min;
                 s.active[i] = key_min != kNone64;
@@ -689,6 +724,14 @@ void run_batch(Graph& g, const std::uint32_t* mpts, std::uint32_t B, coresg_hip_
     sw.active = g.active.p;
     sw.part = g.slot_part.p;
     sw.err = g.counters.p;
+    // the best-edge cache pays where a vertex's check (one random component
+    // read per slot) costs less than its shared adjacency scan: few slots
+    if (B <= kCacheSlots) {
+        g.sweep_vkey.reserve(nb);
+        g.sweep_vfar.reserve(nb);
+        sw.vkey = g.sweep_vkey.p;
+        sw.vfar = g.sweep_vfar.p;
+    }
 
     const unsigned eblk = capped_blocks(nb, kThreads);
     // launch shape of the round kernel: the autotuned one for this slot
