@@ -56,6 +56,7 @@ constexpr std::uint32_t kRevTargets = 1u << kRevShift;
 constexpr std::uint32_t kRevMaxBuckets = 8192;
 constexpr unsigned kRevBlocks = 2048;
 constexpr unsigned kRevThreads = 1024;
+constexpr std::uint32_t kRevBatch = 4;  // pairs in flight per thread of the bucket pass
 constexpr std::uint32_t kJoinRegStride = 64;  // widest row a join thread keeps in registers
 // matrix-core filter join: threads per node, and the members whose rows a
 // first pass stages per node for rows of at least 64 floats (nodes with more
@@ -313,22 +314,50 @@ __global__ void __launch_bounds__(256)
     for (std::uint32_t x = threadIdx.x; x < nb; x += blockDim.x)
         s_cur[x] = off[std::size_t(x) * gridDim.x + blockIdx.x];
     __syncthreads();
-    for (std::uint32_t v0 = wave * P; v0 < n; v0 += waves * P) {
-        const std::uint32_t v = v0 + seg;
-        if (v >= n) continue;
-        const std::uint32_t nc = new_cnt[v], oc = old_cnt[v];
-        for (std::uint32_t e = sub; e < nc; e += SEG) {
-            const std::uint32_t id = new_list[std::size_t(v) * smp + e];
-            if (id < n) {
-                const std::uint32_t pos = atomicAdd(&s_cur[id >> kRevShift], 1u);
-                pairs[pos] = (static_cast<unsigned long long>(id) << 32) | v;
+    // two of the wavefront's nodes (the sampling pass's v and v + waves P) per
+    // iteration, their counts and then all their entries loaded before any is
+    // placed: two memory waits per two nodes instead of per node
+    constexpr std::uint32_t RN = (kMaxSample + SEG - 1) / SEG;
+    constexpr std::uint32_t RO = (3 * kMaxSample + SEG - 1) / SEG;
+    for (std::uint32_t v0 = wave * P; v0 < n; v0 += 2 * waves * P) {
+        const std::uint32_t vv[2] = {v0 + seg, v0 + waves * P + seg};
+        std::uint32_t nc[2], oc[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            nc[h] = vv[h] < n ? new_cnt[vv[h]] : 0u;
+            oc[h] = vv[h] < n ? old_cnt[vv[h]] : 0u;
+        }
+        std::uint32_t ne[2][RN], oe[2][RO];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (std::uint32_t r = 0; r < RN; ++r) {
+                const std::uint32_t e = sub + r * SEG;
+                ne[h][r] = e < nc[h] ? new_list[std::size_t(vv[h]) * smp + e] : kNone;
+            }
+#pragma unroll
+            for (std::uint32_t r = 0; r < RO; ++r) {
+                const std::uint32_t e = sub + r * SEG;
+                oe[h][r] = e < oc[h] ? old_list[std::size_t(vv[h]) * 3 * smp + e] : kNone;
             }
         }
-        for (std::uint32_t e = sub; e < oc; e += SEG) {
-            const std::uint32_t id = old_list[std::size_t(v) * 3 * smp + e];
-            if (id < n) {
-                const std::uint32_t pos = atomicAdd(&s_cur[id >> kRevShift], 1u);
-                pairs[pos] = (static_cast<unsigned long long>(id) << 32) | v | 0x80000000u;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (std::uint32_t r = 0; r < RN; ++r) {
+                const std::uint32_t id = ne[h][r];
+                if (id < n) {
+                    const std::uint32_t pos = atomicAdd(&s_cur[id >> kRevShift], 1u);
+                    pairs[pos] = (static_cast<unsigned long long>(id) << 32) | vv[h];
+                }
+            }
+#pragma unroll
+            for (std::uint32_t r = 0; r < RO; ++r) {
+                const std::uint32_t id = oe[h][r];
+                if (id < n) {
+                    const std::uint32_t pos = atomicAdd(&s_cur[id >> kRevShift], 1u);
+                    pairs[pos] = (static_cast<unsigned long long>(id) << 32) | vv[h] | 0x80000000u;
+                }
             }
         }
     }
@@ -352,13 +381,25 @@ __global__ void __launch_bounds__(kRevThreads)
     __syncthreads();
     const std::uint32_t start = off[std::size_t(b) * nblk];
     const std::uint32_t end = b + 1 < gridDim.x ? off[std::size_t(b + 1) * nblk] : *total;
-    for (std::uint32_t e = start + threadIdx.x; e < end; e += kRevThreads) {
-        const unsigned long long pr = pairs[e];
-        const std::uint32_t id = static_cast<std::uint32_t>(pr >> 32);
-        const std::uint32_t src = static_cast<std::uint32_t>(pr) & 0x7FFFFFFFu;
-        const bool is_old = (static_cast<std::uint32_t>(pr) & 0x80000000u) != 0;
-        const std::uint32_t pos = atomicAdd(is_old ? &s_co[id - lo] : &s_cn[id - lo], 1u);
-        if (pos < smp) (is_old ? rev_old : rev_new)[std::size_t(id) * smp + pos] = src;
+    // kRevBatch pairs loaded per thread before any is placed: one memory wait
+    // per batch instead of one per pair
+    for (std::uint32_t e0 = start + threadIdx.x; e0 < end; e0 += kRevBatch * kRevThreads) {
+        unsigned long long pb[kRevBatch];
+#pragma unroll
+        for (std::uint32_t u = 0; u < kRevBatch; ++u) {
+            const std::uint32_t e = e0 + u * kRevThreads;
+            pb[u] = e < end ? pairs[e] : ~0ull;
+        }
+#pragma unroll
+        for (std::uint32_t u = 0; u < kRevBatch; ++u) {
+            if (e0 + u * kRevThreads >= end) break;
+            const unsigned long long pr = pb[u];
+            const std::uint32_t id = static_cast<std::uint32_t>(pr >> 32);
+            const std::uint32_t src = static_cast<std::uint32_t>(pr) & 0x7FFFFFFFu;
+            const bool is_old = (static_cast<std::uint32_t>(pr) & 0x80000000u) != 0;
+            const std::uint32_t pos = atomicAdd(is_old ? &s_co[id - lo] : &s_cn[id - lo], 1u);
+            if (pos < smp) (is_old ? rev_old : rev_new)[std::size_t(id) * smp + pos] = src;
+        }
     }
     __syncthreads();
     for (std::uint32_t t = threadIdx.x; t < kRevTargets && lo + t < n; t += kRevThreads) {
@@ -426,8 +467,11 @@ __global__ void __launch_bounds__(256)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // entry e = sub + r SEG: its id, and its slot in the table
+        // entry e = sub + r SEG: its id, its threshold and its slot in the
+        // table; all ids are loaded, then all thresholds, before the table's
+        // LDS atomics, so each kind of load waits for memory once
         std::uint32_t id[R], slot[R];
+        float th[R];
 #pragma unroll
         for (std::uint32_t r = 0; r < R; ++r) {
             const std::uint32_t e = sub + r * SEG;
@@ -439,6 +483,12 @@ __global__ void __launch_bounds__(256)
                 else x = rev_old[std::size_t(v) * smp + (e - na0 - no)];
             }
             id[r] = x < n ? x : kNone;
+        }
+#pragma unroll
+        for (std::uint32_t r = 0; r < R; ++r) th[r] = id[r] != kNone ? thr_node[id[r]] : 0.0f;
+#pragma unroll
+        for (std::uint32_t r = 0; r < R; ++r) {
+            const std::uint32_t e = sub + r * SEG;
             slot[r] = 0;
             if (id[r] != kNone) {
                 std::uint32_t h = (id[r] * 0x9E3779B1u) >> 24;
@@ -464,7 +514,7 @@ __global__ void __launch_bounds__(256)
             const std::uint32_t pos = ns + __popcll(m & lt);
             if (keep) {
                 out_ids[std::size_t(v) * S + pos] = id[r];
-                out_thr[std::size_t(v) * S + pos] = thr_node[id[r]];
+                out_thr[std::size_t(v) * S + pos] = th[r];
             }
             na += __popcll(__ballot(keep && e < na0) & segmask);
             ns += __popcll(m);
