@@ -238,14 +238,32 @@ __global__ void __launch_bounds__(256)
         for (std::uint32_t x = threadIdx.x; x < nb; x += blockDim.x) s_hist[x] = 0;
         __syncthreads();
     }
-    for (std::uint32_t v0 = wave * P; v0 < n; v0 += waves * P) {
-        const std::uint32_t v = v0 + seg;
+    // two of the wavefront's nodes (v and v + waves P, the nodes the scatter
+    // pass walks in the same block) per iteration: both nodes' counts and
+    // list slots are loaded before either is sampled (a node's k slots are
+    // read whatever its count, so these loads do not wait for it)
+    for (std::uint32_t v00 = wave * P; v00 < n; v00 += 2 * waves * P) {
+      std::uint32_t c2[2], id2[2], fl2[2];
+      float last2[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const std::uint32_t v = v00 + h * waves * P + seg;
         const bool live = v < n;
         const std::size_t base = std::size_t(v) * k;
-        const std::uint32_t c = live ? count[v] : 0u;
+        c2[h] = live ? count[v] : 0u;
+        id2[h] = live && sub < k ? ids[base + sub] : kNone;
+        fl2[h] = live && sub < k ? flags[base + sub] : 0u;
+        last2[h] = live && sub == 0 ? dist[base + k - 1] : 0.0f;
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const std::uint32_t v = v00 + h * waves * P + seg;
+        const bool live = v < n;
+        const std::size_t base = std::size_t(v) * k;
+        const std::uint32_t c = c2[h];
         const bool has = sub < c;
-        const std::uint32_t id = has ? ids[base + sub] : kNone;
-        const bool isnew = has && flags[base + sub] != 0;
+        const std::uint32_t id = has ? id2[h] : kNone;
+        const bool isnew = has && fl2[h] != 0;
         const bool isold = has && !isnew;
         const std::uint64_t mnew = __ballot(isnew), mold = __ballot(isold);
         const std::uint32_t rn = __popcll(mnew & lt), ro = __popcll(mold & lt);
@@ -276,8 +294,9 @@ __global__ void __launch_bounds__(256)
             old_cnt[v] = to < max_old ? to : max_old;
             // the node's join threshold D[k-1]^2 for this round (to_rank of
             // its current k-th distance, nndescent.cpp:41-42)
-            thr_node[v] = metric_to_rank(dist[base + k - 1], cosine);
+            thr_node[v] = metric_to_rank(last2[h], cosine);
         }
+      }
     }
     if (BUCKETS) {
         __syncthreads();
