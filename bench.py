@@ -213,13 +213,14 @@ def serving_run(chip, datagen, k, seed_n, batches, insert_rate, extract_rate, bu
         if delay > 0:
             time.sleep(delay)
         rid = sess.submit_insert(inserts[i]) if kind == "insert" else sess.submit_extract(ext_sets[i])
-        ids.append((kind, rid))
+        ids.append((kind, rid, i))
     lat = {"insert": [], "extract": []}
     run_ms = {"insert": [], "extract": []}
     phases = {"knn_seconds": [], "hubness_seconds": [], "bridge_seconds": [], "total_seconds": []}
     batch_sizes, failed, msts = [], 0, 0
     by_size = {}  # extract latency by the number of mpts values asked for
-    for kind, rid in ids:
+    ins_sized = []  # (batch points, latency ms, run ms) per insert
+    for kind, rid, i in ids:
         st = sess.wait(rid)
         if st["state"] != chip.REQ_DONE:
             failed += 1
@@ -229,6 +230,7 @@ def serving_run(chip, datagen, k, seed_n, batches, insert_rate, extract_rate, bu
         if kind == "insert":
             for key in phases:
                 phases[key].append(st["insert_stats"][key])
+            ins_sized.append((len(inserts[i]), st["latency_ms"], st["run_ms"]))
         if kind == "extract":
             batch_sizes.append(st["batch_requests"])
             msts += st["num_msts"]
@@ -253,6 +255,12 @@ def serving_run(chip, datagen, k, seed_n, batches, insert_rate, extract_rate, bu
                                for c, v in sorted(by_size.items())},
         extract_msts=msts, extracts_per_sweep_mean=float(np.mean(batch_sizes)) if batch_sizes else None,
         insert_run_ms_mean=float(np.mean(run_ms["insert"])) if run_ms["insert"] else None,
+        insert_by_batch_points={
+            f"[{lo},{hi})": dict(requests=len(v), latency_p50_ms=percentile([e[1] for e in v], 50),
+                               latency_max_ms=max(e[1] for e in v),
+                               run_mean_ms=float(np.mean([e[2] for e in v])))
+            for lo, hi in ((1000, 5000), (5000, 20000), (20000, 50001))
+            for v in [[e for e in ins_sized if lo <= e[0] < hi]] if v},
         extract_run_ms_mean=float(np.mean(run_ms["extract"])) if run_ms["extract"] else None,
         insert_phase_ms_mean={k.replace("_seconds", ""): float(np.mean(v)) * 1e3
                               for k, v in phases.items() if v},

@@ -2294,6 +2294,25 @@ __device__ __forceinline__ void list_insert(float (&td)[KP], std::uint32_t (&ti)
     }
 }
 
+// After a graph search (its KP keys in the last of a new point's slots,
+// sorted) the final list is the best k over all slots, so a column whose key
+// is not below the search's k-th key cannot enter it: that key (bd, bi) bounds
+// a scan's list from the start, and its square (loosened like the list's)
+// the squared prefilter thr2. Without a search slot or a k-th key: no bound.
+template <std::uint32_t KP>
+__device__ __forceinline__ void search_bound(const InsertArgs& a, std::uint32_t p, bool has,
+                                             float& bd, std::uint32_t& bi, float& thr2) {
+    bd = __builtin_inff();
+    bi = kNone;
+    if (!has || !a.partial || a.slots <= a.chunks) return;
+    const unsigned long long bk =
+        a.partial[(std::size_t(p - a.n0) * a.slots + a.slots - 1) * KP + a.k - 1];
+    if (bk == ~0ull) return;
+    bd = __uint_as_float(static_cast<std::uint32_t>(bk >> 32));
+    bi = static_cast<std::uint32_t>(bk);
+    thr2 = bd * bd * (1.0f + 0x1p-20f);
+}
+
 // One block per 64 new points. Every step stages kInsTile points in LDS; a
 // lane holds its new point's row in registers and computes the exact squared
 // distance (reference summation order) to every 4th staged point, keeping its
@@ -2336,6 +2355,9 @@ __global__ void __launch_bounds__(kInsThreads) insert_scan_kernel(InsertArgs a) 
         // squared prefilter for td[KP-1], loosened by 2^-20 so that no pair
         // whose rounded sqrt ties or beats it is skipped
         float thr2 = __builtin_inff();
+        float bd;
+        std::uint32_t bi;
+        search_bound<KP>(a, p, has, bd, bi, thr2);
         for (std::uint32_t c0 = col0; c0 < col1; c0 += kInsTile) {
             __syncthreads();  // the previous tile is consumed
             const std::uint32_t cend = min(kInsTile, col1 - c0);
@@ -2363,9 +2385,9 @@ __global__ void __launch_bounds__(kInsThreads) insert_scan_kernel(InsertArgs a) 
                 if (!has || j == p) continue;
                 if (d2 <= thr2) {
                     const float d = sqrt_exact(d2);
-                    if (key_less(d, j, td[KP - 1], ti[KP - 1])) {
+                    if (key_less(d, j, td[KP - 1], ti[KP - 1]) && key_less(d, j, bd, bi)) {
                         list_insert<KP>(td, ti, d, j);
-                        thr2 = td[KP - 1] * td[KP - 1] * (1.0f + 0x1p-20f);
+                        thr2 = fminf(thr2, td[KP - 1] * td[KP - 1] * (1.0f + 0x1p-20f));
                     }
                 }
                 if (d2 < s_thr[c]) {
@@ -2539,6 +2561,9 @@ __global__ void __launch_bounds__(kInsThreads) insert_scan_gram_kernel(InsertArg
             ti[i] = kNone;
         }
         float thr2 = __builtin_inff();
+        float bd;
+        std::uint32_t bi;
+        search_bound<KP>(a, p, has, bd, bi, thr2);
         for (std::uint32_t c0 = col0; c0 < col1; c0 += kGramTile) {
             __syncthreads();  // the previous tile is consumed
             const std::uint32_t cend = min(kGramTile, col1 - c0);
@@ -2633,9 +2658,9 @@ __global__ void __launch_bounds__(kInsThreads) insert_scan_gram_kernel(InsertArg
                 const float d2 = l2sqr_reg<STRIDE>(row, s_tile + c * STRIDE);
                 if (d2 <= thr2) {
                     const float d = sqrt_exact(d2);
-                    if (key_less(d, j, td[KP - 1], ti[KP - 1])) {
+                    if (key_less(d, j, td[KP - 1], ti[KP - 1]) && key_less(d, j, bd, bi)) {
                         list_insert<KP>(td, ti, d, j);
-                        thr2 = td[KP - 1] * td[KP - 1] * (1.0f + 0x1p-20f);
+                        thr2 = fminf(thr2, td[KP - 1] * td[KP - 1] * (1.0f + 0x1p-20f));
                     }
                 }
                 if (d2 < thr_c) {

@@ -144,13 +144,15 @@ def test_insert_stream_grows_and_matches_oracle():
     g.close()
 
 
-def test_insert_graph_search_lists_valid():
+@pytest.mark.parametrize("gram", [1, 2])
+def test_insert_graph_search_lists_valid(gram):
     """insert_search = 1 (beam search over the existing lists instead of the
-    scan): every stored entry is its pair's exact distance, lists are sorted
-    and distinct, the batch's own pairs are exact, the new points' lists reach
-    the exact lists closely, and everything downstream equals the oracle on the
-    resulting lists."""
-    print("[gpu] insert graph search: 15k + 5k x 16", flush=True)
+    scan; the batch's own pairs by the exact scan, gram 1, or the FP16
+    Gram-filter scan, gram 2): every stored entry is its pair's exact
+    distance, lists are sorted and distinct, the batch's own pairs are exact,
+    the new points' lists reach the exact lists closely, and everything
+    downstream equals the oracle on the resulting lists."""
+    print(f"[gpu] insert graph search: 15k + 5k x 16, gram={gram}", flush=True)
     x, k = datagen.workload("mixture-20k-d16")
     n0, n = 15000, 20000
     xp = oracle.pad(x)
@@ -158,7 +160,7 @@ def test_insert_graph_search_lists_valid():
     g.set_points(x[:n0])
     g.build(chip.build_params())
     ids0, dist0, flags0, count0 = g.get_knn()
-    st = g.insert(x[n0:], chip.build_params(insert_search=1))
+    st = g.insert(x[n0:], chip.build_params(insert_search=1, gram=gram))
     assert g.n == n
     ids, dist, flags, count = g.get_knn()
     assert oracle.lib().orc_validate(ids, dist, count, n, k) == 1
@@ -177,6 +179,24 @@ def test_insert_graph_search_lists_valid():
     # measured on this workload: 0.9935 and 0.9910 (the refinement rounds
     # after the search close most of what the beam misses)
     assert rec_new >= 0.98 and rec_old >= 0.98
+    # the batch's own pairs are scanned exactly (pruned only by the search's
+    # k-th key, which no dropped pair can beat): a batch point clearly closer
+    # to a new point than that point's final k-th neighbour is in its list
+    # (refinement only lowers the k-th, so this holds after it too)
+    xb = x[n0:].astype(np.float64)
+    nb = xb.shape[0]
+    kth = dist.reshape(n, k)[n0:, k - 1].astype(np.float64)
+    assert np.all(count[n0:] == k)
+    lists = ids.reshape(n, k)[n0:]
+    for r0 in range(0, nb, 1000):
+        blk = xb[r0:r0 + 1000]
+        d = np.sqrt(np.maximum(
+            (blk * blk).sum(1)[:, None] + (xb * xb).sum(1)[None, :] - 2.0 * blk @ xb.T, 0.0))
+        for r in range(blk.shape[0]):
+            q = r0 + r
+            near = np.nonzero(d[r] < kth[q] * (1.0 - 1e-4))[0]
+            near = near[near != q] + n0
+            assert np.isin(near, lists[q]).all(), f"new point {n0 + q}: batch neighbour missing"
     check_downstream(g, x, k)
     g.close()
 
