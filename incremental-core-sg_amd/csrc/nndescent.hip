@@ -2320,8 +2320,13 @@ __device__ __forceinline__ void search_bound(const InsertArgs& a, std::uint32_t 
 // to the staged point's list when that one is an existing point and the
 // distance beats its current k-th. The four partial lists of a new point are
 // merged at the end (waves 2,3 -> 0,1 -> 0) and its first k entries written.
+// Rows of up to 24 floats are held to three waves per SIMD (168 VGPRs and 12
+// bytes of spilled state per lane at d = 24) instead of the two that 215
+// VGPRs allow: a 50k-point batch's scan of its own pairs at d = 24 took 9.4
+// -> 7.2 ms on MI355X; wider rows spill far more.
 template <std::uint32_t STRIDE, std::uint32_t KP>
-__global__ void __launch_bounds__(kInsThreads) insert_scan_kernel(InsertArgs a) {
+__global__ void __launch_bounds__(kInsThreads)
+    __attribute__((amdgpu_waves_per_eu(STRIDE <= 24 ? 3 : 1))) insert_scan_kernel(InsertArgs a) {
     extern __shared__ float lds[];
     float* s_tile = lds;                                 // kInsTile * STRIDE
     float* s_thr = lds + kInsTile * STRIDE;              // kInsTile
