@@ -81,9 +81,13 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
 // graph (points, k-NN lists, hubness state, ownership flags, the insert scan's
 // per-chunk lists, the edge list and adjacency while they are laid out) and
 // its extraction snapshot (distances and counts, edge list, adjacency, a
-// 32-slot sweep's per-vertex state and the labels/edge outputs). Per point /
-// per edge, in bytes; an upper bound of what the arrays take.
-std::size_t version_bytes(std::size_t n, std::size_t E, std::uint32_t stride, std::uint32_t k) {
+// 32-slot sweep's per-vertex state and the labels/edge outputs), and the
+// descent's buffers for the refinement rounds (proposal buffer of 512 keys,
+// sampled and reverse lists of smp entries each, gathered members and
+// thresholds, reverse-list pairs). Per point / per edge, in bytes; an upper
+// bound of what the arrays take.
+std::size_t version_bytes(std::size_t n, std::size_t E, std::uint32_t stride, std::uint32_t k,
+                          std::uint32_t smp) {
     const std::size_t per_point =
         4 * stride                    // points
         + 9 * k + 4                   // ids, dist, flags, count
@@ -95,7 +99,12 @@ std::size_t version_bytes(std::size_t n, std::size_t E, std::uint32_t stride, st
         + 2 * 4                       // snapshot: offsets
         + 32 * (4 + 4 + 4 + 4 + 8 + 8 + 1)  // sweep: core, comp, parent, newroot, best, key, active
         + (8 + 4)                     // one-slot sweeps: each vertex's best edge (key, far end)
-        + 4 * 4;                      // labels and scan scratch
+        + 4 * 4                       // labels and scan scratch
+        + 8 * 512 + 4                 // descent: proposal buffer, fill counts
+        + 4 * 6 * std::size_t(smp)    // new, old (3 smp), reverse new and old lists
+        + 8 * 6 * std::size_t(smp)    // gathered member ids and thresholds (6 smp)
+        + 8 * std::size_t(k)          // reverse-list (target, source) pairs
+        + 7 * 4;                      // list counts, join threshold
     const std::size_t per_edge = 12    // (u, v, w) of the sorted list
                                  + 16  // scattered records while laying it out
                                  + 24  // adjacency, both directions
@@ -209,7 +218,7 @@ void coresg_hip_session::insert_one(Request& r) {
                            double(g.n ? g.n : 1);
         const std::size_t n1 = std::size_t(g.n) + r.count;
         const std::size_t need = version_bytes(n1, std::size_t(epp * double(n1)) + 1, g.stride,
-                                               g.k);
+                                               g.k, params.sample ? params.sample : 16u);
         std::unique_lock<std::mutex> lk(mu);
         // older snapshots that in-flight extracts still hold free up as those
         // finish; the current one stays until the new version replaces it
